@@ -1,0 +1,237 @@
+"""ctypes bindings of libfelis_hip.so (the C-ABI declared in include/felis.h).
+
+The product path: every call goes to the gfx950 library. Loading fails loudly
+when the library is missing; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libfelis_hip.so"
+MAX_HARM = 8
+
+OK = 0
+ERR_ARG, ERR_HIP, ERR_GUARD, ERR_RANGE, ERR_NAN, ERR_UNSUPPORTED, ERR_STATE, ERR_IO = -1, -2, -3, -4, -5, -6, -7, -8
+PLAIN_FP64, MIXED_FP32 = 0, 1
+UX, UY, SCUX, SCUY, EZ = 0, 1, 2, 3, 4
+
+# every entry point include/felis.h declares
+EXPORTS = [
+    "felis_create", "felis_destroy", "felis_last_error", "felis_sync", "felis_get_tables",
+    "felis_upload_particles", "felis_download_particles", "felis_upload_field", "felis_download_field",
+    "felis_slice_energy", "felis_solve_field", "felis_transverse_push", "felis_push_theta_gamma",
+    "felis_drift_theta_gamma", "felis_periodic_particles", "felis_shift_field", "felis_update_wake",
+    "felis_set_wake", "felis_update_total_energy", "felis_update_bfactor", "felis_calculate_twiss",
+    "felis_run",
+]
+
+
+class FelisError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"felis error {code}: {msg}")
+        self.code = code
+
+
+class Desc(C.Structure):
+    _fields_ = [
+        ("dimension", C.c_int), ("mode", C.c_int), ("precision", C.c_int),
+        ("nx", C.c_int), ("ny", C.c_int), ("slice_n", C.c_int), ("n_harm", C.c_int),
+        ("harmony", C.c_int * MAX_HARM),
+        ("num_slice", C.c_int), ("n_species", C.c_int), ("nr", C.c_int), ("sc_l", C.c_int), ("sc_f", C.c_int),
+        ("bessel_n", C.c_int), ("abc_n", C.c_int), ("wake_on", C.c_int), ("pad0", C.c_int),
+        ("abc_sig", C.c_double), ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+        ("min_x", C.c_double), ("min_y", C.c_double), ("min_z", C.c_double), ("max_z", C.c_double),
+        ("ks", C.c_double), ("lambda0", C.c_double), ("gamR", C.c_double), ("dr", C.c_double),
+        ("gamma_ref", C.c_double),
+    ]
+
+
+class Lattice(C.Structure):
+    _fields_ = [
+        ("K0", C.c_double), ("ue", C.c_double), ("K0_alpha", C.c_double), ("ku", C.c_double),
+        ("g", C.c_double), ("lambdaU", C.c_double),
+        ("und_type", C.c_int), ("current_flag", C.c_int), ("drift_flag", C.c_int), ("pad", C.c_int),
+    ]
+
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+_lib = None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(f"{LIB_PATH} is not built: run python -m felis_amd.build")
+        L = C.CDLL(str(LIB_PATH))
+        L.felis_create.argtypes = [C.POINTER(Desc), C.POINTER(C.c_void_p)]
+        L.felis_destroy.argtypes = [C.c_void_p]
+        L.felis_last_error.argtypes = [C.c_void_p]
+        L.felis_last_error.restype = C.c_char_p
+        L.felis_sync.argtypes = [C.c_void_p]
+        L.felis_get_tables.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
+        L.felis_upload_particles.argtypes = [C.c_void_p, _i64p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.felis_download_particles.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.felis_upload_field.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.felis_download_field.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.felis_slice_energy.argtypes = [C.c_void_p, _dp]
+        L.felis_solve_field.argtypes = [C.c_void_p, C.POINTER(Lattice)]
+        L.felis_transverse_push.argtypes = [C.c_void_p, C.POINTER(Lattice)]
+        L.felis_push_theta_gamma.argtypes = [C.c_void_p, C.POINTER(Lattice), C.c_int]
+        L.felis_drift_theta_gamma.argtypes = [C.c_void_p, C.POINTER(Lattice), C.c_int]
+        L.felis_periodic_particles.argtypes = [C.c_void_p, C.c_int]
+        L.felis_shift_field.argtypes = [C.c_void_p, C.c_int]
+        L.felis_update_wake.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.felis_set_wake.argtypes = [C.c_void_p, _dp]
+        L.felis_update_total_energy.argtypes = [C.c_void_p, _dp, _dp]
+        L.felis_update_bfactor.argtypes = [C.c_void_p, _dp, _dp]
+        L.felis_calculate_twiss.argtypes = [C.c_void_p, _dp]
+        L.felis_run.argtypes = [C.c_void_p, C.POINTER(Lattice), C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "arrays must be C-contiguous"
+    if a.dtype == np.int64:
+        return a.ctypes.data_as(_i64p)
+    assert a.dtype in (np.float64, np.complex128), a.dtype
+    return a.ctypes.data_as(_dp)
+
+
+class Context:
+    """One device context (the reference Domain on the GPU)."""
+
+    def __init__(self, desc: Desc):
+        self.L = load()
+        self.desc = desc
+        h = C.c_void_p()
+        rc = self.L.felis_create(C.byref(desc), C.byref(h))
+        if rc != OK:
+            raise FelisError(rc, "felis_create failed")
+        self.h = h
+        d = desc
+        self.field_shape = (d.n_harm, d.slice_n + 2, d.ny, d.nx)
+        self.ez_shape = (d.sc_l, (d.slice_n + 2) * d.sc_f * d.nr)
+        self.np = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.felis_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc: int):
+        if rc != OK:
+            raise FelisError(rc, self.L.felis_last_error(self.h).decode())
+
+    # -- state -------------------------------------------------------------------
+    def tables(self):
+        d = self.desc
+        al = np.zeros(d.nx, np.complex128); alP = np.zeros(d.nx, np.complex128)
+        be = np.zeros(d.ny, np.complex128); beP = np.zeros(d.ny, np.complex128)
+        maxh = d.harmony[d.n_harm - 1]
+        bes = np.zeros((d.bessel_n, maxh // 2 + 2))
+        self._chk(self.L.felis_get_tables(self.h, _p(al), _p(alP), _p(be), _p(beP), _p(bes)))
+        return al, alP, be, beP, bes
+
+    def upload_particles(self, seg_off, seg_weight, x, y, px, py, theta, gamma):
+        arrs = [np.ascontiguousarray(a, np.float64) for a in (x, y, px, py, theta, gamma)]
+        so = np.ascontiguousarray(seg_off, np.int64)
+        sw = np.ascontiguousarray(seg_weight, np.float64)
+        self.np = int(so[-1])
+        self._chk(self.L.felis_upload_particles(self.h, _p(so), _p(sw), *[_p(a) for a in arrs]))
+
+    def download_particles(self):
+        out = [np.zeros(self.np) for _ in range(6)]
+        self._chk(self.L.felis_download_particles(self.h, *[_p(a) for a in out]))
+        x, y, px, py, th, ga = out
+        return dict(x=x, y=y, px=px, py=py, theta=th, gamma=ga)
+
+    def upload_field(self, which: int, arr: np.ndarray):
+        a = np.ascontiguousarray(arr, np.complex128)
+        self._chk(self.L.felis_upload_field(self.h, which, _p(a)))
+
+    def download_field(self, which: int) -> np.ndarray:
+        shape = self.ez_shape if which == EZ else self.field_shape
+        a = np.zeros(shape, np.complex128)
+        self._chk(self.L.felis_download_field(self.h, which, _p(a)))
+        return a
+
+    def set_wake(self, wakeE):
+        self._chk(self.L.felis_set_wake(self.h, _p(np.ascontiguousarray(wakeE, np.float64))))
+
+    # -- operators ---------------------------------------------------------------
+    def solve_field(self, lat: Lattice):
+        self._chk(self.L.felis_solve_field(self.h, C.byref(lat)))
+
+    def transverse_push(self, lat: Lattice):
+        self._chk(self.L.felis_transverse_push(self.h, C.byref(lat)))
+
+    def push_theta_gamma(self, lat: Lattice, it: int = 0):
+        self._chk(self.L.felis_push_theta_gamma(self.h, C.byref(lat), it))
+
+    def drift_theta_gamma(self, lat: Lattice, it: int = 0):
+        self._chk(self.L.felis_drift_theta_gamma(self.h, C.byref(lat), it))
+
+    def periodic(self, it: int = 0):
+        self._chk(self.L.felis_periodic_particles(self.h, it))
+
+    def shift(self, it: int = 0):
+        self._chk(self.L.felis_shift_field(self.h, it))
+
+    def update_wake(self, wakeF):
+        sn = self.desc.slice_n
+        wE = np.zeros(sn); den = np.zeros(sn); tot = np.zeros(1)
+        self._chk(self.L.felis_update_wake(self.h, _p(np.ascontiguousarray(wakeF, np.float64)), _p(wE), _p(den),
+                                           _p(tot)))
+        return wE, den, float(tot[0])
+
+    def total_energy(self):
+        nh = self.desc.n_harm
+        ex = np.zeros(nh); ey = np.zeros(nh)
+        self._chk(self.L.felis_update_total_energy(self.h, _p(ex), _p(ey)))
+        return ex, ey
+
+    def slice_energy(self):
+        d = self.desc
+        out = np.zeros((2, d.n_harm, d.slice_n))
+        self._chk(self.L.felis_slice_energy(self.h, _p(out)))
+        return out
+
+    def bfactor(self):
+        nh = self.desc.n_harm
+        b = np.zeros(2 * nh); tot = np.zeros(1)
+        self._chk(self.L.felis_update_bfactor(self.h, _p(b), _p(tot)))
+        return b[0::2] + 1j * b[1::2], float(tot[0])
+
+    def twiss(self):
+        ns = self.desc.n_species
+        s = np.zeros(7 * ns)
+        self._chk(self.L.felis_calculate_twiss(self.h, _p(s)))
+        return s.reshape(ns, 7)
+
+    def run(self, lat_tab, it0: int, n_steps: int, wake_step: int = 0, wakeF=None, outputs: bool = True):
+        d = self.desc
+        arr = (Lattice * len(lat_tab))(*lat_tab)
+        e = np.zeros((n_steps, 2 * d.n_harm)) if outputs else None
+        b = np.zeros((n_steps, 2 * d.n_harm + 1)) if outputs else None
+        t = np.zeros((n_steps, 7 * d.n_species)) if outputs else None
+        wf = None if wakeF is None else np.ascontiguousarray(wakeF, np.float64)
+        self._chk(self.L.felis_run(self.h, arr, it0, n_steps, wake_step, _p(wf), _p(e), _p(b), _p(t)))
+        return e, b, t
+
+    def sync(self):
+        self._chk(self.L.felis_sync(self.h))

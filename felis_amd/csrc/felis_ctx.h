@@ -1,0 +1,92 @@
+// Context: the device-resident equivalent of the reference Domain
+// (mesh.h:66-145) for one GPU that owns every slice.
+//
+// HBM layout
+//   U   [pol][h][phys slot][ny][nx]  complex T; slots form a ring of
+//       slice_n+2 so that slippage (solveField.cpp:19-47) is a rotation.
+//       logical slot s lives at phys (s + ring_base) mod (slice_n+2).
+//   Sc  [pol][h][slot][ny][nx]       complex T, not rotated.
+//   Uc  [pol][h][slice][ny][nx]      complex T, ADI half-step scratch.
+//   Ez  [l][slot][f][nr]             complex double.
+//   particles SoA: theta double; x, y, px, py, gamma in R (gamma stored as
+//       an offset from gamma_ref in the mixed path); segment-major.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "felis_common.h"
+
+struct felis_ctx {
+  felis_desc d{};
+  int S = 0;       // slots incl. the two ghosts
+  size_t N = 0;    // cells per slice
+  int bo = 0;      // Bessel orders
+  double dB = 0;   // Bessel table step
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::vector<void *> allocs, part_allocs;
+
+  // host copies of the setup tables
+  std::vector<felis::cpx<double>> h_abc, h_adix, h_adiy;
+  std::vector<double> h_bessel, h_den;
+  std::vector<long long> h_seg_off;
+  std::vector<double> h_seg_w;
+  std::vector<int> species_tile;
+
+  // device state
+  void *U = nullptr, *Sc = nullptr, *Uc = nullptr;
+  felis::cpx<double> *Ez = nullptr, *ScR = nullptr, *ez_d = nullptr;
+  double *ez_b = nullptr;
+  felis::cpx<double> *abc = nullptr, *adix = nullptr, *adiy = nullptr;
+  double *bessel = nullptr, *wakeE = nullptr, *wakeF = nullptr, *den = nullptr;
+  double *slice_e = nullptr, *red_out = nullptr;
+  unsigned *errw = nullptr;
+  int ring_base = 0;
+
+  long long np = 0;
+  int ntiles = 0;
+  double *theta = nullptr;
+  void *gamma = nullptr, *x = nullptr, *y = nullptr, *px = nullptr, *py = nullptr;
+  long long *seg_off = nullptr;
+  double *seg_w = nullptr;
+  int *tile_seg = nullptr, *tile_cnt = nullptr;
+  long long *tile_start = nullptr;
+  double *tile_red = nullptr;
+
+  // batched step (felis_run) resources
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  void *run_scratch = nullptr;
+
+  bool mixed() const { return d.precision == FELIS_MIXED_FP32; }
+  size_t esize() const { return mixed() ? 2 * sizeof(float) : 2 * sizeof(double); }
+  size_t rsize() const { return mixed() ? sizeof(float) : sizeof(double); }
+  size_t pol_bytes() const { return (size_t)d.n_harm * S * N * esize(); }
+  felis::Tiles tiles() const { return felis::Tiles{tile_seg, tile_start, tile_cnt}; }
+  felis::GridArgs grid() const;
+
+  int check_err();
+  void free_particles();
+  void release_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+  }
+
+  // per-phase operators (enqueue only; the C-ABI wrappers check errors)
+  int solve_field(const felis_lattice *lat);
+  int transverse(const felis_lattice *lat);
+  int push_theta_gamma(const felis_lattice *lat);
+  int drift(const felis_lattice *lat);
+  int periodic();
+  int shift();
+  int energy(double *dev_out);
+  int bfactor(double *dev_out);
+  int twiss(double *dev_out);
+  int update_wake(const double *wakeF_host, double *total_cnt);
+};
+
+int lat_args(felis_ctx *ctx, const felis_lattice *lat, felis::LatArgs *out);
+int dep_args(felis_ctx *ctx, const felis_lattice *lat, felis::DepArgs *out);

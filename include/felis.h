@@ -1,0 +1,150 @@
+/*
+ * felis.h -- C-ABI of libfelis_hip.so, the MI355X (gfx950) implementation of
+ * the FELIS per-undulator-step slice kernel.
+ *
+ * Each entry point replaces one free function of the reference's operator
+ * surface (mesh.h:150-178), called by the driver loop in the same order
+ * (main.cpp:92-156). The reference functions mutate a Domain; here a context
+ * owns the equivalent device state, and the caller passes the lattice state
+ * that the reference reads from the Domain (K0, ue, K0_alpha, ku, g,
+ * undType, currentFlag, driftFlag: updateK_quadG.cpp:8-94).
+ *
+ * Conventions
+ *   - Every function returns int: FELIS_OK (0) or a negative felis_status.
+ *     No exception crosses the ABI; felis_last_error() gives the message.
+ *   - Host arrays are caller-owned and copied. Complex arrays are interleaved
+ *     (re, im) doubles.
+ *   - Field arrays use the reference layout U[h][slot*N + j*nx + i] with
+ *     slots 0..slice_n+1 (boundary.cpp:49-60, one rank owns every slice).
+ *   - Particles are SoA, segment-major: segment = species*slice_n + slice,
+ *     seg_off[n_species*slice_n + 1], one weight per segment (particle.h:32).
+ *   - All work is ordered on the context's single HIP stream; a context is
+ *     used by one host thread at a time.
+ */
+#ifndef FELIS_H
+#define FELIS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FELIS_MAX_HARM 8
+
+enum felis_status {
+  FELIS_OK = 0,
+  FELIS_ERR_ARG = -1,         /* bad argument or shape */
+  FELIS_ERR_HIP = -2,         /* a HIP runtime call failed */
+  FELIS_ERR_GUARD = -3,       /* |dTheta| > 2*pi*numSlice (particlePush.cpp:370-374) */
+  FELIS_ERR_RANGE = -4,       /* theta out of range after the wrap (rearrangeParticles.cpp:105-108) */
+  FELIS_ERR_NAN = -5,         /* NaN theta met by the wrap */
+  FELIS_ERR_UNSUPPORTED = -6, /* input outside what the kernels model (see felis_last_error) */
+  FELIS_ERR_STATE = -7,       /* call out of order (e.g. no particles uploaded) */
+  FELIS_ERR_IO = -8           /* deck or file error */
+};
+
+enum felis_precision {
+  FELIS_PLAIN_FP64 = 0, /* every quantity in FP64, as the reference */
+  FELIS_MIXED_FP32 = 1  /* FP32 push/ADI/deposit, FP64 theta, slippage phase and reductions */
+};
+
+enum felis_field { FELIS_UX = 0, FELIS_UY = 1, FELIS_SCUX = 2, FELIS_SCUY = 3, FELIS_EZ = 4 };
+
+/* Grid, beam and solver description: the Domain scalars of mesh.h:66-145
+ * that the hot path reads, as derived by parameterSetting.cpp:42-285. */
+typedef struct {
+  int dimension;   /* 3 (the 1D path, dimension=1, is host-only) */
+  int mode;        /* 1 Static, 2 Time_Dependent (mesh.h:16-21) */
+  int precision;   /* felis_precision */
+  int nx, ny;      /* array sizes: deck nx+1, ny+1 (parameterSetting.cpp:108-111) */
+  int slice_n;
+  int n_harm;
+  int harmony[FELIS_MAX_HARM];
+  int num_slice;   /* slices_in_bucket */
+  int n_species;
+  int nr, sc_l, sc_f; /* radial grids, longitudinal and azimuthal space-charge modes */
+  int bessel_n;    /* Bessel table rows (num_grids, default 2001) */
+  int abc_n;       /* absorbing layer width in cells (ABC_N) */
+  int wake_on;
+  int pad0;
+  double abc_sig;  /* ABC_coef */
+  double dx, dy, dz, min_x, min_y, min_z, max_z;
+  double ks, lambda0, gamR, dr;
+  double gamma_ref; /* FELIS_MIXED_FP32: gamma is stored as an FP32 offset from this */
+} felis_desc;
+
+/* Lattice state at one half step (updateK_quadG.cpp:8-94 outputs). */
+typedef struct {
+  double K0, ue, K0_alpha, ku, g, lambdaU;
+  int und_type;    /* 1 BiPolar, 2 QuadPolar */
+  int current_flag;
+  int drift_flag;
+  int pad;
+} felis_lattice;
+
+typedef struct felis_ctx felis_ctx;
+
+/* -------- lifetime (replaces boundary.cpp:20-130 allocation and tables) -- */
+int felis_create(const felis_desc *desc, felis_ctx **out);
+int felis_destroy(felis_ctx *ctx);
+const char *felis_last_error(const felis_ctx *ctx);
+int felis_sync(felis_ctx *ctx);
+/* ABC coefficients alpha, alphaP (nx), beta, betaP (ny) as complex and the
+ * Bessel table [bessel_n][maxH/2+2] (boundary.cpp:73-123; one order more than
+ * the reference for an even maxH, whose last order it reads past its rows). */
+int felis_get_tables(felis_ctx *ctx, double *alpha, double *alphaP, double *beta,
+                     double *betaP, double *bessel);
+
+/* -------- state transfer (replaces loadBeam/loadSeed/restore*HDF) ------- */
+int felis_upload_particles(felis_ctx *ctx, const int64_t *seg_off, const double *seg_weight,
+                           const double *x, const double *y, const double *px, const double *py,
+                           const double *theta, const double *gamma);
+int felis_download_particles(felis_ctx *ctx, double *x, double *y, double *px, double *py,
+                             double *theta, double *gamma);
+/* which = felis_field. Ux/Uy/ScUx/ScUy: [n_harm][(slice_n+2)*nx*ny] complex;
+ * Ez: [sc_l][(slice_n+2)*sc_f*nr] complex. */
+int felis_upload_field(felis_ctx *ctx, int which, const double *data);
+int felis_download_field(felis_ctx *ctx, int which, double *data);
+/* Per-slice slice power sum_{cells,h,pol} |U|^2 of slots 1..slice_n. */
+int felis_slice_energy(felis_ctx *ctx, double *out /* [2][n_harm][slice_n] */);
+
+/* -------- the per-step operator surface (mesh.h:153-176) --------------- */
+/* solveField.cpp:49-78: deposition (if current_flag) + radial Ez + ADI. */
+int felis_solve_field(felis_ctx *ctx, const felis_lattice *lat);
+/* particlePush.cpp:32-117 (one dz/2 RK4). */
+int felis_transverse_push(felis_ctx *ctx, const felis_lattice *lat);
+/* particlePush.cpp:198-386 (3D RK4 of theta, gamma). */
+int felis_push_theta_gamma(felis_ctx *ctx, const felis_lattice *lat, int iteration);
+/* particlePush.cpp:119-195. */
+int felis_drift_theta_gamma(felis_ctx *ctx, const felis_lattice *lat, int iteration);
+/* rearrangeParticles.cpp:77-114. */
+int felis_periodic_particles(felis_ctx *ctx, int iteration);
+/* solveField.cpp:19-47 + fieldShareZ.cpp:39-78: slippage by one slice. */
+int felis_shift_field(felis_ctx *ctx, int iteration);
+/* wakeField.cpp:9-50 + 161-223. wakeF[slice_n] in; wakeE, den out (may be NULL). */
+int felis_update_wake(felis_ctx *ctx, const double *wakeF, double *wakeE, double *den,
+                      double *total_cnt);
+/* Sets the per-slice wake field directly (restart path). */
+int felis_set_wake(felis_ctx *ctx, const double *wakeE);
+
+/* -------- per-step diagnostics (main.cpp:123-125) ----------------------- */
+/* updateTotalEnergy.cpp:25-36: raw sums of |Ux|^2, |Uy|^2 per harmonic. */
+int felis_update_total_energy(felis_ctx *ctx, double *ex, double *ey);
+/* saveFile.cpp:124-154: b[2*h] + i b[2*h+1] = sum w exp(i H theta); total weight. */
+int felis_update_bfactor(felis_ctx *ctx, double *b, double *total_n);
+/* twiss.cpp:34-57: per species x^2, y^2, x'^2, y'^2, xx', yy' sums and count. */
+int felis_calculate_twiss(felis_ctx *ctx, double *sums /* [n_species][7] */);
+
+/* -------- whole loop (main.cpp:92-156 with I/O off) --------------------- */
+/* lat_tab: 2*n_steps+1 entries, [0] = lattice before step it0, [2k+1] =
+ * (it0+k, 0), [2k+2] = (it0+k, 0.5). Outputs per step (any may be NULL):
+ * energy[k][2*n_harm] (x sums then y sums), bfac[k][2*n_harm+1] (complex b,
+ * total weight), twiss[k][n_species*7]. wakeF (slice_n) may be NULL. */
+int felis_run(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps, int wake_step,
+              const double *wakeF, double *energy, double *bfac, double *twiss);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FELIS_H */

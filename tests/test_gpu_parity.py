@@ -1,0 +1,172 @@
+"""GPU parity: each C-ABI operator against the FP64 oracle on the same seeded state.
+
+Tolerances: the plain FP64 path differs from the oracle only by rounding
+(precomputed reciprocals in the Thomas sweeps, atomic summation order in
+the deposit), so per-phase results agree to 1e-11 relative or better;
+integer/index work (theta wrap, slippage) is bit-exact.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import fel_cases as fc
+
+pytestmark = pytest.mark.gpu
+
+fl = pytest.importorskip("felis_amd._lib")
+
+
+def _ctx(case, precision=0):
+    return fc.gpu_context(case, precision)
+
+
+CASES = {
+    "planar_h1": dict(),
+    "planar_h135": dict(harmonics=(1, 3, 5), nx=21, ny=19),
+    "helical_h12": dict(harmonics=(1, 2), ue=0.6, und_type=2),
+    "ka_minus1_h123": dict(harmonics=(1, 2, 3), ue=0.3, K0_alpha=-1),
+    "two_species_uneven": dict(n_species=2, uneven=True, ppslice=96),
+}
+
+
+@pytest.fixture(params=list(CASES))
+def case(request):
+    return fc.make_case(**CASES[request.param])
+
+
+def test_transverse_push(case):
+    o = case.oracle_state()
+    ctx = _ctx(case)
+    for half in (0, 1):
+        lat = case.lattice(g=(12.0 if half else -7.0))
+        assert o.transverse_push(lat) == 0
+        ctx.transverse_push(fc.to_lib_lattice(lat))
+    p = ctx.download_particles()
+    for k in ("x", "y", "px", "py"):
+        assert fc.rel_err(p[k], getattr(o, k)) < 1e-13, k
+    assert np.array_equal(p["theta"], o.theta) and np.array_equal(p["gamma"], o.gamma)
+
+
+def test_push_theta_gamma(case):
+    o = case.oracle_state()
+    ctx = _ctx(case)
+    lat = case.lattice()
+    assert o.push_theta_gamma(lat) == 0
+    ctx.push_theta_gamma(fc.to_lib_lattice(lat))
+    p = ctx.download_particles()
+    th0 = case.state["theta"]
+    g0 = case.state["gamma"]
+    # compare the increments: they carry the physics
+    assert fc.rel_err(p["theta"] - th0, o.theta - th0) < 1e-11
+    assert fc.rel_err(p["gamma"] - g0, o.gamma - g0) < 1e-10
+    assert np.array_equal(p["x"], o.x)
+
+
+def test_drift(case):
+    o = case.oracle_state()
+    ctx = _ctx(case)
+    lat = case.lattice(ku=0.0, drift_flag=1, current_flag=0)
+    assert o.drift_theta_gamma(lat) == 0
+    ctx.drift_theta_gamma(fc.to_lib_lattice(lat))
+    p = ctx.download_particles()
+    th0 = case.state["theta"]
+    assert fc.rel_err(p["theta"] - th0, o.theta - th0) < 1e-12
+    assert np.array_equal(p["gamma"], o.gamma)
+
+
+def test_periodic_bit_exact():
+    case = fc.make_case(num_slice=2)
+    s = case.state
+    rng = np.random.default_rng(5)
+    dth = 2 * math.pi * 2
+    s["theta"][:] = rng.uniform(-3 * dth, 4 * dth, s["theta"].size)
+    s["theta"][:4] = [-dth, 0.0, dth, -1e-17]
+    o = case.oracle_state()
+    ctx = _ctx(case)
+    assert o.periodic() == 0
+    ctx.periodic()
+    assert np.array_equal(ctx.download_particles()["theta"], o.theta)
+
+
+def test_solve_field_with_current(case):
+    o = case.oracle_state()
+    ctx = _ctx(case)
+    lat = case.lattice()
+    assert o.solve_field(lat) == 0
+    ctx.solve_field(fc.to_lib_lattice(lat))
+    for which, ref in ((fl.SCUX, o.ScUx), (fl.SCUY, o.ScUy)):
+        got = ctx.download_field(which)
+        assert fc.rel_err(got, ref) < 1e-12
+    ez = ctx.download_field(fl.EZ)
+    assert fc.rel_err(ez, o.Ez) < 1e-12
+    for which, ref in ((fl.UX, o.Ux), (fl.UY, o.Uy)):
+        got = ctx.download_field(which)
+        assert fc.rel_err(got, ref) < 1e-12
+
+
+def test_solve_field_no_current(case):
+    o = case.oracle_state()
+    ctx = _ctx(case)
+    lat = case.lattice(current_flag=0)
+    assert o.solve_field(lat) == 0
+    ctx.solve_field(fc.to_lib_lattice(lat))
+    for which, ref in ((fl.UX, o.Ux), (fl.UY, o.Uy)):
+        assert fc.rel_err(ctx.download_field(which), ref) < 1e-12
+
+
+def test_shift_bit_exact(case):
+    o = case.oracle_state()
+    ctx = _ctx(case)
+    for _ in range(3):
+        o.shift()
+        ctx.shift()
+    for which, ref in ((fl.UX, o.Ux), (fl.UY, o.Uy)):
+        assert np.array_equal(ctx.download_field(which), ref)
+
+
+def test_diagnostics(case):
+    o = case.oracle_state()
+    ctx = _ctx(case)
+    ex, ey = o.total_energy()
+    gx, gy = ctx.total_energy()
+    assert fc.rel_err(gx, ex) < 1e-13 and fc.rel_err(gy, ey) < 1e-13
+    b, tot = o.bfactor()
+    gb, gtot = ctx.bfactor()
+    assert abs(gtot - tot) <= 1e-12 * tot
+    assert np.max(np.abs(gb - b)) <= 1e-11 * tot
+    sums, _ = o.twiss()
+    gs = ctx.twiss()
+    assert fc.rel_err(gs, sums) < 1e-12
+
+
+def test_update_wake():
+    case = fc.make_case(slice_n=40, n_species=2, uneven=True)
+    o = case.oracle_state()
+    ctx = _ctx(case)
+    rng = np.random.default_rng(3)
+    wakeF = 1e14 * rng.normal(size=case.slice_n)
+    den, tot = o.update_wake(wakeF)
+    wE, gden, gtot = ctx.update_wake(wakeF)
+    # the line density is host arithmetic in both; contraction differs by an ulp
+    assert abs(gtot - tot) <= 1e-15 * tot
+    assert fc.rel_err(gden, den) < 1e-15
+    assert fc.rel_err(wE, o.wakeE) < 1e-14
+
+
+def test_run_plain_matches_oracle_loop():
+    """The whole step loop (main.cpp:92-156) for 12 steps: gain curve and bunching."""
+    case = fc.make_case(slice_n=6, harmonics=(1, 3), ppslice=128)
+    o = case.oracle_state()
+    ctx = _ctx(case)
+    nsteps = 12
+    lat0 = case.lattice(current_flag=0)
+    tab = [lat0] + [case.lattice() for _ in range(2 * nsteps)]
+    rc, e_o, b_o, t_o = o.run(tab, 0, nsteps, True)
+    assert rc == 0
+    e_g, b_g, t_g = ctx.run([fc.to_lib_lattice(l) for l in tab], 0, nsteps)
+    assert fc.rel_err(e_g, e_o) < 1e-9
+    nh = case.params.n_harm
+    bg = np.abs(b_g[:, 0:2 * nh:2] + 1j * b_g[:, 1:2 * nh:2]) / b_g[:, 2 * nh:2 * nh + 1]
+    assert fc.rel_err(bg, b_o[:, :nh]) < 1e-8
+    assert fc.rel_err(t_g, t_o) < 1e-9
