@@ -1,0 +1,213 @@
+#!/usr/bin/env python3
+"""Benchmark of the FELIS per-step slice kernel on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config cfg1] [--precision plain|mixed]
+
+A step is one undulator period of the whole slice loop of main.cpp:92-156
+(diagnostics, field solve, transverse and longitudinal push, theta wrap,
+slippage) over every slice of the configuration, with I/O off. The beam and
+seed are generated on the device from the configuration's input deck
+(felis_amd/decks/), so inputs are resident in HBM when timing starts.
+
+Prints one JSON line: value = slices x periods / s over the timed steps, the
+roofline of the dominant kernel (algorithmic bytes per launch over its mean
+HIP-event duration), and the FP64 oracle timed on the host cores on a bounded
+sample of the same workload (cpu_baseline).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "slice·periods/s at ≤1e-3 rel power err vs FP64 oracle; batched+surrogate vs plain"
+PEAK_HBM_GBS = 8000.0
+WORKLOADS = {
+    "cfg1": "time-dependent 256 slices x 4096 particles, 101x101 grid, harmonic 1, flat 3 kA, 10 m (385 periods)",
+    "cfg2": "time-dependent 2048 slices, 256-16384 particles/slice (11,670,128), 129x129, harmonics 1,3,5, "
+            "Gaussian 4 kA, 20 m (770 periods)",
+}
+
+
+def kernel_bytes(name: str, d, np_: int, mixed: bool) -> float:
+    """Algorithmic HBM bytes of one launch of a kernel: each operand it must
+    read or write, once (the bilinear field gather is counted as cached)."""
+    rs = 4 if mixed else 8          # particle coordinate bytes (theta is 8)
+    es = 8 if mixed else 16         # complex field element bytes
+    cells = d.nx * d.ny
+    interior = (d.nx - 2) * (d.ny - 2)
+    nsys = 2 * d.n_harm * d.slice_n
+    table = {
+        "transverse": np_ * 9 * rs,                       # x,y,px,py,gamma in; x,y,px,py out
+        "push": np_ * (8 + 5 * rs + 8 + rs),              # theta + 5 coords in; theta, gamma out
+        "drift": np_ * (8 + 4 * rs + 8),
+        "periodic": np_ * 16,
+        "deposit": np_ * (8 + 5 * rs),                    # particle reads (scatter counted as cached)
+        "ez_deposit": np_ * (8 + 2 * rs),
+        "adi_x": nsys * interior * 3 * es,                # U in, Sc in, Uc out
+        "adi_y": nsys * interior * 3 * es,                # Uc in, Sc in, U out
+        "energy": nsys * cells * es,
+        "particle_step": np_ * 2 * (8 + 5 * rs),          # full particle state in and out
+        "adi_fused": nsys * interior * 2 * es,            # U in and out
+    }
+    return float(table.get(name, 0.0))
+
+
+def run_gpu(args):
+    from felis_amd import _lib as fl
+    deck = fl.Deck(ROOT / "felis_amd" / "decks" / f"{args.config}.inp")
+    prec = fl.MIXED_FP32 if args.precision == "mixed" else fl.PLAIN_FP64
+    desc = deck.desc(prec)
+    ctx = fl.Context(desc)
+    deck.load(ctx, seed=args.seed)
+    init = None
+    if not args.no_cpu:
+        init = dict(particles=ctx.download_particles(), Ux=ctx.download_field(fl.UX), Uy=ctx.download_field(fl.UY))
+    W, K = args.warmup, args.steps
+    tab = deck.lattice(0, W + K)
+    wakeF = deck.wake_function()[0] if desc.wake_on else None
+    ctx.sync()
+    if W:
+        ctx.run(tab[0:2 * W + 1], 0, W, deck.wake_step, wakeF)
+    ctx.sync()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    energy, bfac, twiss = ctx.run(tab[2 * W:2 * (W + K) + 1], W, K, deck.wake_step, wakeF)
+    ctx.sync()
+    t1 = time.perf_counter()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    return dict(deck=deck, desc=desc, ctx=ctx, seconds=t1 - t0, prof=prof, energy=energy, init=init, tab=tab,
+                wakeF=wakeF, np=ctx.np)
+
+
+def run_cpu(args, g):
+    """The FP64 oracle on the first steps of the same workload (bounded sample)."""
+    from oracle import pyoracle as po
+    d = g["desc"]
+    threads = int(args.cpu_threads or min(16, os.cpu_count() or 1))
+    po.set_threads(threads)
+    p = po.Params()
+    p.nx, p.ny, p.slice_n, p.n_harm = d.nx, d.ny, d.slice_n, d.n_harm
+    for h in range(d.n_harm):
+        p.harmony[h] = d.harmony[h]
+    maxh = d.harmony[d.n_harm - 1]
+    p.num_slice, p.n_species, p.nr, p.sc_l, p.sc_f = d.num_slice, d.n_species, d.nr, d.sc_l, d.sc_f
+    p.bn, p.b_orders, p.wake_on, p.dimension = d.bessel_n, maxh // 2 + 2, d.wake_on, 3
+    p.dx, p.dy, p.dz, p.min_x, p.min_y = d.dx, d.dy, d.dz, d.min_x, d.min_y
+    p.ks, p.lambda0, p.gamR, p.dr = d.ks, d.lambda0, d.gamR, d.dr
+    p.dBessel = maxh * 2.0 * np.pi / d.bessel_n
+    al, alP, be, beP = po.abc_setup(d.nx, d.ny, d.abc_n, d.abc_sig, d.dx, d.dy, d.dz, d.ks)
+    bes = po.bessel_table(d.bessel_n, maxh)
+    seg_off, seg_w = g["deck"].beam_layout(d.n_species)
+    part = g["init"]["particles"]
+    S = d.slice_n + 2
+    st = po.OracleState(params=p, alpha=al, alphaP=alP, beta=be, betaP=beP, bessel=bes, seg_off=seg_off,
+                        seg_weight=seg_w, x=part["x"].copy(), y=part["y"].copy(), px=part["px"].copy(),
+                        py=part["py"].copy(), theta=part["theta"].copy(), gamma=part["gamma"].copy(),
+                        Ux=g["init"]["Ux"].copy(), Uy=g["init"]["Uy"].copy(),
+                        ScUx=np.zeros((d.n_harm, S, d.ny, d.nx), np.complex128),
+                        ScUy=np.zeros((d.n_harm, S, d.ny, d.nx), np.complex128),
+                        Ez=np.zeros((d.sc_l, S * d.sc_f * d.nr), np.complex128), wakeE=np.zeros(d.slice_n))
+    tab = g["tab"]
+    lat = [po.Lattice() for _ in range(2 * args.cpu_steps + 1)]
+    for k in range(2 * args.cpu_steps + 1):
+        for name, _ in po.Lattice._fields_:
+            setattr(lat[k], name, getattr(tab[k], name))
+    t0 = time.perf_counter()
+    rc, energy, _, _ = st.run(lat, 0, args.cpu_steps, d.mode == 2, g["deck"].wake_step, g["wakeF"])
+    t1 = time.perf_counter()
+    if rc != 0:
+        raise RuntimeError(f"oracle run failed: {rc}")
+    return dict(seconds=t1 - t0, threads=threads, energy=energy)
+
+
+def gpu_parity(args, g, cpu):
+    """GPU from the same initial state over the sampled steps vs the oracle."""
+    from felis_amd import _lib as fl
+    d = g["desc"]
+    ctx = fl.Context(d)
+    part = g["init"]["particles"]
+    seg_off, seg_w = g["deck"].beam_layout(d.n_species)
+    ctx.upload_particles(seg_off, seg_w, part["x"], part["y"], part["px"], part["py"], part["theta"],
+                         part["gamma"])
+    ctx.upload_field(fl.UX, g["init"]["Ux"])
+    ctx.upload_field(fl.UY, g["init"]["Uy"])
+    e, _, _ = ctx.run(g["tab"][:2 * args.cpu_steps + 1], 0, args.cpu_steps, g["deck"].wake_step, g["wakeF"])
+    ctx.close()
+    nh = d.n_harm
+    tot_g = e[:, :nh].sum(1) + e[:, nh:].sum(1)
+    tot_o = cpu["energy"][:, :nh].sum(1) + cpu["energy"][:, nh:].sum(1)
+    return float(np.max(np.abs(tot_g - tot_o) / np.abs(tot_o)))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default="cfg1", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="plain", choices=["plain", "mixed"])
+    ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (slices shard across GPUs "
+              "only by design, see the sharding notes); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    g = run_gpu(args)
+    d = g["desc"]
+    K = args.steps
+    value = d.slice_n * K / g["seconds"]
+    mixed = args.precision == "mixed"
+    # dominant kernel and its roofline
+    prof = g["prof"]
+    dom = max(prof, key=lambda k: prof[k][0])
+    ms, n = prof[dom]
+    avg_s = ms / n * 1e-3
+    bpl = kernel_bytes(dom, d, g["np"], mixed)
+    achieved = bpl / avg_s / 1e9 if bpl else 0.0
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "slice·periods/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": g["seconds"] / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if mixed else "f64",
+        "data": "synthetic: quiet-start beam and Gaussian seed generated on the device from the deck",
+        "config": {"workload": f"{args.config}: {WORKLOADS[args.config]}", "precision": args.precision,
+                   "slices": d.slice_n, "particles": g["np"], "grid": f"{d.nx}x{d.ny}",
+                   "harmonics": [d.harmony[i] for i in range(d.n_harm)]},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None, "bytes_per_launch": bpl,
+                     "avg_launch_us": avg_s * 1e6},
+        "kernels_ms_per_step": {k: v[0] / K for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])},
+    }
+    if not args.no_cpu:
+        cpu = run_cpu(args, g)
+        out["cpu_baseline"] = {"value": d.slice_n * args.cpu_steps / cpu["seconds"], "unit": "slice·periods/s",
+                               "cores": cpu["threads"], "kind": "port",
+                               "sample": f"first {args.cpu_steps} periods of the same workload and initial state, "
+                                         f"FP64 oracle (oracle/felis_oracle.c), {cpu['threads']} threads"}
+        out["parity"] = {"steps": args.cpu_steps, "max_rel_err_total_energy": gpu_parity(args, g, cpu)}
+    g["ctx"].close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

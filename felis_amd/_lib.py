@@ -26,8 +26,24 @@ EXPORTS = [
     "felis_slice_energy", "felis_solve_field", "felis_transverse_push", "felis_push_theta_gamma",
     "felis_drift_theta_gamma", "felis_periodic_particles", "felis_shift_field", "felis_update_wake",
     "felis_set_wake", "felis_update_total_energy", "felis_update_bfactor", "felis_calculate_twiss",
-    "felis_run",
+    "felis_run", "felis_find_parameter", "felis_deck_open", "felis_deck_close", "felis_deck_error",
+    "felis_deck_desc", "felis_deck_steps", "felis_deck_lattice", "felis_deck_wake_function",
+    "felis_deck_beam_layout", "felis_deck_load_beam", "felis_deck_load_seed", "felis_profile",
+    "felis_profile_read", "felis_device_info",
 ]
+
+
+def device_info() -> dict:
+    info = (C.c_int * 8)()
+    rc = load().felis_device_info(info)
+    if rc != OK:
+        raise FelisError(rc, "felis_device_info")
+    keys = ["cus", "lds_per_block", "lds_per_cu", "wave", "l2_bytes", "clock_khz", "mem_clock_khz", "bus_bits"]
+    return dict(zip(keys, list(info)))
+
+# felis_kernel_id order
+KERNELS = ["transverse", "push", "drift", "periodic", "deposit", "ez_deposit", "ez_solve", "adi_x", "adi_y",
+           "shift", "energy", "bfactor", "twiss", "wake", "zero", "particle_step", "adi_fused"]
 
 
 class FelisError(RuntimeError):
@@ -92,6 +108,21 @@ def load() -> C.CDLL:
         L.felis_update_bfactor.argtypes = [C.c_void_p, _dp, _dp]
         L.felis_calculate_twiss.argtypes = [C.c_void_p, _dp]
         L.felis_run.argtypes = [C.c_void_p, C.POINTER(Lattice), C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+        L.felis_find_parameter.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]
+        L.felis_deck_open.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+        L.felis_deck_close.argtypes = [C.c_void_p]
+        L.felis_deck_error.argtypes = [C.c_void_p]
+        L.felis_deck_error.restype = C.c_char_p
+        L.felis_deck_desc.argtypes = [C.c_void_p, C.c_int, C.POINTER(Desc)]
+        L.felis_deck_steps.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.felis_deck_lattice.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(Lattice)]
+        L.felis_deck_wake_function.argtypes = [C.c_void_p, _dp, _dp]
+        L.felis_deck_beam_layout.argtypes = [C.c_void_p, _i64p, _dp]
+        L.felis_deck_load_beam.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
+        L.felis_deck_load_seed.argtypes = [C.c_void_p, C.c_void_p]
+        L.felis_profile.argtypes = [C.c_void_p, C.c_int]
+        L.felis_profile_read.argtypes = [C.c_void_p, _dp, _i64p]
+        L.felis_device_info.argtypes = [C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -235,3 +266,84 @@ class Context:
 
     def sync(self):
         self._chk(self.L.felis_sync(self.h))
+
+    def profile(self, enable: bool):
+        self._chk(self.L.felis_profile(self.h, int(enable)))
+
+    def profile_read(self) -> dict:
+        ms = np.zeros(len(KERNELS)); n = np.zeros(len(KERNELS), np.int64)
+        self._chk(self.L.felis_profile_read(self.h, _p(ms), _p(n)))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(KERNELS) if n[i]}
+
+
+def find_parameter(block: str, rank: int, key: str, path: str):
+    """findparam.cpp:30-116 lookup: the value string, or None."""
+    buf = C.create_string_buffer(512)
+    rc = load().felis_find_parameter(block.encode(), rank, key.encode(), str(path).encode(), buf, 512)
+    if rc < 0:
+        raise FelisError(rc, f"cannot read {path}")
+    return buf.value.decode() if rc == 1 else None
+
+
+class Deck:
+    """An input deck parsed by the C++ host (parameterSetting.cpp semantics)."""
+
+    def __init__(self, path):
+        self.L = load()
+        h = C.c_void_p()
+        rc = self.L.felis_deck_open(str(path).encode(), C.byref(h))
+        self.h = h
+        if rc != OK:
+            msg = self.L.felis_deck_error(h).decode() if h else "open failed"
+            self.close()
+            raise FelisError(rc, msg)
+        info = (C.c_int * 8)()
+        self.L.felis_deck_steps(self.h, info)
+        self.max_step, self.wake_step, self.save_step, self.save_start = info[0], info[1], info[2], info[3]
+        self.field_save, self.particle_save, self.mode, self.slice_n = bool(info[4]), bool(info[5]), info[6], info[7]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.felis_deck_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def desc(self, precision: int = PLAIN_FP64) -> Desc:
+        d = Desc()
+        rc = self.L.felis_deck_desc(self.h, precision, C.byref(d))
+        if rc != OK:
+            raise FelisError(rc, "felis_deck_desc")
+        return d
+
+    def lattice(self, it0: int, n_steps: int):
+        tab = (Lattice * (2 * n_steps + 1))()
+        self.L.felis_deck_lattice(self.h, it0, n_steps, tab)
+        return list(tab)
+
+    def wake_function(self):
+        wf = np.zeros(self.slice_n); imp = np.zeros(1001)
+        rc = self.L.felis_deck_wake_function(self.h, _p(wf), _p(imp))
+        if rc != OK:
+            raise FelisError(rc, "wake function")
+        return wf, imp
+
+    def beam_layout(self, n_species: int):
+        so = np.zeros(n_species * self.slice_n + 1, np.int64)
+        sw = np.zeros(n_species * self.slice_n)
+        self.L.felis_deck_beam_layout(self.h, _p(so), _p(sw))
+        return so, sw
+
+    def load(self, ctx: Context, seed: int = 20261203):
+        """Beam (loadBeam.cpp) and seed field (loadSeed.cpp) generated on the device."""
+        rc = self.L.felis_deck_load_beam(self.h, ctx.h, seed)
+        if rc != OK:
+            raise FelisError(rc, self.L.felis_last_error(ctx.h).decode())
+        ctx.np = int(self.beam_layout(ctx.desc.n_species)[0][-1])
+        rc = self.L.felis_deck_load_seed(self.h, ctx.h)
+        if rc != OK:
+            raise FelisError(rc, self.L.felis_last_error(ctx.h).decode())

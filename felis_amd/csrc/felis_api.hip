@@ -45,6 +45,7 @@ static int fail(felis_ctx *ctx, int code, const std::string &msg) {
 static int zero_async(felis_ctx *ctx, void *p, size_t bytes) {
   size_t n = bytes / sizeof(unsigned);
   if (n == 0) return FELIS_OK;
+  KTimer kt(ctx, FELIS_K_ZERO);
   hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)std::min<size_t>((n + 255) / 256, 65535 * 8)), dim3(256), 0,
                      ctx->stream, (unsigned *)p, (long long)n, 0u);
   LAUNCHCHK();
@@ -276,6 +277,21 @@ extern "C" int felis_create(const felis_desc *desc, felis_ctx **out) {
     return e;
   }
   build_tables(ctx);
+  {
+    int dev = 0, lds = 65536;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+      ctx->lds_max = lds - 1024;  // room for the kernel's static LDS
+    if (ctx->lds_max > 65536 - 1024) {
+      // opt in to more than 64 KiB of dynamic LDS where the device allows it
+      if (hipFuncSetAttribute((const void *)k_deposit_win<double, double>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, ctx->lds_max) != hipSuccess ||
+          hipFuncSetAttribute((const void *)k_deposit_win<float, float>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, ctx->lds_max) != hipSuccess)
+        ctx->lds_max = 65536 - 1024;
+    }
+    (void)hipGetLastError();
+  }
   const size_t es = ctx->esize();
   const size_t fsz = (size_t)2 * d.n_harm * ctx->S * ctx->N * es;
   const int nsys = 2 * d.n_harm * d.slice_n;
@@ -284,7 +300,7 @@ extern "C" int felis_create(const felis_desc *desc, felis_ctx **out) {
       (e = dalloc(ctx, &ctx->Uc, (size_t)nsys * ctx->N * es)) ||
       (e = dalloc(ctx, &ctx->Ez, (size_t)d.sc_l * ctx->S * d.sc_f * d.nr * sizeof(cpx<double>))) ||
       (e = dalloc(ctx, &ctx->ScR, (size_t)ctx->S * d.sc_l * nrow * sizeof(cpx<double>))) ||
-      (e = dalloc(ctx, &ctx->ez_b, (size_t)d.slice_n * d.sc_l * d.sc_f * d.nr * sizeof(double))) ||
+      (e = dalloc(ctx, &ctx->ez_b, (size_t)2 * d.sc_l * d.sc_f * d.nr * sizeof(double))) ||
       (e = dalloc(ctx, &ctx->ez_d, (size_t)d.slice_n * d.sc_l * d.sc_f * d.nr * sizeof(cpx<double>))) ||
       (e = dalloc(ctx, &ctx->abc, ctx->h_abc.size() * sizeof(cpx<double>))) ||
       (e = dalloc(ctx, &ctx->adix, ctx->h_adix.size() * sizeof(cpx<double>))) ||
@@ -324,9 +340,61 @@ void felis_ctx::free_particles() {
   ntiles = 0;
 }
 
+hipEvent_t felis_ctx::ev_get() {
+  if (ev_used == ev_pool.size()) {
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    ev_pool.push_back(e);
+  }
+  return ev_pool[ev_used++];
+}
+
+// Folds recorded event pairs into the per-kernel totals.
+void felis_ctx::ev_flush() {
+  if (ev_marks.empty()) {
+    ev_used = 0;
+    return;
+  }
+  (void)hipStreamSynchronize(stream);
+  for (auto &m : ev_marks) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, m.second.first, m.second.second) == hipSuccess) {
+      prof_ms[m.first] += ms;
+      prof_n[m.first] += 1;
+    }
+  }
+  ev_marks.clear();
+  ev_used = 0;
+}
+
+extern "C" int felis_profile(felis_ctx *ctx, int enable) {
+  if (!ctx) return FELIS_ERR_ARG;
+  ctx->ev_flush();
+  if (enable) {
+    for (int k = 0; k < FELIS_K_COUNT; ++k) {
+      ctx->prof_ms[k] = 0;
+      ctx->prof_n[k] = 0;
+    }
+  }
+  ctx->profiling = enable != 0;
+  return FELIS_OK;
+}
+
+extern "C" int felis_profile_read(felis_ctx *ctx, double *ms, int64_t *launches) {
+  if (!ctx) return FELIS_ERR_ARG;
+  ctx->ev_flush();
+  for (int k = 0; k < FELIS_K_COUNT; ++k) {
+    if (ms) ms[k] = ctx->prof_ms[k];
+    if (launches) launches[k] = ctx->prof_n[k];
+  }
+  return FELIS_OK;
+}
+
 extern "C" int felis_destroy(felis_ctx *ctx) {
   if (!ctx) return FELIS_OK;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  ctx->ev_marks.clear();
+  for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
   ctx->release_graph();
   ctx->free_particles();
   for (void *p : ctx->allocs) (void)hipFree(p);
@@ -336,6 +404,25 @@ extern "C" int felis_destroy(felis_ctx *ctx) {
 }
 
 extern "C" const char *felis_last_error(const felis_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" int felis_device_info(int *info) {
+  if (!info) return FELIS_ERR_ARG;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return FELIS_ERR_HIP;
+  const hipDeviceAttribute_t at[8] = {hipDeviceAttributeMultiprocessorCount,
+                                      hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                      hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                      hipDeviceAttributeWarpSize,
+                                      hipDeviceAttributeL2CacheSize,
+                                      hipDeviceAttributeClockRate,
+                                      hipDeviceAttributeMemoryClockRate,
+                                      hipDeviceAttributeMemoryBusWidth};
+  for (int k = 0; k < 8; ++k) {
+    info[k] = 0;
+    if (hipDeviceGetAttribute(&info[k], at[k], dev) != hipSuccess) return FELIS_ERR_HIP;
+  }
+  return FELIS_OK;
+}
 
 extern "C" int felis_sync(felis_ctx *ctx) {
   if (!ctx) return FELIS_ERR_ARG;
@@ -385,11 +472,9 @@ static int download_coord(felis_ctx *ctx, const void *src, double *dst, double o
   return FELIS_OK;
 }
 
-extern "C" int felis_upload_particles(felis_ctx *ctx, const int64_t *seg_off, const double *seg_weight,
-                                      const double *x, const double *y, const double *px,
-                                      const double *py, const double *theta, const double *gamma) {
-  if (!ctx || !seg_off || !seg_weight) return FELIS_ERR_ARG;
-  const felis_desc &d = ctx->d;
+// Allocates the particle arrays and the tile table for a segment layout.
+int felis_ctx::alloc_particles(const int64_t *seg_off, const double *seg_weight) {
+  felis_ctx *ctx = this;
   int nseg = d.n_species * d.slice_n;
   if (seg_off[0] != 0) return fail(ctx, FELIS_ERR_ARG, "seg_off[0] must be 0");
   for (int s = 0; s < nseg; ++s)
@@ -414,6 +499,15 @@ extern "C" int felis_upload_particles(felis_ctx *ctx, const int64_t *seg_off, co
   }
   ctx->species_tile[d.n_species] = (int)tseg.size();
   ctx->ntiles = (int)tseg.size();
+  std::vector<int> cseg, ccnt;
+  std::vector<long long> cstart;
+  for (int s = 0; s < nseg; ++s)
+    for (long long b = seg_off[s]; b < seg_off[s + 1]; b += kChunk) {
+      cseg.push_back(s);
+      cstart.push_back(b);
+      ccnt.push_back((int)std::min<long long>(kChunk, seg_off[s + 1] - b));
+    }
+  ctx->nchunks = (int)cseg.size();
   const size_t np = (size_t)ctx->np, rs = ctx->rsize();
   auto pal = [&](void **p, size_t bytes) -> int {
     if (bytes == 0) bytes = 16;
@@ -433,7 +527,10 @@ extern "C" int felis_upload_particles(felis_ctx *ctx, const int64_t *seg_off, co
       (e = pal((void **)&ctx->tile_seg, tseg.size() * sizeof(int))) ||
       (e = pal((void **)&ctx->tile_start, tstart.size() * sizeof(long long))) ||
       (e = pal((void **)&ctx->tile_cnt, tcnt.size() * sizeof(int))) ||
-      (e = pal((void **)&ctx->tile_red, std::max<size_t>(tseg.size(), 1) * (2 * FELIS_MAX_HARM + 1) * sizeof(double))))
+      (e = pal((void **)&ctx->tile_red, std::max<size_t>(tseg.size(), 1) * (2 * FELIS_MAX_HARM + 1) * sizeof(double))) ||
+      (e = pal((void **)&ctx->chunk_seg, cseg.size() * sizeof(int))) ||
+      (e = pal((void **)&ctx->chunk_start, cstart.size() * sizeof(long long))) ||
+      (e = pal((void **)&ctx->chunk_cnt, ccnt.size() * sizeof(int))))
     return e;
   HIPCHK(hipMemcpy(ctx->seg_off, seg_off, (nseg + 1) * sizeof(long long), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(ctx->seg_w, seg_weight, nseg * sizeof(double), hipMemcpyHostToDevice));
@@ -441,7 +538,21 @@ extern "C" int felis_upload_particles(felis_ctx *ctx, const int64_t *seg_off, co
     HIPCHK(hipMemcpy(ctx->tile_seg, tseg.data(), tseg.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->tile_start, tstart.data(), tstart.size() * sizeof(long long), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->tile_cnt, tcnt.data(), tcnt.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->chunk_seg, cseg.data(), cseg.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->chunk_start, cstart.data(), cstart.size() * sizeof(long long), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->chunk_cnt, ccnt.data(), ccnt.size() * sizeof(int), hipMemcpyHostToDevice));
   }
+  return FELIS_OK;
+}
+
+extern "C" int felis_upload_particles(felis_ctx *ctx, const int64_t *seg_off, const double *seg_weight,
+                                      const double *x, const double *y, const double *px,
+                                      const double *py, const double *theta, const double *gamma) {
+  if (!ctx || !seg_off || !seg_weight) return FELIS_ERR_ARG;
+  const felis_desc &d = ctx->d;
+  int e = ctx->alloc_particles(seg_off, seg_weight);
+  if (e) return e;
+  const size_t np = (size_t)ctx->np;
   if (np) {
     if (!theta) return fail(ctx, FELIS_ERR_ARG, "null theta");
     HIPCHK(hipMemcpy(ctx->theta, theta, np * sizeof(double), hipMemcpyHostToDevice));
@@ -559,6 +670,7 @@ static int launch_transverse(felis_ctx *ctx, const LatArgs &l) {
   if (!ctx->np) return FELIS_OK;
   GridArgs g = ctx->grid();
   unsigned nb = (unsigned)((ctx->np + 255) / 256);
+  KTimer kt(ctx, FELIS_K_TRANSVERSE);
   hipLaunchKernelGGL((k_transverse<R>), dim3(nb), dim3(256), 0, ctx->stream, (long long)ctx->np, g, l,
                      (R *)ctx->x, (R *)ctx->y, (R *)ctx->px, (R *)ctx->py, (const R *)ctx->gamma);
   LAUNCHCHK();
@@ -583,6 +695,7 @@ template <typename R, typename T>
 static int launch_push(felis_ctx *ctx, const LatArgs &l) {
   if (!ctx->ntiles) return FELIS_OK;
   GridArgs g = ctx->grid();
+  KTimer kt(ctx, FELIS_K_PUSH);
   hipLaunchKernelGGL((k_push_tg<R, T>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, g, l, ctx->tiles(),
                      ctx->theta, (R *)ctx->gamma, (const R *)ctx->x, (const R *)ctx->y, (const R *)ctx->px,
                      (const R *)ctx->py, (const cpx<T> *)ctx->U,
@@ -603,6 +716,7 @@ template <typename R, typename T>
 static int launch_drift(felis_ctx *ctx, const LatArgs &l) {
   if (!ctx->ntiles) return FELIS_OK;
   GridArgs g = ctx->grid();
+  KTimer kt(ctx, FELIS_K_DRIFT);
   hipLaunchKernelGGL((k_drift<R, T>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, g, l, ctx->tiles(),
                      ctx->theta, (const R *)ctx->x, (const R *)ctx->y, (const R *)ctx->px, (const R *)ctx->py,
                      (const cpx<T> *)ctx->U, (const cpx<T> *)((char *)ctx->U + ctx->pol_bytes()), ctx->wakeE,
@@ -621,6 +735,7 @@ int felis_ctx::periodic() {
   felis_ctx *ctx = this;
   if (!np) return FELIS_OK;
   unsigned nb = (unsigned)((np + 255) / 256);
+  KTimer kt(ctx, FELIS_K_PERIODIC);
   hipLaunchKernelGGL(k_periodic, dim3(nb), dim3(256), 0, stream, (long long)np, d.num_slice * 2 * M_PI, theta, errw);
   LAUNCHCHK();
   return FELIS_OK;
@@ -630,6 +745,7 @@ template <typename T>
 static int launch_zero_slot(felis_ctx *ctx, int phys) {
   GridArgs g = ctx->grid();
   dim3 grid((unsigned)((ctx->N + 255) / 256), 2 * ctx->d.n_harm);
+  KTimer kt(ctx, FELIS_K_SHIFT);
   hipLaunchKernelGGL((k_zero_slot<T>), grid, dim3(256), 0, ctx->stream, (cpx<T> *)ctx->U, phys, g.S,
                      (long long)ctx->N, 2 * ctx->d.n_harm);
   LAUNCHCHK();
@@ -650,11 +766,12 @@ static int launch_deposit(felis_ctx *ctx, const DepArgs &da, double A) {
   size_t fsz = (size_t)2 * d.n_harm * ctx->S * ctx->N * ctx->esize();
   int ez = zero_async(ctx, ctx->Sc, fsz);
   if (ez) return ez;
-  if (ctx->ntiles) {
-    hipLaunchKernelGGL((k_deposit<R, T>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, g, da, ctx->tiles(),
-                       ctx->theta, (const R *)ctx->gamma, (const R *)ctx->x, (const R *)ctx->y,
+  if (ctx->nchunks) {
+    KTimer kt(ctx, FELIS_K_DEPOSIT);
+    hipLaunchKernelGGL((k_deposit_win<R, T>), dim3(ctx->nchunks), dim3(512), ctx->lds_max, ctx->stream, g, da,
+                       ctx->chunks(), ctx->theta, (const R *)ctx->gamma, (const R *)ctx->x, (const R *)ctx->y,
                        (const R *)ctx->px, (const R *)ctx->py, ctx->seg_w, ctx->bessel, (cpx<T> *)ctx->Sc,
-                       (cpx<T> *)((char *)ctx->Sc + ctx->pol_bytes()), ctx->errw);
+                       (cpx<T> *)((char *)ctx->Sc + ctx->pol_bytes()), ctx->lds_max, ctx->errw);
     LAUNCHCHK();
   }
   // radial space charge: only the last species survives (solveField.cpp:246-400)
@@ -664,13 +781,17 @@ static int launch_deposit(felis_ctx *ctx, const DepArgs &da, double A) {
   int t0 = ctx->species_tile[d.n_species - 1], t1 = ctx->species_tile[d.n_species];
   if (t1 > t0) {
     size_t lds = (size_t)d.sc_l * nrow * sizeof(cpx<double>);
+    KTimer kt(ctx, FELIS_K_EZ_DEPOSIT);
     hipLaunchKernelGGL((k_ez_deposit<R>), dim3(t1 - t0), dim3(kTile), lds, ctx->stream, g, da, ctx->tiles(), t0,
                        ctx->theta, (const R *)ctx->x, (const R *)ctx->y, ctx->seg_w, (cpx<double> *)ctx->ScR);
     LAUNCHCHK();
   }
   int nsys = d.slice_n * d.sc_l * d.sc_f;
-  hipLaunchKernelGGL(k_ez_solve, dim3((nsys + 63) / 64), dim3(64), 0, ctx->stream, g, A,
-                     (const cpx<double> *)ctx->ScR, (cpx<double> *)ctx->Ez, ctx->ez_b, (cpx<double> *)ctx->ez_d);
+  KTimer kt(ctx, FELIS_K_EZ_SOLVE);
+  hipLaunchKernelGGL(k_ez_coef, dim3(1), dim3(64), 0, ctx->stream, g, A, ctx->ez_b);
+  LAUNCHCHK();
+  hipLaunchKernelGGL(k_ez_solve, dim3((nsys + 63) / 64), dim3(64), 0, ctx->stream, g, (const double *)ctx->ez_b,
+                     (const cpx<double> *)ctx->ScR, (cpx<double> *)ctx->Ez, (cpx<double> *)ctx->ez_d);
   LAUNCHCHK();
   return FELIS_OK;
 }
@@ -686,12 +807,16 @@ static int launch_adi(felis_ctx *ctx, int cf) {
   const cpx<T> *U = (const cpx<T> *)ctx->U;
   const cpx<T> *Sc = (const cpx<T> *)ctx->Sc;
   cpx<T> *Uc = (cpx<T> *)ctx->Uc;
+  {
+  KTimer kt(ctx, FELIS_K_ADI_X);
   if (n <= 64) hipLaunchKernelGGL((k_adi_x<T, 1>), gx, dim3(256), 0, ctx->stream, g, c, cf, U, Sc, Uc);
   else if (n <= 128) hipLaunchKernelGGL((k_adi_x<T, 2>), gx, dim3(256), 0, ctx->stream, g, c, cf, U, Sc, Uc);
   else if (n <= 192) hipLaunchKernelGGL((k_adi_x<T, 3>), gx, dim3(256), 0, ctx->stream, g, c, cf, U, Sc, Uc);
   else hipLaunchKernelGGL((k_adi_x<T, 4>), gx, dim3(256), 0, ctx->stream, g, c, cf, U, Sc, Uc);
   LAUNCHCHK();
+  }
   dim3 gy((unsigned)nsys, (unsigned)((n + 127) / 128));
+  KTimer kt(ctx, FELIS_K_ADI_Y);
   hipLaunchKernelGGL((k_adi_y<T>), gy, dim3(128), 0, ctx->stream, g, c, cf, (cpx<T> *)ctx->U, Sc,
                      (const cpx<T> *)Uc);
   LAUNCHCHK();
@@ -725,6 +850,7 @@ template <typename T>
 static int launch_energy(felis_ctx *ctx, double *dev_out) {
   GridArgs g = ctx->grid();
   int nsys = 2 * ctx->d.n_harm * ctx->d.slice_n;
+  KTimer kt(ctx, FELIS_K_ENERGY);
   hipLaunchKernelGGL((k_energy_slices<T>), dim3(nsys), dim3(256), 0, ctx->stream, g, (const cpx<T> *)ctx->U,
                      ctx->slice_e);
   LAUNCHCHK();
@@ -743,6 +869,7 @@ static int launch_bfactor(felis_ctx *ctx, double *dev_out) {
   GridArgs g = ctx->grid();
   int nv = 2 * ctx->d.n_harm + 1;
   if (!ctx->ntiles) return zero_async(ctx, dev_out, nv * sizeof(double));
+  KTimer kt(ctx, FELIS_K_BFACTOR);
   hipLaunchKernelGGL((k_bfactor_tiles<R>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, g, ctx->tiles(),
                      ctx->theta, ctx->seg_w, ctx->tile_red);
   LAUNCHCHK();
@@ -761,6 +888,7 @@ static int launch_twiss(felis_ctx *ctx, double *dev_out) {
   const felis_desc &d = ctx->d;
   int e0 = zero_async(ctx, dev_out, 7 * d.n_species * sizeof(double));
   if (e0 || !ctx->ntiles) return e0;
+  KTimer kt(ctx, FELIS_K_TWISS);
   hipLaunchKernelGGL((k_twiss_tiles<R>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, g, ctx->tiles(),
                      (const R *)ctx->x, (const R *)ctx->y, (const R *)ctx->px, (const R *)ctx->py,
                      (const R *)ctx->gamma, ctx->tile_red);
@@ -805,6 +933,7 @@ int felis_ctx::update_wake(const double *wakeF_host, double *total_cnt) {
   HIPCHK(hipMemcpyAsync(ctx->den, den.data(), sn * sizeof(double), hipMemcpyHostToDevice, stream));
   if (wakeF_host) HIPCHK(hipMemcpyAsync(wakeF, wakeF_host, sn * sizeof(double), hipMemcpyHostToDevice, stream));
   HIPCHK(hipStreamSynchronize(stream));
+  KTimer kt(ctx, FELIS_K_WAKE);
   hipLaunchKernelGGL(k_wake, dim3((sn + 127) / 128), dim3(128), 0, stream, sn, dZ, tot * 1.602e-19, ctx->den, wakeF,
                      wakeE);
   LAUNCHCHK();

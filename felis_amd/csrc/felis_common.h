@@ -78,7 +78,8 @@ __host__ __device__ inline cpx<T> cdiv(cpx<T> a, cpx<T> b) {
   return mk<T>((a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d);
 }
 
-constexpr int kTile = 256;  // particles per tile (one workgroup)
+constexpr int kTile = 256;    // particles per tile (one workgroup)
+constexpr int kChunk = 4096;  // particles per deposit workgroup
 
 // ---------------------------------------------------------------- arguments
 

@@ -53,6 +53,22 @@ struct felis_ctx {
   int *tile_seg = nullptr, *tile_cnt = nullptr;
   long long *tile_start = nullptr;
   double *tile_red = nullptr;
+  // deposit chunks: <= kChunk particles of one segment per workgroup
+  int nchunks = 0;
+  int *chunk_seg = nullptr, *chunk_cnt = nullptr;
+  long long *chunk_start = nullptr;
+  int lds_max = 65536;  // dynamic LDS bytes a workgroup may use
+  felis::Tiles chunks() const { return felis::Tiles{chunk_seg, chunk_start, chunk_cnt}; }
+
+  // per-kernel timing (felis_profile)
+  bool profiling = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_marks;
+  size_t ev_used = 0;
+  double prof_ms[FELIS_K_COUNT] = {};
+  long long prof_n[FELIS_K_COUNT] = {};
+  hipEvent_t ev_get();
+  void ev_flush();
 
   // batched step (felis_run) resources
   hipGraph_t graph = nullptr;
@@ -68,6 +84,7 @@ struct felis_ctx {
 
   int check_err();
   void free_particles();
+  int alloc_particles(const int64_t *seg_off, const double *seg_weight);
   void release_graph() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -86,6 +103,27 @@ struct felis_ctx {
   int bfactor(double *dev_out);
   int twiss(double *dev_out);
   int update_wake(const double *wakeF_host, double *total_cnt);
+};
+
+// Records HIP events around one kernel launch while profiling is on.
+struct KTimer {
+  felis_ctx *c;
+  int id;
+  hipEvent_t a = nullptr;
+  KTimer(felis_ctx *ctx, int kid) : c(ctx), id(kid) {
+    if (c->profiling) {
+      a = c->ev_get();
+      (void)hipEventRecord(a, c->stream);
+    }
+  }
+  ~KTimer() {
+    if (c->profiling && a) {
+      hipEvent_t b = c->ev_get();
+      (void)hipEventRecord(b, c->stream);
+      c->ev_marks.push_back({id, {a, b}});
+      if (c->ev_marks.size() >= 4096) c->ev_flush();
+    }
+  }
 };
 
 int lat_args(felis_ctx *ctx, const felis_lattice *lat, felis::LatArgs *out);

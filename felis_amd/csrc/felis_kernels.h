@@ -397,6 +397,162 @@ __global__ void __launch_bounds__(kTile)
   }
 }
 
+// Per-particle deposit factors of one harmonic (solveField.cpp:274-301):
+// returns etaX*fx*mc and etaY*fy*mc (the corner weights are applied later).
+__device__ inline void deposit_factors(const GridArgs &g, const DepArgs &d, const double *__restrict__ bes, int H,
+                                       double macro, double th, double gam, double xi2, cpx<double> expP,
+                                       cpx<double> &cx, cpx<double> &cy, bool &bad) {
+  const double ka = d.ka;
+  double dbH = (double)H;
+  int ib = (int)((dbH * d.xi) / g.dB);
+  double w1 = (dbH * d.xi / g.dB) - ib, w0 = 1.0 - w1;
+  if (ib < 0 || ib + 1 >= g.bn) {
+    bad = true;
+    cx = cy = mk(0.0, 0.0);
+    return;
+  }
+  cpx<double> fx, fy;
+  if (H % 2 == 1) {
+    double sign = ((H - 1) / 2 % 2 == 0) ? 1.0 : -1.0;
+    double J1 = bessel_interp(bes, g.bo, ib, (H - 1) / 2, w0, w1);
+    double J2 = bessel_interp(bes, g.bo, ib, (H + 1) / 2, w0, w1);
+    fx = mk(sign * (J1 - ka * J2), 0.0);
+    fy = mk(sign * (J1 + ka * J2), 0.0);
+  } else {
+    double sign = (H / 2 % 2 == 0) ? 1.0 : -1.0;
+    double J1 = bessel_interp(bes, g.bo, ib, (H - 2) / 2, w0, w1);
+    double J2 = bessel_interp(bes, g.bo, ib, H / 2, w0, w1);
+    double J3 = bessel_interp(bes, g.bo, ib, (H + 2) / 2, w0, w1);
+    double s = sign * xi2 * dbH * 0.5;
+    cpx<double> expM = conjg(expP);
+    fx = (expM * (J1 - ka * J2) + expP * (J2 - ka * J3)) * s;
+    fy = (expM * (J1 + ka * J2) + expP * (J2 + ka * J3)) * s;
+  }
+  double sp, cp;
+  sincos(-dbH * (th + d.Phi * 0.5), &sp, &cp);
+  cpx<double> mc = mk(cp, sp) * (dbH * macro * d.coef_U / gam);
+  cx = mk(d.etaXr, d.etaXi) * fx * mc;
+  cy = mk(d.etaYr, d.etaYi) * fy * mc;
+}
+
+template <typename T>
+__device__ inline void lds_add(cpx<T> *p, cpx<double> v) {
+  atomicAdd(&p->re, (T)v.re);
+  atomicAdd(&p->im, (T)v.im);
+}
+
+// Deposit with an LDS-privatised window: one workgroup per chunk of a
+// segment (particles of one species in one slice). The window is the
+// bounding box of the chunk's corners; harmonics are taken `hp` at a time so
+// the window fits the LDS. Particles whose corners leave the slice's last
+// row or column (the reference's flat-index spill, A.5) add straight to HBM.
+template <typename R, typename T>
+__global__ void __launch_bounds__(512)
+    k_deposit_win(GridArgs g, DepArgs d, Tiles ch, const double *__restrict__ theta, const R *__restrict__ gamma,
+                  const R *__restrict__ x, const R *__restrict__ y, const R *__restrict__ px,
+                  const R *__restrict__ py, const double *__restrict__ seg_w, const double *__restrict__ bes,
+                  cpx<T> *__restrict__ Scx, cpx<T> *__restrict__ Scy, int lds_bytes, unsigned *err) {
+  extern __shared__ double lds_d[];
+  cpx<T> *win = reinterpret_cast<cpx<T> *>(lds_d);
+  __shared__ int bb[4];
+  const int c = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+  const int seg = ch.seg[c];
+  const int slot = seg % g.slice_n + 1;
+  const long long start = ch.start[c];
+  const int cnt = ch.cnt[c];
+  const double macro = seg_w[seg];
+  if (tid == 0) {
+    bb[0] = 1 << 30; bb[1] = -1; bb[2] = 1 << 30; bb[3] = -1;
+  }
+  __syncthreads();
+  int mi = 1 << 30, Mi = -1, mj = 1 << 30, Mj = -1;
+  for (int k = tid; k < cnt; k += nt) {
+    double xx = x[start + k], yy = y[start + k];
+    int idxI = (int)((xx - g.minX) / g.dx), idxJ = (int)((yy - g.minY) / g.dy);
+    if (idxI >= 0 && idxI < g.nx - 1 && idxJ >= 0 && idxJ < g.ny - 1) {
+      mi = min(mi, idxI); Mi = max(Mi, idxI); mj = min(mj, idxJ); Mj = max(Mj, idxJ);
+    }
+  }
+  atomicMin(&bb[0], mi); atomicMax(&bb[1], Mi); atomicMin(&bb[2], mj); atomicMax(&bb[3], Mj);
+  __syncthreads();
+  const int wi0 = bb[0], wj0 = bb[2];
+  const bool empty = bb[1] < 0;
+  const int ww = empty ? 0 : bb[1] + 2 - wi0, wh = empty ? 0 : bb[3] + 2 - wj0;
+  const int W = ww * wh;
+  int hp = W > 0 ? lds_bytes / (W * 2 * (int)sizeof(cpx<T>)) : g.nh;
+  if (hp > g.nh) hp = g.nh;
+  const cpx<double> etaX = mk(d.etaXr, d.etaXi), etaY = mk(d.etaYr, d.etaYi);
+  (void)etaX; (void)etaY;
+  bool any_even = false;
+  for (int h = 0; h < g.nh; ++h) any_even |= (g.harm[h] % 2 == 0);
+  const int step = hp > 0 ? hp : 1;
+  for (int h0 = 0; h0 < g.nh; h0 += step) {
+    const int hc = min(step, g.nh - h0);
+    if (hp > 0) {
+      for (int q = tid; q < hc * 2 * W; q += nt) win[q] = mk<T>(0, 0);
+      __syncthreads();
+    }
+    bool bad = false;
+    for (int k = tid; k < cnt; k += nt) {
+      long long n = start + k;
+      double xx = x[n], yy = y[n];
+      int idxI = (int)((xx - g.minX) / g.dx), idxJ = (int)((yy - g.minY) / g.dy);
+      if (!(idxI >= 0 && idxI < g.nx && idxJ >= 0 && idxJ < g.ny)) continue;
+      double wx[2], wy[2];
+      wx[1] = (xx - g.minX) / g.dx - idxI; wx[0] = 1.0 - wx[1];
+      wy[1] = (yy - g.minY) / g.dy - idxJ; wy[0] = 1.0 - wy[1];
+      double th = theta[n], gam = (double)gamma[n] + g.gref;
+      double xi2 = 0.0;
+      cpx<double> expP = mk(1.0, 0.0);
+      if (any_even) {
+        double ppx = px[n], ppy = py[n], ue = d.ue, ka = d.ka;
+        xi2 = d.xi2c * sqrt((1 + ka) * (ppx * ppx + ue * ue * ppy * ppy) + (1 - ka) * (ppx * ppx * ue * ue + ppy * ppy));
+        double B = atan(2.0 * ka * ue * (ppx - ppy) / ((1.0 + ka) * (ppx + ue * ue * ppy) + (1 - ka) * (ppx * ue * ue + ppy)));
+        double sb, cb;
+        sincos(B - d.Phi * 0.5, &sb, &cb);
+        expP = mk(cb, sb);
+      }
+      const bool inwin = hp > 0 && idxI < g.nx - 1 && idxJ < g.ny - 1;
+      for (int hh = 0; hh < hc; ++hh) {
+        const int h = h0 + hh;
+        cpx<double> cx, cy;
+        deposit_factors(g, d, bes, g.harm[h], macro, th, gam, xi2, expP, cx, cy, bad);
+#pragma unroll
+        for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+          for (int jj = 0; jj < 2; ++jj) {
+            double w = wx[ii] * wy[jj];
+            if (inwin) {
+              int q = (idxJ + jj - wj0) * ww + (idxI + ii - wi0);
+              lds_add(win + (hh * 2 + 0) * W + q, cx * (-w));
+              lds_add(win + (hh * 2 + 1) * W + q, cy * (-w));
+            } else {
+              long long o = (long long)h * g.S * g.N + (long long)slot * g.N + (long long)idxJ * g.nx + idxI +
+                            jj * g.nx + ii;
+              atomic_add_c(Scx + o, cx * (-w));
+              atomic_add_c(Scy + o, cy * (-w));
+            }
+          }
+      }
+    }
+    if (bad) atomicOr(err, EB_BESSEL);
+    if (hp > 0) {
+      __syncthreads();
+      for (int q = tid; q < hc * 2 * W; q += nt) {
+        cpx<T> v = win[q];
+        if (v.re == (T)0 && v.im == (T)0) continue;
+        int hh = q / (2 * W), pol = (q / W) & 1, r = q % W;
+        int jy = wj0 + r / ww, ix = wi0 + r % ww;
+        cpx<T> *dst = (pol ? Scy : Scx) + (long long)(h0 + hh) * g.S * g.N + (long long)slot * g.N +
+                      (long long)jy * g.nx + ix;
+        unsafeAtomicAdd(&dst->re, v.re);
+        unsafeAtomicAdd(&dst->im, v.im);
+      }
+      __syncthreads();
+    }
+  }
+}
+
 // solveField.cpp:322-359: radial deposit of the last species into the
 // per-slot source vector [slot][l][F*nr + 1] (the spare entry takes the
 // idxR = nr-1 write that spills past the array in the reference).
@@ -459,12 +615,41 @@ __global__ void __launch_bounds__(kTile)
   }
 }
 
-// solveField.cpp:361-397: Thomas solve per (slot, l, f); the back
-// substitution reads slot 0 of Ez (A.3).
-__global__ void __launch_bounds__(64) k_ez_solve(GridArgs g, double A,
+// solveField.cpp:361-391, the field-independent half: forward elimination of
+// the real tridiagonal matrix per (l, f), shared by every slice.
+// coef[(l*F + f)*2*nr + j] = w_j (multiplier), [.. + nr + j] = b'_j.
+__global__ void k_ez_coef(GridArgs g, double A, double *__restrict__ coef) {
+  const int L = g.L, F = g.F, nr = g.nr;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= L * F) return;
+  int f = t % F, ll = t / F;
+  double *w = coef + (long long)t * 2 * nr, *b = w + nr;
+  double aprev = (f == 0) ? 1.0 : 0.0;
+  b[0] = (f == 0) ? -(1.0 + (ll + 1.0) * (ll + 1.0) * A / 8.0) : 1.0;
+  w[0] = 0.0;
+  for (int j = 1; j < nr; ++j) {
+    double aj, bj, cj;
+    if (j < nr - 1) {
+      aj = j + 0.5;
+      bj = -2.0 * j - (ll + 1.0) * (ll + 1.0) * A * j - f * f * log((j + 0.5) / (j - 0.5));
+      cj = j - 0.5;
+    } else {
+      aj = 0.0;
+      bj = 1.0;
+      cj = 0.0;
+    }
+    double wj = cj / b[j - 1];
+    b[j] = bj - wj * aprev;
+    w[j] = wj;
+    aprev = aj;
+  }
+}
+
+// solveField.cpp:387-396 per (slot, l, f): the source recurrence and the back
+// substitution, which reads slot 0 of Ez (A.3).
+__global__ void __launch_bounds__(64) k_ez_solve(GridArgs g, const double *__restrict__ coef,
                                                  const cpx<double> *__restrict__ ScR,
-                                                 cpx<double> *__restrict__ Ez, double *__restrict__ bw,
-                                                 cpx<double> *__restrict__ dw) {
+                                                 cpx<double> *__restrict__ Ez, cpx<double> *__restrict__ dw) {
   const int L = g.L, F = g.F, nr = g.nr;
   int tid = blockIdx.x * blockDim.x + threadIdx.x;
   int nsys = g.slice_n * L * F;
@@ -475,47 +660,19 @@ __global__ void __launch_bounds__(64) k_ez_solve(GridArgs g, double A,
   int slot = sl + 1;
   const int row = F * nr + 1;
   const cpx<double> *S = ScR + ((long long)slot * L + ll) * row;
-  double *b = bw + (long long)tid * nr;
+  const double *w = coef + (long long)(ll * F + f) * 2 * nr, *b = w + nr;
   cpx<double> *d = dw + (long long)tid * nr;
-  double a0, c0;
-  // row 0
-  if (f == 0) {
-    a0 = 1.0;
-    b[0] = -(1.0 + (ll + 1.0) * (ll + 1.0) * A / 8.0);
-    c0 = 0.0;
-    d[0] = S[0];
-  } else {
-    a0 = 0.0;
-    b[0] = 1.0;
-    c0 = 0.0;
-    d[0] = mk(0.0, 0.0);
-  }
-  (void)c0;
-  double aprev = a0;
+  cpx<double> dp = (f == 0) ? S[0] : mk(0.0, 0.0);
+  d[0] = dp;
   for (int j = 1; j < nr; ++j) {
-    double aj, bj, cj;
-    cpx<double> dj;
-    if (j < nr - 1) {
-      aj = j + 0.5;
-      bj = -2.0 * j - (ll + 1.0) * (ll + 1.0) * A * j - f * f * log((j + 0.5) / (j - 0.5));
-      cj = j - 0.5;
-      dj = S[f * nr + j];
-    } else {
-      aj = 0.0;
-      bj = 1.0;
-      cj = 0.0;
-      dj = mk(0.0, 0.0);
-    }
-    double w = cj / b[j - 1];
-    bj -= w * aprev;
-    dj = dj - d[j - 1] * w;
-    b[j] = bj;
-    d[j] = dj;
-    aprev = aj;
+    cpx<double> dj = (j < nr - 1) ? S[f * nr + j] : mk(0.0, 0.0);
+    dp = dj - dp * w[j];
+    d[j] = dp;
   }
   const long long es = (long long)g.S * F * nr;
   cpx<double> *ez = Ez + ll * es;
   ez[f * nr + nr - 1] = mk(0.0, 0.0);
+  const double a0 = (f == 0) ? 1.0 : 0.0;
   for (int j = nr - 2; j >= 0; --j) {
     double aj = (j == 0) ? a0 : j + 0.5;
     cpx<double> v = d[j] - ez[f * nr + j + 1] * aj;

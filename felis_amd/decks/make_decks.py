@@ -1,0 +1,165 @@
+"""Writes the input decks of the benchmark configurations (reference deck format).
+
+    python felis_amd/decks/make_decks.py      # regenerates felis_amd/decks/*.inp
+
+Common beam and undulator: gamma0 = 8000, energy spread 1e-4, sigma_x,y =
+30 um at 0.4 mm mrad (beta = 18 m), planar undulator lambda_u = 2.6 cm,
+K = 1.87, Gaussian seed of 10 kW with a 40 um waist (spot_sigma_R = 20 um,
+waist = 2 sigma_R), laser_alpha = 1 so that the seed couples (a planar
+K0_alpha = 1 undulator reads Ux only). The photon energy is the resonance of
+gamma0; the slice window is (N + 1/2) wavelengths so int() gives N slices.
+
+  cfg0  Static, 1 slice, 81x81, 2048 beamlets x 4 phases, 4 m
+  cfg1  Time_Dependent, 256 slices, 101x101, 1024 x 4 per slice, flat 3 kA, 10 m
+  cfg2  Time_Dependent, 2048 slices, 129x129, harmonics 1,3,5, Gaussian
+        current (sigma 400 slices, 4 kA), max(32, int(2048 n0)) x 8, 20 m
+  cfg2s the first 256 slices' worth of cfg2 physics (parity-size cut)
+"""
+from __future__ import annotations
+
+import math
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+MC2 = 0.510998918
+PLANK_H = 4.135667696e-15
+C_LIGHT = 299792458.0
+GAMMA0 = 8000.0
+K0 = 1.87
+LAMBDA_U = 0.026
+LAMBDA0 = LAMBDA_U / (2 * GAMMA0 ** 2) * (1 + K0 ** 2 / 2)
+PHOTON = PLANK_H * C_LIGHT / LAMBDA0
+EMIT = 0.4          # mm mrad
+SIG = 30e-6
+BETA = SIG ** 2 * GAMMA0 / (EMIT * 1e-6)
+
+
+def deck(*, mode: str, slices: int, n: int, harmonics, length: float, beam: str, extra_undulator: str = "",
+         wake: str = "activate = OFF\n radius = 4.5e-3") -> str:
+    maxz_um = (slices + 0.5) * LAMBDA0 * 1e6
+    harm = "\n".join(f" harmony{i} = {h}" for i, h in enumerate(harmonics))
+    return f"""// generated by felis_amd/decks/make_decks.py
+[Save]
+ field_save = OFF
+ particle_save = OFF
+ total_length = {length}   // [m]
+ save_step = 100000
+ save_start = 100000
+
+[Domain]
+ dimension = 3
+ mode = {mode}
+ photon_energy = {PHOTON:.12f}   // [eV], resonance of gamma0
+ minZ = 0            // [um]
+ maxZ = {maxz_um:.12f}   // [um]
+ minX = -300         // [um]
+ maxX = 300
+ minY = -300
+ maxY = 300
+ nx = {n - 1}
+ ny = {n - 1}
+ num_harmony = {len(harmonics)}
+{harm}
+ slices_in_bucket = 1
+ lambdaUs_in_iteration = 1
+ ABC_N = 10
+ ABC_coef = 1
+
+[Seed]
+ power = 1e4          // [W]
+ spot_sigma_R = 20    // [um]
+ rms_duration = 1e6   // [fs] (flat over the window)
+ focus = 0            // [m]
+ laser_alpha = 1
+
+[Undulator]
+ undulator_type = BiPolar
+ polarity = 0
+ K0_alpha = 1
+ numbers = 1
+ lambdaU = 2.6        // [cm]
+ K0 = {K0}
+ unit_start = 0
+ unit_end = {length + 1.0}
+ undulator_start = 0
+ undulator_end = {length + 1.0}
+{extra_undulator}
+[Wake_field]
+ {wake}
+
+[EBeam]
+ beam_energy = {(GAMMA0 - 1) * MC2:.9f}   // [MeV]
+ energy_spread = 0.01     // [%]
+ norm_emittance_x = {EMIT}
+ norm_emittance_y = {EMIT}
+ beta_x = {BETA:.9f}
+ beta_y = {BETA:.9f}
+ alpha_x = 0
+ alpha_y = 0
+{beam}
+"""
+
+
+def flat_beam(current: float, beamlets: int, phases: int) -> str:
+    return f""" load_type = Polygon
+ noise_ONOFF = ON
+ peak_current = {current}
+ beamlets_in_bucket = {beamlets}
+ number_in_beamlet = {phases}
+ z_nodes = 2
+ z0 = -1
+ z1 = 1
+ z_n0 = 1
+ z_n1 = 1
+ energy_nodes = 2
+ energy_z0 = -1
+ energy_z1 = 1
+ energy_n0 = 1
+ energy_n1 = 1
+ energySpread_nodes = 2
+ energySpread_z0 = -1
+ energySpread_z1 = 1
+ energySpread_n0 = 1
+ energySpread_n1 = 1
+ emit_nodes = 2
+ emit_z0 = -1
+ emit_z1 = 1
+ emit_n0 = 1
+ emit_n1 = 1
+"""
+
+
+def gauss_beam(current: float, beamlets: int, phases: int, center: int, sigma: int, floor: int) -> str:
+    return f""" load_type = Gaussian
+ noise_ONOFF = ON
+ peak_current = {current}
+ beamlets_in_bucket = {beamlets}
+ number_in_beamlet = {phases}
+ min_beamlets = {floor}
+ sigma_z = {sigma * LAMBDA0:.15e}      // [m]
+ position_z = {center * LAMBDA0:.15e}  // [m]
+ gaussian_power = 2
+ energy_chirp = 0
+"""
+
+
+DECKS = {
+    "cfg0.inp": dict(mode="Static", slices=1, n=81, harmonics=(1,), length=4.0,
+                     beam=flat_beam(3000, 2048, 4)),
+    "cfg1.inp": dict(mode="Time_Dependent", slices=256, n=101, harmonics=(1,), length=10.0,
+                     beam=flat_beam(3000, 1024, 4)),
+    "cfg2.inp": dict(mode="Time_Dependent", slices=2048, n=129, harmonics=(1, 3, 5), length=20.0,
+                     beam=gauss_beam(4000, 2048, 8, 1024, 400, 32)),
+    "cfg2s.inp": dict(mode="Time_Dependent", slices=64, n=129, harmonics=(1, 3, 5), length=20.0,
+                      beam=gauss_beam(4000, 256, 8, 32, 12, 8)),
+}
+
+
+def main():
+    for name, kw in DECKS.items():
+        (HERE / name).write_text(deck(**kw))
+        print("wrote", HERE / name)
+
+
+if __name__ == "__main__":
+    main()

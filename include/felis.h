@@ -144,6 +144,51 @@ int felis_calculate_twiss(felis_ctx *ctx, double *sums /* [n_species][7] */);
 int felis_run(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps, int wake_step,
               const double *wakeF, double *energy, double *bfac, double *twiss);
 
+/* -------- device ------------------------------------------------------------ */
+/* info[8]: compute units, max LDS bytes per workgroup, LDS bytes per CU,
+ * wavefront size, L2 bytes, clock kHz, memory clock kHz, memory bus width. */
+int felis_device_info(int *info);
+
+/* -------- per-kernel timing ------------------------------------------------ */
+/* Kernels timed with HIP events on the context's stream while profiling is
+ * on (the reference has wall-clock minutes only, main.cpp:95-100). */
+enum felis_kernel_id {
+  FELIS_K_TRANSVERSE = 0, FELIS_K_PUSH, FELIS_K_DRIFT, FELIS_K_PERIODIC, FELIS_K_DEPOSIT,
+  FELIS_K_EZ_DEPOSIT, FELIS_K_EZ_SOLVE, FELIS_K_ADI_X, FELIS_K_ADI_Y, FELIS_K_SHIFT,
+  FELIS_K_ENERGY, FELIS_K_BFACTOR, FELIS_K_TWISS, FELIS_K_WAKE, FELIS_K_ZERO,
+  FELIS_K_PARTICLE_STEP, FELIS_K_ADI_FUSED, FELIS_K_COUNT
+};
+/* enable = 1 clears the counters and starts timing, 0 stops it. */
+int felis_profile(felis_ctx *ctx, int enable);
+/* Summed milliseconds and launch counts per felis_kernel_id (FELIS_K_COUNT each). */
+int felis_profile_read(felis_ctx *ctx, double *ms, int64_t *launches);
+
+/* -------- input deck and initial state (host side of the reference) ----- */
+typedef struct felis_deck felis_deck;
+/* findparam.cpp:30-116: value of key in the rank-th [block] of file into out.
+ * Returns 1 found, 0 not found, negative on error. */
+int felis_find_parameter(const char *block, int rank, const char *key, const char *file, char *out, int cap);
+/* parameterSetting.cpp:42-569: parses a deck and derives the Domain scalars.
+ * *out is set even on failure so that felis_deck_error can explain. */
+int felis_deck_open(const char *path, felis_deck **out);
+int felis_deck_close(felis_deck *deck);
+const char *felis_deck_error(const felis_deck *deck);
+int felis_deck_desc(const felis_deck *deck, int precision, felis_desc *out);
+/* info[8]: maxStep, wake update_step, save_step, save_start, field_save,
+ * particle_save, mode, sliceN */
+int felis_deck_steps(const felis_deck *deck, int *info);
+/* lattice table for felis_run from the deck's initial Domain state
+ * (updateK_quadG.cpp:8-94 at (it,0) and (it,0.5) for it = it0..it0+n-1). */
+int felis_deck_lattice(const felis_deck *deck, int it0, int n_steps, felis_lattice *tab);
+/* wakeField.cpp:53-159: wake function wakeF[sliceN]; impZ[1001] may be NULL. */
+int felis_deck_wake_function(const felis_deck *deck, double *wakeF, double *impZ);
+/* loadBeam.cpp:94-157: particle counts and weights per (species, slice). */
+int felis_deck_beam_layout(const felis_deck *deck, int64_t *seg_off, double *seg_weight);
+/* loadBeam.cpp:28-269 generated on the device; seed keys the shot noise. */
+int felis_deck_load_beam(const felis_deck *deck, felis_ctx *ctx, uint64_t seed);
+/* loadSeed.cpp:25-80 generated on the device. */
+int felis_deck_load_seed(const felis_deck *deck, felis_ctx *ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -255,13 +255,33 @@ int orc_push_theta_gamma_3d(orc_state *st, const orc_lattice *lat, int iteration
   const double coef[5] = {0, 0, 0.5, 0.5, 1};
   size_t fs = field_stride(p);
   size_t es = (size_t)(p->slice_n + 2) * F * nr;
-  cplx Em[16];
-  for (int l = 0; l < L; ++l) Em[l] = 0;
-  cplx Ux[ORC_MAX_HARM], Uy[ORC_MAX_HARM];
   int err = ORC_OK;
-  double wakeE = 0.0;
-  for (int s = 0; s < p->n_species; ++s) {
-    for (int sl = 0; sl < p->slice_n; ++sl) {
+  /* Slices are independent unless an in-grid particle has idxR >= nr, where
+   * the reference carries Em over from the previous particle: then the walk
+   * is serial, otherwise the slices run in parallel. */
+  int nseg = p->n_species * p->slice_n;
+  int carry = 0;
+  if (g_threads > 1) {
+    int64_t np = st->seg_off[nseg];
+#pragma omp parallel for num_threads(g_threads) reduction(| : carry) schedule(static)
+    for (int64_t n = 0; n < np; ++n) {
+      double x0 = st->x[n], y0 = st->y[n];
+      int idxI = (int)((x0 - minX) / dx), idxJ = (int)((y0 - minY) / dy);
+      if (idxI >= 0 && idxI < nx - 1 && idxJ >= 0 && idxJ < ny - 1 &&
+          (int)(sqrt(x0 * x0 + y0 * y0) / dr + 0.5) >= nr)
+        carry = 1;
+    }
+  }
+  int nt = (g_threads > 1 && !carry) ? g_threads : 1;
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 1) reduction(min : err)
+  for (int seg = 0; seg < nseg; ++seg) {
+    int s = seg / p->slice_n, sl = seg % p->slice_n;
+    static __thread cplx Em[16];
+    if (seg == 0 || nt > 1)
+      for (int l = 0; l < L; ++l) Em[l] = 0;
+    cplx Ux[ORC_MAX_HARM], Uy[ORC_MAX_HARM];
+    double wakeE = 0.0;
+    {
       int slot = sl + 1;
       if (p->wake_on) wakeE = st->wakeE[sl] / (K_MC2 * 1.0e6);
       int64_t b = st->seg_off[(size_t)s * p->slice_n + sl];
@@ -332,7 +352,7 @@ int orc_push_theta_gamma_3d(orc_state *st, const orc_lattice *lat, int iteration
             double w[2];
             int idx = (int)((1.0 * H * xi) / dB);
             w[1] = (H * xi / dB) - idx; w[0] = 1.0 - w[1];
-            if (idx < 0 || idx + 1 >= p->bn) return ORC_ERR_ARG;
+            if (idx < 0 || idx + 1 >= p->bn) { err = ORC_ERR_ARG; goto seg_done; }
             cplx fx, fy;
             if (H % 2 == 1) {
               double sign = ((H - 1) / 2 % 2 == 0) ? 1.0 : -1.0;
@@ -356,12 +376,13 @@ int orc_push_theta_gamma_3d(orc_state *st, const orc_lattice *lat, int iteration
           kg[m] = ks * K0 * sqrt(ue * ue + 1.0) * sumG + e_mc2 * sumEz;
         }
         double dth = dz / 6.0 * (kth[1] + 2.0 * kth[2] + 2.0 * kth[3] + kth[4]);
-        if (dth > dPhi || dth < -dPhi) { err = ORC_ERR_GUARD; return err; }
+        if (dth > dPhi || dth < -dPhi) { err = ORC_ERR_GUARD; goto seg_done; }
         st->theta[n] += dth;
         double dgam = dz / 6.0 * (kg[1] + 2.0 * kg[2] + 2.0 * kg[3] + kg[4]) - dz * wakeE;
         st->gamma[n] += dgam;
       }
     }
+  seg_done:;
   }
   return err;
 }
