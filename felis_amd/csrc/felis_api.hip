@@ -260,7 +260,7 @@ extern "C" int felis_create(const felis_desc *desc, felis_ctx **out) {
   const felis_desc &d = *desc;
   if (d.dimension != 3) return FELIS_ERR_UNSUPPORTED;
   if (d.nx < 6 || d.ny < 6 || d.slice_n < 1 || d.n_harm < 1 || d.n_harm > FELIS_MAX_HARM ||
-      d.num_slice < 1 || d.n_species < 1 || d.nr < 3 || d.sc_l < 1 || d.sc_l > 4 || d.sc_f < 1 ||
+      d.num_slice < 1 || d.n_species < 1 || d.nr < 3 || d.nr > 256 || d.sc_l < 1 || d.sc_l > 4 || d.sc_f < 1 ||
       d.bessel_n < 2 || d.abc_n < 1 || d.nx > 64 * 4 + 2 || d.ny > 64 * 4 + 2)
     return FELIS_ERR_ARG;
   if (d.precision != FELIS_PLAIN_FP64 && d.precision != FELIS_MIXED_FP32) return FELIS_ERR_ARG;
@@ -310,6 +310,7 @@ extern "C" int felis_create(const felis_desc *desc, felis_ctx **out) {
       (e = dalloc(ctx, &ctx->wakeF, (size_t)d.slice_n * sizeof(double))) ||
       (e = dalloc(ctx, &ctx->den, (size_t)d.slice_n * sizeof(double))) ||
       (e = dalloc(ctx, &ctx->slice_e, (size_t)nsys * sizeof(double))) ||
+      (e = dalloc(ctx, &ctx->win, (size_t)ctx->S * sizeof(int4))) ||
       (e = dalloc(ctx, &ctx->red_out, (size_t)1024 * sizeof(double))) ||
       (e = dalloc(ctx, &ctx->errw, sizeof(unsigned)))) {
     felis_destroy(ctx);
@@ -786,14 +787,49 @@ static int launch_deposit(felis_ctx *ctx, const DepArgs &da, double A) {
                        ctx->theta, (const R *)ctx->x, (const R *)ctx->y, ctx->seg_w, (cpx<double> *)ctx->ScR);
     LAUNCHCHK();
   }
+  (void)A;
+  return FELIS_OK;
+}
+
+// solveField.cpp:361-397 for every slice from the radial source in ScR,
+// which is cleared afterwards for the next deposit.
+int felis_ctx::ez_solve(const felis_lattice *lat) {
+  felis_ctx *ctx = this;
+  GridArgs g = grid();
+  double K0 = lat->K0, ue = lat->ue;
+  double A = d.ks * d.ks * (1.0 + 0.5 * (1.0 + ue * ue) * K0 * K0) / (d.gamR * d.gamR) * d.dr * d.dr;
   int nsys = d.slice_n * d.sc_l * d.sc_f;
   KTimer kt(ctx, FELIS_K_EZ_SOLVE);
-  hipLaunchKernelGGL(k_ez_coef, dim3(1), dim3(64), 0, ctx->stream, g, A, ctx->ez_b);
+  // the elimination coefficients depend on the lattice only through A
+  if (A != ez_A) {
+    hipLaunchKernelGGL(k_ez_coef, dim3((unsigned)((d.sc_l * d.sc_f + 63) / 64)), dim3(64), 0, stream, g, A, ez_b);
+    LAUNCHCHK();
+    ez_A = A;
+  }
+  const int E = (d.nr + 63) / 64;
+  const cpx<double> *src = ScR;
+  cpx<double> *ez = Ez;
+  switch (E) {
+    case 1: hipLaunchKernelGGL((k_ez_solve<1>), dim3(nsys), dim3(64), 0, stream, g, ez_b, src, ez); break;
+    case 2: hipLaunchKernelGGL((k_ez_solve<2>), dim3(nsys), dim3(64), 0, stream, g, ez_b, src, ez); break;
+    case 3: hipLaunchKernelGGL((k_ez_solve<3>), dim3(nsys), dim3(64), 0, stream, g, ez_b, src, ez); break;
+    case 4: hipLaunchKernelGGL((k_ez_solve<4>), dim3(nsys), dim3(64), 0, stream, g, ez_b, src, ez); break;
+    default:
+      err = "number of radial grid points above 256";
+      return FELIS_ERR_UNSUPPORTED;
+  }
   LAUNCHCHK();
-  hipLaunchKernelGGL(k_ez_solve, dim3((nsys + 63) / 64), dim3(64), 0, ctx->stream, g, (const double *)ctx->ez_b,
-                     (const cpx<double> *)ctx->ScR, (cpx<double> *)ctx->Ez, (cpx<double> *)ctx->ez_d);
+  hipLaunchKernelGGL(k_ez_slot0, dim3(1), dim3(64), 0, stream, g, ez);
   LAUNCHCHK();
   return FELIS_OK;
+}
+
+// The deposit half of solve_Sc_3D (solveField.cpp:187-359) without the solve.
+int felis_ctx::deposit_only(const felis_lattice *lat) {
+  DepArgs da;
+  int e = dep_args(this, lat, &da);
+  if (e) return e;
+  return mixed() ? launch_deposit<float, float>(this, da, 0.0) : launch_deposit<double, double>(this, da, 0.0);
 }
 
 template <typename T>
@@ -835,13 +871,9 @@ int felis_ctx::solve_field(const felis_lattice *lat) {
         if (e) return e;
       }
   } else {
-    DepArgs da;
-    int e = dep_args(ctx, lat, &da);
+    int e = deposit_only(lat);
     if (e) return e;
-    double K0 = lat->K0, ue = lat->ue;
-    double A = d.ks * d.ks * (1.0 + 0.5 * (1.0 + ue * ue) * K0 * K0) / (d.gamR * d.gamR) * d.dr * d.dr;
-    e = mixed() ? launch_deposit<float, float>(ctx, da, A) : launch_deposit<double, double>(ctx, da, A);
-    if (e) return e;
+    if ((e = ez_solve(lat))) return e;
   }
   return mixed() ? launch_adi<float>(ctx, lat->current_flag) : launch_adi<double>(ctx, lat->current_flag);
 }

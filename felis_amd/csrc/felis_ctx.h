@@ -38,6 +38,9 @@ struct felis_ctx {
   void *U = nullptr, *Sc = nullptr, *Uc = nullptr;
   felis::cpx<double> *Ez = nullptr, *ScR = nullptr, *ez_d = nullptr;
   double *ez_b = nullptr;
+  double ez_A = -1.0;  // A of the coefficients held in ez_b (negative: none)
+  // per slot: interior cells [x0..x1] x [y0..y1] the last batched deposit wrote
+  int4 *win = nullptr;
   felis::cpx<double> *abc = nullptr, *adix = nullptr, *adiy = nullptr;
   double *bessel = nullptr, *wakeE = nullptr, *wakeF = nullptr, *den = nullptr;
   double *slice_e = nullptr, *red_out = nullptr;
@@ -103,6 +106,8 @@ struct felis_ctx {
   int bfactor(double *dev_out);
   int twiss(double *dev_out);
   int update_wake(const double *wakeF_host, double *total_cnt);
+  int deposit_only(const felis_lattice *lat);
+  int ez_solve(const felis_lattice *lat);
 };
 
 // Records HIP events around one kernel launch while profiling is on.

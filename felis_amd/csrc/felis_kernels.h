@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(kTile)
 }
 
 // rearrangeParticles.cpp:77-114
-__global__ void __launch_bounds__(256) k_periodic(long long np, double dTh,
+static __global__ void __launch_bounds__(256) k_periodic(long long np, double dTh,
                                                   double *__restrict__ theta, unsigned *err) {
   long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= np) return;
@@ -618,7 +618,7 @@ __global__ void __launch_bounds__(kTile)
 // solveField.cpp:361-391, the field-independent half: forward elimination of
 // the real tridiagonal matrix per (l, f), shared by every slice.
 // coef[(l*F + f)*2*nr + j] = w_j (multiplier), [.. + nr + j] = b'_j.
-__global__ void k_ez_coef(GridArgs g, double A, double *__restrict__ coef) {
+static __global__ void k_ez_coef(GridArgs g, double A, double *__restrict__ coef) {
   const int L = g.L, F = g.F, nr = g.nr;
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= L * F) return;
@@ -647,36 +647,69 @@ __global__ void k_ez_coef(GridArgs g, double A, double *__restrict__ coef) {
 
 // solveField.cpp:387-396 per (slot, l, f): the source recurrence and the back
 // substitution, which reads slot 0 of Ez (A.3).
+// One wavefront per (slice, l, f). The forward elimination of the source,
+// d_j = S_j - w_j d_{j-1}, is a first-order linear recurrence solved with a
+// wave-wide scan of affine maps (lane owns rows lane*E .. lane*E+E-1); the
+// back substitution only reads slot 0 of Ez (A.3), so it is elementwise.
+template <int E>
 __global__ void __launch_bounds__(64) k_ez_solve(GridArgs g, const double *__restrict__ coef,
                                                  const cpx<double> *__restrict__ ScR,
-                                                 cpx<double> *__restrict__ Ez, cpx<double> *__restrict__ dw) {
+                                                 cpx<double> *__restrict__ Ez) {
   const int L = g.L, F = g.F, nr = g.nr;
-  int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  int nsys = g.slice_n * L * F;
-  if (tid >= nsys) return;
-  int f = tid % F;
-  int ll = (tid / F) % L;
-  int sl = tid / (F * L);
-  int slot = sl + 1;
+  const int sid = blockIdx.x, lane = threadIdx.x;
+  const int f = sid % F, ll = (sid / F) % L, sl = sid / (F * L), slot = sl + 1;
   const int row = F * nr + 1;
   const cpx<double> *S = ScR + ((long long)slot * L + ll) * row;
-  const double *w = coef + (long long)(ll * F + f) * 2 * nr, *b = w + nr;
-  cpx<double> *d = dw + (long long)tid * nr;
-  cpx<double> dp = (f == 0) ? S[0] : mk(0.0, 0.0);
-  d[0] = dp;
-  for (int j = 1; j < nr; ++j) {
-    cpx<double> dj = (j < nr - 1) ? S[f * nr + j] : mk(0.0, 0.0);
-    dp = dj - dp * w[j];
-    d[j] = dp;
-  }
+  const double *w = coef + (long long)(ll * F + f) * 2 * nr, *bp = w + nr;
   const long long es = (long long)g.S * F * nr;
   cpx<double> *ez = Ez + ll * es;
-  ez[f * nr + nr - 1] = mk(0.0, 0.0);
+  double av[E];
+  cpx<double> bv[E];
+  double A = 1.0;
+  cpx<double> B = mk(0.0, 0.0);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int j = lane * E + e;
+    if (j < nr) {
+      av[e] = j == 0 ? 0.0 : -w[j];
+      bv[e] = j == 0 ? (f == 0 ? S[0] : mk(0.0, 0.0)) : (j < nr - 1 ? S[f * nr + j] : mk(0.0, 0.0));
+    } else {
+      av[e] = 1.0;
+      bv[e] = mk(0.0, 0.0);
+    }
+    B = B * av[e] + bv[e];
+    A = A * av[e];
+  }
+#pragma unroll
+  for (int dd = 1; dd < 64; dd <<= 1) {
+    const double Ap = __shfl_up(A, dd);
+    const double Br = __shfl_up(B.re, dd), Bi = __shfl_up(B.im, dd);
+    if (lane >= dd) {
+      B = mk(B.re + A * Br, B.im + A * Bi);
+      A = A * Ap;
+    }
+  }
+  const double pr = __shfl_up(B.re, 1), pi = __shfl_up(B.im, 1);
+  cpx<double> dprev = lane == 0 ? mk(0.0, 0.0) : mk(pr, pi);
   const double a0 = (f == 0) ? 1.0 : 0.0;
-  for (int j = nr - 2; j >= 0; --j) {
-    double aj = (j == 0) ? a0 : j + 0.5;
-    cpx<double> v = d[j] - ez[f * nr + j + 1] * aj;
-    ez[(long long)slot * F * nr + f * nr + j] = mk(v.re / b[j], v.im / b[j]);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int j = lane * E + e;
+    dprev = dprev * av[e] + bv[e];
+    if (j <= nr - 2) {
+      const double aj = (j == 0) ? a0 : j + 0.5;
+      // slot 0 of Ez, whose last entry the reference clears before reading
+      const cpx<double> z = j + 1 == nr - 1 ? mk(0.0, 0.0) : ez[f * nr + j + 1];
+      const cpx<double> v = dprev - z * aj;
+      ez[(long long)slot * F * nr + f * nr + j] = mk(v.re / bp[j], v.im / bp[j]);
+    }
+  }
+}
+
+static __global__ void k_ez_slot0(GridArgs g, cpx<double> *__restrict__ Ez) {
+  for (int t = threadIdx.x; t < g.L * g.F; t += blockDim.x) {
+    const int ll = t / g.F, f = t % g.F;
+    Ez[(long long)ll * g.S * g.F * g.nr + f * g.nr + g.nr - 1] = mk(0.0, 0.0);
   }
 }
 
@@ -835,7 +868,7 @@ __global__ void __launch_bounds__(128) k_adi_y(GridArgs g, AdiCoef c, int cf, cp
 
 // ---------------------------------------------------------------- slippage
 
-__global__ void k_fill_u32(unsigned *p, long long n, unsigned v) {
+static __global__ void k_fill_u32(unsigned *p, long long n, unsigned v) {
   for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += (long long)gridDim.x * blockDim.x)
     p[k] = v;
@@ -884,7 +917,7 @@ __global__ void __launch_bounds__(256) k_energy_slices(GridArgs g, const cpx<T> 
 }
 
 // rows of a [nrow][ncol] matrix summed in column order by one wave per row
-__global__ void __launch_bounds__(64) k_sum_rows(const double *__restrict__ in, int ncol, int stride_row,
+static __global__ void __launch_bounds__(64) k_sum_rows(const double *__restrict__ in, int ncol, int stride_row,
                                                  int stride_col, double *__restrict__ out) {
   int r = blockIdx.x;
   double acc = 0.0;
@@ -948,7 +981,7 @@ __global__ void __launch_bounds__(kTile) k_twiss_tiles(GridArgs g, Tiles tl, con
 
 // wakeField.cpp:28-35: wakeE[ii] = sum_{i>=ii} den[i] wakeF[i-ii] dZ coef,
 // accumulated in the reference's order (i descending) without contraction.
-__global__ void k_wake(int sn, double dZ, double coef, const double *__restrict__ den,
+static __global__ void k_wake(int sn, double dZ, double coef, const double *__restrict__ den,
                        const double *__restrict__ wakeF, double *__restrict__ wakeE) {
   int ii = blockIdx.x * blockDim.x + threadIdx.x;
   if (ii >= sn) return;

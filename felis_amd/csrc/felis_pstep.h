@@ -1,0 +1,582 @@
+// Particle half of the batched step (felis_run, FP32 with FP64 phase).
+//
+//   k_pstep        one workgroup per tile (<= 256 particles of one segment):
+//                  transversePush x2 (particlePush.cpp:32-117), push_theta_gamma
+//                  (198-386) or drift (119-195), the theta wrap
+//                  (rearrangeParticles.cpp:77-114) and the moments of the next
+//                  step's bFactor and twiss (saveFile.cpp:124-154,
+//                  twiss.cpp:34-57) as per-tile FP64 partial sums.
+//   k_deposit_slot one workgroup per slice: the next solveField's bunching
+//                  deposit (solveField.cpp:246-312) into an LDS window over the
+//                  slice's beam, stored once, and the radial space-charge
+//                  source of the last species (solveField.cpp:322-359).
+//   k_reduce_diag  the step's diagnostics from the partial sums, in a fixed
+//                  order.
+#pragma once
+
+#include "felis_common.h"
+#include "felis_kernels.h"
+
+namespace felis {
+
+constexpr int kRun = 8;  // consecutive particles one deposit thread combines
+
+struct PStepArgs {
+  LatArgs l0, l1;  // transversePush at (it,0); everything else at (it,0.5)
+  int drift;       // driftFlag of (it,0.5)
+  int nred;        // partial-sum row length: 2*nh + 1 + 7
+  double gamR;
+};
+
+template <typename T>
+__device__ inline cpx<T> cpow_int(cpx<T> b, int n) {
+  cpx<T> r = mk<T>(1, 0);
+  while (n) {
+    if (n & 1) r = r * b;
+    b = b * b;
+    n >>= 1;
+  }
+  return r;
+}
+
+// psi mod 2pi in FP64, then e^{i psi} in FP32
+__device__ inline cpx<float> expi_f32(double psi) {
+  psi -= 6.283185307179586 * floor(psi * 0.15915494309189535);
+  float s, c;
+  sincosf((float)psi, &s, &c);
+  return mk(c, s);
+}
+
+// transversePush of one particle in FP32 (particlePush.cpp:76-111)
+__device__ inline void transverse_f32(const LatArgs &l, float dz, float gam, float &x, float &y, float &px,
+                                      float &py) {
+  const float ku = (float)l.ku, K0 = (float)l.K0, ue = (float)l.ue;
+  const float aa = K0 * K0 * 0.25f * ue * ku;
+  const float bb = K0 * K0 * 0.25f * (1.0f + ue * ue) * ku * ku;
+  const float cc = (float)(kECharge * l.g / kEMass / kC);
+  const int ka = (int)l.K0_alpha;
+  float xCoef = 0.f, yCoef = 0.f;
+  if (l.und_type == 1) {
+    if (ka == 1) xCoef = 1.f;
+    else if (ka == -1) yCoef = 1.f;
+  } else {
+    xCoef = 1.f;
+    yCoef = 1.f;
+  }
+  const float ig = 1.0f / gam, ig2 = ig * ig;
+  const float dd = ku * K0 * K0 * 0.25f * ig2 * ue;
+  const float fx = bb * ig * xCoef + cc, fy = bb * ig * yCoef - cc;
+  float kx = 0, ky = 0, kpx = 0, kpy = 0, sx = 0, sy = 0, spx = 0, spy = 0;
+  const float cm[5] = {0, 0, 0.5f, 0.5f, 1};
+  const float wt[5] = {0, 1, 2, 2, 1};
+#pragma unroll
+  for (int m = 1; m < 5; ++m) {
+    float xx = x + dz * kx * cm[m], yy = y + dz * ky * cm[m];
+    float ppx = px + dz * kpx * cm[m], ppy = py + dz * kpy * cm[m];
+    kx = ppx * ig - aa * (ka * xx + yy) * ig2;
+    ky = ppy * ig + aa * (ka * xx - yy) * ig2;
+    kpx = ka * (ppx - ppy) * dd - fx * xx;
+    kpy = ka * (ppx + ppy) * dd - fy * yy;
+    sx += wt[m] * kx;
+    sy += wt[m] * ky;
+    spx += wt[m] * kpx;
+    spy += wt[m] * kpy;
+  }
+  x += dz / 6.0f * sx;
+  y += dz / 6.0f * sy;
+  px += dz / 6.0f * spx;
+  py += dz / 6.0f * spy;
+}
+
+// push_theta_gamma_3D of one particle (particlePush.cpp:260-379): the field
+// coupling in FP32, the slippage phase ku - ks/(2 gamma^2) K2, gamma and
+// theta in FP64. One sincos per RK stage; harmonics by complex powers.
+template <int NH>
+__device__ inline void push_tg_f32(const GridArgs &g, const LatArgs &l, int slot, double wakeE,
+                                   const cpx<float> *__restrict__ Ux, const cpx<float> *__restrict__ Uy,
+                                   const cpx<double> *__restrict__ Ez, const double *__restrict__ bes, float x0,
+                                   float y0, float px, float py, double &th, double &gam, unsigned *err) {
+  const double ku = l.ku, ks = g.ks, K0 = l.K0, ue = l.ue;
+  const float ka = (float)l.K0_alpha;
+  const double dz = g.dz;
+  const double xd = x0, yd = y0;
+  const int idxI = (int)((xd - g.minX) / g.dx);
+  const int idxJ = (int)((yd - g.minY) / g.dy);
+  if (!(idxI >= 0 && idxI < g.nx - 1 && idxJ >= 0 && idxJ < g.ny - 1)) return;
+  const float wx1 = (float)((xd - g.minX) / g.dx - idxI), wx0 = 1.f - wx1;
+  const float wy1 = (float)((yd - g.minY) / g.dy - idxJ), wy0 = 1.f - wy1;
+  const float r2 = x0 * x0 + y0 * y0, pr2 = px * px + py * py;
+  const double th0 = th, gam0 = gam;
+  const double K2 = 1.0 + (double)pr2 + (1.0 + ue * ue) * K0 * K0 * 0.5 * (1.0 + ku * ku * 0.5 * (double)r2);
+  const float xi = (float)(ks / ku * K0 * K0 / (8.0 * gam0 * gam0) * (1.0 - ue * ue));
+  // longitudinal space charge, first mode (sc_l = 1 in the batched path)
+  cpx<float> Em = mk(0.f, 0.f);
+  {
+    const int F = g.F, nr = g.nr;
+    const double r = sqrt((double)r2);
+    const int idxR = (int)(r / g.dr + 0.5);
+    if (idxR >= 0 && idxR < nr) {
+      const float wr1 = idxR > 0 ? (float)((r * r / (g.dr * g.dr) - idxR * idxR) / (2.0 * idxR)) : 1.f;
+      const float wr0 = idxR > 0 ? 1.f - wr1 : 1.f;
+      for (int f = 0; f < F; ++f) {
+        const cpx<double> *p = Ez + (long long)slot * nr * F + nr * f + idxR;
+        Em = Em + cvt<float>(p[0]) * wr0 + cvt<float>(p[1]) * wr1;
+      }
+    } else {
+      atomicOr(err, EB_EMCARRY);
+    }
+  }
+  const long long base = (long long)phys_slot(slot, g.ring_base, g.S) * g.N + (long long)idxJ * g.nx + idxI;
+  bool any_even = false;
+#pragma unroll
+  for (int h = 0; h < NH; ++h) any_even |= (g.harm[h] % 2 == 0);
+  float xi2 = 0.f;
+  cpx<float> expP = mk(1.f, 0.f);
+  if (any_even) {
+    const float uef = (float)ue;
+    xi2 = (float)(sqrt(2.0) * K0 / (1.0 + K0 * K0 * 0.5 * (1.0 + ue * ue))) *
+          sqrtf((1 + ka) * (px * px + uef * uef * py * py) + (1 - ka) * (px * px * uef * uef + py * py));
+    const float B = atanf(2.0f * ka * uef * (px - py) /
+                          ((1.0f + ka) * (px + uef * uef * py) + (1 - ka) * (px * uef * uef + py)));
+    float sb, cb;
+    sincosf(B - (float)(l.Phi * 0.5), &sb, &cb);
+    expP = mk(cb, sb);
+  }
+  const cpx<float> etaX = mk((float)l.etaXr, (float)l.etaXi), etaY = mk((float)l.etaYr, (float)l.etaYi);
+  cpx<float> cu[NH];
+  float sumU2 = 0.f;
+#pragma unroll
+  for (int h = 0; h < NH; ++h) {
+    const cpx<float> *a0 = Ux + (long long)h * g.S * g.N + base;
+    const cpx<float> *b0 = Uy + (long long)h * g.S * g.N + base;
+    const cpx<float> ux =
+        a0[0] * (wx0 * wy0) + a0[g.nx] * (wx0 * wy1) + a0[1] * (wx1 * wy0) + a0[g.nx + 1] * (wx1 * wy1);
+    const cpx<float> uy =
+        b0[0] * (wx0 * wy0) + b0[g.nx] * (wx0 * wy1) + b0[1] * (wx1 * wy0) + b0[g.nx + 1] * (wx1 * wy1);
+    const int H = g.harm[h];
+    sumU2 += (norm2(ux) + norm2(uy)) / (2.0f * H * H);
+    const int idx = (int)((1.0f * H * xi) / (float)g.dB);
+    const float w1 = (H * xi / (float)g.dB) - idx, w0 = 1.f - w1;
+    cpx<float> fx = mk(0.f, 0.f), fy = mk(0.f, 0.f);
+    if (idx < 0 || idx + 1 >= g.bn) {
+      atomicOr(err, EB_BESSEL);
+    } else if (H % 2 == 1) {
+      const float sign = ((H - 1) / 2 % 2 == 0) ? 1.f : -1.f;
+      const float J1 = (float)bessel_interp(bes, g.bo, idx, (H - 1) / 2, w0, w1);
+      const float J2 = (float)bessel_interp(bes, g.bo, idx, (H + 1) / 2, w0, w1);
+      fx = mk(sign * (ka * J1 - J2), 0.f);
+      fy = mk(sign * (ka * J1 + J2), 0.f);
+    } else {
+      const float sign = (H / 2 % 2 == 0) ? 1.f : -1.f;
+      const float J1 = (float)bessel_interp(bes, g.bo, idx, (H - 2) / 2, w0, w1);
+      const float J2 = (float)bessel_interp(bes, g.bo, idx, H / 2, w0, w1);
+      const float J3 = (float)bessel_interp(bes, g.bo, idx, (H + 2) / 2, w0, w1);
+      const float s = sign * xi2 * H * 0.5f;
+      const cpx<float> expM = conjg(expP);
+      fx = (expP * (ka * J1 - J2) + expM * (ka * J2 - J3)) * s;
+      fy = (expP * (ka * J1 + J2) + expM * (ka * J2 + J3)) * s;
+    }
+    cu[h] = etaX * fx * ux - etaY * fy * uy;
+  }
+  const float sq = (float)(K0 * sqrt(ue * ue + 1.0));
+  const float ks_f = (float)ks;
+  const float e_mc2 = (float)(kECharge / kEMass / kC / kC);
+  const double halfPhi = l.Phi * 0.5;
+  const cpx<float> rotm = expi_f32(-halfPhi);  // e^{-i Phi/2}
+  const double cmv[5] = {0, 0, 0.5, 0.5, 1};
+  const double wt[5] = {0, 1, 2, 2, 1};
+  double kth = 0.0, kg = 0.0, sth = 0.0, sg = 0.0;
+#pragma unroll
+  for (int m = 1; m < 5; ++m) {
+    const double thm = th0 + dz * kth * cmv[m];
+    const double gm = gam0 + dz * kg * cmv[m];
+    const cpx<float> e1 = expi_f32(thm + halfPhi);
+    const float igm = (float)(0.5 / gm);
+    float sumTh = 0.f, sumG = 0.f;
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {
+      const int H = g.harm[h];
+      const cpx<float> t = cpow_int(e1, H) * cu[h];
+      sumTh -= t.im / (float)H;
+      sumG += t.re * igm;
+    }
+    const cpx<float> eth = e1 * rotm;  // e^{i theta}
+    const float sumEz = 2.0f * (Em.re * eth.re - Em.im * eth.im);
+    const double c2 = ks / (2.0 * gm * gm);
+    kth = (ku - c2 * K2) - (double)((float)c2 * (sumU2 + sq * sumTh));
+    kg = (double)(ks_f * sq * sumG + e_mc2 * sumEz);
+    sth += wt[m] * kth;
+    sg += wt[m] * kg;
+  }
+  const double dth = dz / 6.0 * sth;
+  if (dth > g.dPhi || dth < -g.dPhi) {
+    atomicOr(err, EB_GUARD);
+    return;
+  }
+  th = th0 + dth;
+  gam = gam0 + (dz / 6.0 * sg - dz * wakeE);
+}
+
+// One workgroup of kTile threads per tile, one particle per thread.
+template <int NH>
+__global__ void __launch_bounds__(kTile)
+    k_pstep(GridArgs g, PStepArgs a, Tiles tl, double *__restrict__ theta, float *__restrict__ gamma,
+            float *__restrict__ x, float *__restrict__ y, float *__restrict__ px, float *__restrict__ py,
+            const double *__restrict__ seg_w, const cpx<float> *__restrict__ Ux, const cpx<float> *__restrict__ Uy,
+            const cpx<double> *__restrict__ Ez, const double *__restrict__ bes, const double *__restrict__ wakeE,
+            double *__restrict__ tile_red, unsigned *err) {
+  __shared__ double red[(2 * NH + 1 + 7) * (kTile / 64)];
+  const int t = blockIdx.x, k = threadIdx.x, lane = k & 63, wv = k >> 6;
+  const int seg = tl.seg[t];
+  const int sl = seg % g.slice_n, slot = sl + 1;
+  const bool on = k < tl.cnt[t];
+  const double w = seg_w[seg];
+  constexpr int NV = 2 * NH + 1 + 7;
+  double acc[NV];
+#pragma unroll
+  for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+  if (on) {
+    const long long n = tl.start[t] + k;
+    const double wk = g.wake_on ? wakeE[sl] / (kMc2 * 1.0e6) : 0.0;
+    double th = theta[n];
+    double gm = g.gref + (double)gamma[n];
+    float xx = x[n], yy = y[n], ppx = px[n], ppy = py[n];
+    const float dzh = (float)(g.dz * 0.5);
+    const float gf = (float)gm;
+    transverse_f32(a.l0, dzh, gf, xx, yy, ppx, ppy);
+    transverse_f32(a.l1, dzh, gf, xx, yy, ppx, ppy);
+    if (!a.drift) {
+      push_tg_f32<NH>(g, a.l1, slot, wk, Ux, Uy, Ez, bes, xx, yy, ppx, ppy, th, gm, err);
+    } else {
+      const double xd = xx, yd = yy;
+      const int idxI = (int)((xd - g.minX) / g.dx), idxJ = (int)((yd - g.minY) / g.dy);
+      if (idxI >= 0 && idxI < g.nx - 1 && idxJ >= 0 && idxJ < g.ny - 1) {
+        const float wx1 = (float)((xd - g.minX) / g.dx - idxI), wx0 = 1.f - wx1;
+        const float wy1 = (float)((yd - g.minY) / g.dy - idxJ), wy0 = 1.f - wy1;
+        const long long base =
+            (long long)phys_slot(slot, g.ring_base, g.S) * g.N + (long long)idxJ * g.nx + idxI;
+        float sumU2 = 0.f;
+#pragma unroll
+        for (int h = 0; h < NH; ++h) {
+          const cpx<float> *a0 = Ux + (long long)h * g.S * g.N + base;
+          const cpx<float> *b0 = Uy + (long long)h * g.S * g.N + base;
+          const cpx<float> ux =
+              a0[0] * (wx0 * wy0) + a0[g.nx] * (wx0 * wy1) + a0[1] * (wx1 * wy0) + a0[g.nx + 1] * (wx1 * wy1);
+          const cpx<float> uy =
+              b0[0] * (wx0 * wy0) + b0[g.nx] * (wx0 * wy1) + b0[1] * (wx1 * wy0) + b0[g.nx + 1] * (wx1 * wy1);
+          const int H = g.harm[h];
+          sumU2 += (norm2(ux) + norm2(uy)) / (2.0f * H * H);
+        }
+        const double K2 = 1.0 + (double)(ppx * ppx + ppy * ppy);
+        const double tt = a.l1.ku - g.ks / (2.0 * a.gamR * a.gamR) * (K2 + (double)sumU2);
+        th += tt * g.dz;
+        th -= g.dz * wk;
+      }
+    }
+    // theta wrap (rearrangeParticles.cpp:84-108)
+    const double dTh = g.dPhi;
+    double del = 0.0;
+    if (th >= dTh) del = dTh * (int)(th / dTh);
+    else if (th < 0) del = dTh * (int)(th / dTh - 1.0);
+    th -= del;
+    if (th != th) atomicOr(err, EB_NAN);
+    else if (th > dTh || th < 0) atomicOr(err, EB_RANGE);
+    theta[n] = th;
+    gamma[n] = (float)(gm - g.gref);
+    x[n] = xx;
+    y[n] = yy;
+    px[n] = ppx;
+    py[n] = ppy;
+    // moments of the next step's bFactor and twiss, accumulated in FP64
+    const cpx<float> e1 = expi_f32(th);
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {
+      const cpx<float> eh = cpow_int(e1, g.harm[h]);
+      acc[2 * h] = (double)eh.re * w;
+      acc[2 * h + 1] = (double)eh.im * w;
+    }
+    acc[2 * NH] = w;
+    const double inv = 1.0 / gm;
+    const double xp = (double)ppx * inv, yp = (double)ppy * inv, xv = xx, yv = yy;
+    acc[2 * NH + 1] = xv * xv;
+    acc[2 * NH + 2] = yv * yv;
+    acc[2 * NH + 3] = xp * xp;
+    acc[2 * NH + 4] = yp * yp;
+    acc[2 * NH + 5] = xv * xp;
+    acc[2 * NH + 6] = yv * yp;
+    acc[2 * NH + 7] = 1.0;
+  }
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    double v = acc[q];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+    if (lane == 0) red[q * (kTile / 64) + wv] = v;
+  }
+  __syncthreads();
+  if (k < NV) {
+    double s = 0.0;
+    for (int u = 0; u < kTile / 64; ++u) s += red[k * (kTile / 64) + u];
+    tile_red[(long long)t * a.nred + k] = s;
+  }
+}
+
+// One workgroup per slice: every species' particles of the slot deposit into
+// an LDS window over their bounding box (harmonics hp at a time), which is then
+// stored to the interior of Sc with plain stores: those cells are written by
+// this workgroup only and were cleared by the ADI that read them. Window cells
+// on the boundary ring and the reference's flat-index spill (A.5) use atomics.
+template <int NH>
+__global__ void __launch_bounds__(1024)
+    k_deposit_slot(GridArgs g, DepArgs d, int n_species, const long long *__restrict__ seg_off,
+                   const double *__restrict__ seg_w, const double *__restrict__ theta,
+                   const float *__restrict__ gamma, const float *__restrict__ x, const float *__restrict__ y,
+                   const float *__restrict__ px, const float *__restrict__ py, const double *__restrict__ bes,
+                   cpx<float> *__restrict__ Scx, cpx<float> *__restrict__ Scy, cpx<double> *__restrict__ ScR,
+                   int4 *__restrict__ win_out, int lds_bytes, unsigned *err) {
+  extern __shared__ double lds_d[];
+  __shared__ int bb[4];
+  const int sl = blockIdx.x, slot = sl + 1, tid = threadIdx.x, nt = blockDim.x;
+  if (tid == 0) {
+    bb[0] = 1 << 30; bb[1] = -1; bb[2] = 1 << 30; bb[3] = -1;
+  }
+  __syncthreads();
+  int mi = 1 << 30, Mi = -1, mj = 1 << 30, Mj = -1;
+  for (int sp = 0; sp < n_species; ++sp) {
+    const int seg = sp * g.slice_n + sl;
+    const long long b = seg_off[seg], e = seg_off[seg + 1];
+    for (long long n = b + tid; n < e; n += nt) {
+      const double xd = x[n], yd = y[n];
+      const int idxI = (int)((xd - g.minX) / g.dx), idxJ = (int)((yd - g.minY) / g.dy);
+      if (idxI >= 0 && idxI < g.nx - 1 && idxJ >= 0 && idxJ < g.ny - 1) {
+        mi = min(mi, idxI); Mi = max(Mi, idxI); mj = min(mj, idxJ); Mj = max(Mj, idxJ);
+      }
+    }
+  }
+  atomicMin(&bb[0], mi); atomicMax(&bb[1], Mi); atomicMin(&bb[2], mj); atomicMax(&bb[3], Mj);
+  __syncthreads();
+  const int wi0 = bb[0], wj0 = bb[2];
+  const bool empty = bb[1] < 0;
+  const int ww = empty ? 0 : bb[1] + 2 - wi0, wh = empty ? 0 : bb[3] + 2 - wj0;
+  const int W = ww * wh;
+  int hp = W > 0 ? lds_bytes / (W * 2 * (int)sizeof(cpx<float>)) : NH;
+  if (hp > NH) hp = NH;
+  const int step = hp > 0 ? hp : 1;
+  // the only interior Sc cells this deposit can make nonzero
+  if (tid == 0) win_out[slot] = empty ? make_int4(1, 0, 1, 0) : make_int4(wi0, wi0 + ww - 1, wj0, wj0 + wh - 1);
+  bool any_even = false;
+#pragma unroll
+  for (int h = 0; h < NH; ++h) any_even |= (g.harm[h] % 2 == 0);
+  const cpx<float> etaX = mk((float)d.etaXr, (float)d.etaXi), etaY = mk((float)d.etaYr, (float)d.etaYi);
+  const float ka = (float)d.ka;
+  cpx<float> *win = reinterpret_cast<cpx<float> *>(lds_d);
+  for (int h0 = 0; h0 < NH; h0 += step) {
+    const int hc = min(step, NH - h0);
+    if (hp > 0) {
+      for (int q = tid; q < hc * 2 * W; q += nt) win[q] = mk(0.f, 0.f);
+      __syncthreads();
+    }
+    bool bad = false;
+    for (int sp = 0; sp < n_species; ++sp) {
+      const int seg = sp * g.slice_n + sl;
+      const long long b = seg_off[seg], e = seg_off[seg + 1];
+      const double macro = seg_w[seg];
+      // each thread takes kRun consecutive particles; consecutive particles
+      // in the same cell (the phases of a beamlet) are summed in registers
+      // and reach the window with one set of atomics
+      for (long long r0 = b + (long long)tid * kRun; r0 < e; r0 += (long long)nt * kRun) {
+      const long long r1 = min(r0 + (long long)kRun, e);
+      int key = -1, kI = 0, kJ = 0;
+      bool kin = false;
+      cpx<float> acc[4][NH][2];
+#define FELIS_DEPOSIT_FLUSH()                                                                                   \
+  if (key >= 0) {                                                                                               \
+    _Pragma("unroll") for (int c = 0; c < 4; ++c) {                                                             \
+      const int ii = c >> 1, jj = c & 1;                                                                        \
+      _Pragma("unroll") for (int hh = 0; hh < NH; ++hh) {                                                       \
+        if (hh >= hc) break;                                                                                    \
+        const cpx<float> vx = acc[c][hh][0], vy = acc[c][hh][1];                                                \
+        if (kin) {                                                                                              \
+          const int q = (kJ + jj - wj0) * ww + (kI + ii - wi0);                                                 \
+          float *p0 = reinterpret_cast<float *>(lds_d) + 2 * ((hh * 2 + 0) * W + q);                            \
+          float *p1 = reinterpret_cast<float *>(lds_d) + 2 * ((hh * 2 + 1) * W + q);                            \
+          atomicAdd(p0, vx.re);                                                                                 \
+          atomicAdd(p0 + 1, vx.im);                                                                             \
+          atomicAdd(p1, vy.re);                                                                                 \
+          atomicAdd(p1 + 1, vy.im);                                                                             \
+        } else {                                                                                                \
+          const long long o =                                                                                   \
+              (long long)(h0 + hh) * g.S * g.N + (long long)slot * g.N + (long long)kJ * g.nx + kI + jj * g.nx + ii; \
+          unsafeAtomicAdd(&Scx[o].re, vx.re);                                                                   \
+          unsafeAtomicAdd(&Scx[o].im, vx.im);                                                                   \
+          unsafeAtomicAdd(&Scy[o].re, vy.re);                                                                   \
+          unsafeAtomicAdd(&Scy[o].im, vy.im);                                                                   \
+        }                                                                                                       \
+      }                                                                                                         \
+    }                                                                                                           \
+  }
+      for (long long n = r0; n < r1; ++n) {
+        const double xd = x[n], yd = y[n];
+        const int idxI = (int)((xd - g.minX) / g.dx), idxJ = (int)((yd - g.minY) / g.dy);
+        if (!(idxI >= 0 && idxI < g.nx && idxJ >= 0 && idxJ < g.ny)) continue;
+        const int k2 = idxJ * g.nx + idxI;
+        if (k2 != key) {
+          FELIS_DEPOSIT_FLUSH();
+#pragma unroll
+          for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int hh = 0; hh < NH; ++hh) acc[c][hh][0] = acc[c][hh][1] = mk(0.f, 0.f);
+          key = k2;
+          kI = idxI;
+          kJ = idxJ;
+          kin = hp > 0 && idxI < g.nx - 1 && idxJ < g.ny - 1;
+        }
+        float wx[2], wy[2];
+        wx[1] = (float)((xd - g.minX) / g.dx - idxI); wx[0] = 1.f - wx[1];
+        wy[1] = (float)((yd - g.minY) / g.dy - idxJ); wy[0] = 1.f - wy[1];
+        const double th = theta[n];
+        const double gm = g.gref + (double)gamma[n];
+        float xi2 = 0.f;
+        cpx<float> expP = mk(1.f, 0.f);
+        if (any_even) {
+          const float ppx = px[n], ppy = py[n], ue = (float)d.ue;
+          xi2 = (float)d.xi2c *
+                sqrtf((1 + ka) * (ppx * ppx + ue * ue * ppy * ppy) + (1 - ka) * (ppx * ppx * ue * ue + ppy * ppy));
+          const float B = atanf(2.0f * ka * ue * (ppx - ppy) /
+                                ((1.0f + ka) * (ppx + ue * ue * ppy) + (1 - ka) * (ppx * ue * ue + ppy)));
+          float sb, cb;
+          sincosf(B - (float)(d.Phi * 0.5), &sb, &cb);
+          expP = mk(cb, sb);
+        }
+        const cpx<float> em1 = conjg(expi_f32(th + d.Phi * 0.5));  // e^{-i psi}
+        const float amp0 = (float)(macro * d.coef_U / gm);
+#pragma unroll
+        for (int hh = 0; hh < NH; ++hh) {
+          if (hh >= hc) break;
+          const int h = h0 + hh, H = g.harm[h];
+          const int ib = (int)((H * d.xi) / g.dB);
+          const float w1 = (float)((H * d.xi / g.dB) - ib), w0 = 1.f - w1;
+          if (ib < 0 || ib + 1 >= g.bn) {
+            bad = true;
+            continue;
+          }
+          cpx<float> fx, fy;
+          if (H % 2 == 1) {
+            const float sign = ((H - 1) / 2 % 2 == 0) ? 1.f : -1.f;
+            const float J1 = (float)bessel_interp(bes, g.bo, ib, (H - 1) / 2, w0, w1);
+            const float J2 = (float)bessel_interp(bes, g.bo, ib, (H + 1) / 2, w0, w1);
+            fx = mk(sign * (J1 - ka * J2), 0.f);
+            fy = mk(sign * (J1 + ka * J2), 0.f);
+          } else {
+            const float sign = (H / 2 % 2 == 0) ? 1.f : -1.f;
+            const float J1 = (float)bessel_interp(bes, g.bo, ib, (H - 2) / 2, w0, w1);
+            const float J2 = (float)bessel_interp(bes, g.bo, ib, H / 2, w0, w1);
+            const float J3 = (float)bessel_interp(bes, g.bo, ib, (H + 2) / 2, w0, w1);
+            const float s = sign * xi2 * H * 0.5f;
+            const cpx<float> expM = conjg(expP);
+            fx = (expM * (J1 - ka * J2) + expP * (J2 - ka * J3)) * s;
+            fy = (expM * (J1 + ka * J2) + expP * (J2 + ka * J3)) * s;
+          }
+          const cpx<float> mc = cpow_int(em1, H) * (amp0 * (float)H);
+          const cpx<float> cxv = etaX * fx * mc, cyv = etaY * fy * mc;
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const float wgt = -(wx[c >> 1] * wy[c & 1]);
+            acc[c][hh][0] = acc[c][hh][0] + cxv * wgt;
+            acc[c][hh][1] = acc[c][hh][1] + cyv * wgt;
+          }
+        }
+      }
+      FELIS_DEPOSIT_FLUSH();
+#undef FELIS_DEPOSIT_FLUSH
+      }
+    }
+    if (bad) atomicOr(err, EB_BESSEL);
+    if (hp > 0) {
+      __syncthreads();
+      for (int q = tid; q < hc * 2 * W; q += nt) {
+        const cpx<float> v = win[q];
+        const int hh = q / (2 * W), pol = (q / W) & 1, r = q % W;
+        const int jy = wj0 + r / ww, ix = wi0 + r % ww;
+        cpx<float> *dst = (pol ? Scy : Scx) + (long long)(h0 + hh) * g.S * g.N + (long long)slot * g.N +
+                          (long long)jy * g.nx + ix;
+        if (jy > 0 && jy < g.ny - 1 && ix > 0 && ix < g.nx - 1) {
+          *dst = v;
+        } else if (v.re != 0.f || v.im != 0.f) {
+          unsafeAtomicAdd(&dst->re, v.re);
+          unsafeAtomicAdd(&dst->im, v.im);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // radial source of the last species, stored whole for this slot
+  const int F = g.F, nr = g.nr, row = F * nr + 1;
+  cpx<double> *hist = reinterpret_cast<cpx<double> *>(lds_d);
+  for (int q = tid; q < row; q += nt) hist[q] = mk(0.0, 0.0);
+  __syncthreads();
+  {
+    const int seg = (n_species - 1) * g.slice_n + sl;
+    const long long b = seg_off[seg], e = seg_off[seg + 1];
+    const double macro = seg_w[seg];
+    for (long long n = b + tid; n < e; n += nt) {
+      const double xx = x[n], yy = y[n], th = theta[n];
+      const double phi = (xx == 0.0) ? 0.0 : atan2(yy, xx);
+      const double r = sqrt(xx * xx + yy * yy);
+      const int idxR = (int)(r / g.dr + 0.5);
+      if (idxR > 0 && idxR < nr) {
+        const double wr1 = (r * r / (g.dr * g.dr) - idxR * idxR) / (2.0 * idxR), wr0 = 1.0 - wr1;
+        for (int f = 0; f < F; ++f) {
+          double s, c;
+          sincos(-(th + f * phi), &s, &c);
+          const double amp = d.coef_Ez * macro / (1.0 * idxR);
+          cpx<double> *hp2 = hist + f * nr + idxR;
+          atomicAdd(&hp2[0].re, -s * amp * wr0);
+          atomicAdd(&hp2[0].im, c * amp * wr0);
+          atomicAdd(&hp2[1].re, -s * amp * wr1);
+          atomicAdd(&hp2[1].im, c * amp * wr1);
+        }
+      } else if (idxR == 0) {
+        double s, c;
+        sincos(-th, &s, &c);
+        const double amp = d.coef_Ez * macro;
+        atomicAdd(&hist[0].re, -s * amp);
+        atomicAdd(&hist[0].im, c * amp);
+      }
+    }
+  }
+  __syncthreads();
+  for (int q = tid; q < row; q += nt) ScR[(long long)slot * row + q] = hist[q];
+}
+
+// out[o] for o < 2*nh: energy of pol o/nh, harmonic o%nh over the slices that
+// stay after the slippage; next 2*nh+1: bFactor sums; then 7 twiss sums per
+// species. One workgroup per output, strided partial sums then a fixed tree.
+__global__ void __launch_bounds__(256)
+    k_reduce_diag(GridArgs g, const double *__restrict__ slice_e, int skip_last, const double *__restrict__ tile_red,
+                  int nred, const int *__restrict__ species_tile, int n_species, double *__restrict__ out) {
+  __shared__ double sh[256];
+  const int o = blockIdx.x, tid = threadIdx.x, nh = g.nh;
+  double s = 0.0;
+  if (o < 2 * nh) {
+    const double *row = slice_e + (long long)o * g.slice_n;
+    const int n = g.slice_n - (skip_last ? 1 : 0);
+    for (int k = tid; k < n; k += 256) s += row[k];
+  } else if (o < 4 * nh + 1) {
+    const int q = o - 2 * nh;
+    const int n = species_tile[n_species];
+    for (int k = tid; k < n; k += 256) s += tile_red[(long long)k * nred + q];
+  } else {
+    const int q = o - (4 * nh + 1), sp = q / 7, c = q % 7;
+    for (int k = species_tile[sp] + tid; k < species_tile[sp + 1]; k += 256)
+      s += tile_red[(long long)k * nred + 2 * nh + 1 + c];
+  }
+  sh[tid] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) sh[tid] += sh[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) out[o] = sh[0];
+}
+
+}  // namespace felis
