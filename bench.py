@@ -61,17 +61,19 @@ def kernel_bytes(name: str, d, np_: int, mixed: bool) -> float:
     return float(table.get(name, 0.0))
 
 
-def run_gpu(args):
+def run_gpu(args, precision=None, steps=None, warmup=None, keep_init=None):
     from felis_amd import _lib as fl
     deck = fl.Deck(ROOT / "felis_amd" / "decks" / f"{args.config}.inp")
-    prec = fl.MIXED_FP32 if args.precision == "mixed" else fl.PLAIN_FP64
+    precision = precision or args.precision
+    prec = fl.MIXED_FP32 if precision == "mixed" else fl.PLAIN_FP64
     desc = deck.desc(prec)
     ctx = fl.Context(desc)
     deck.load(ctx, seed=args.seed)
     init = None
-    if not args.no_cpu:
+    if (not args.no_cpu) if keep_init is None else keep_init:
         init = dict(particles=ctx.download_particles(), Ux=ctx.download_field(fl.UX), Uy=ctx.download_field(fl.UY))
-    W, K = args.warmup, args.steps
+    W = args.warmup if warmup is None else warmup
+    K = args.steps if steps is None else steps
     tab = deck.lattice(0, W + K)
     wakeF = deck.wake_function()[0] if desc.wake_on else None
     ctx.sync()
@@ -149,22 +151,42 @@ def gpu_parity(args, g, cpu):
     return float(np.max(np.abs(tot_g - tot_o) / np.abs(tot_o)))
 
 
+def calibrate_copy(mib: int):
+    """A plain device-to-device copy of known size, for scaling the HBM byte
+    counters (FETCH_SIZE, WRITE_SIZE) of a profiler run."""
+    import torch
+    n = mib << 20
+    a = torch.ones(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="cfg1", choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="plain", choices=["plain", "mixed"])
+    ap.add_argument("--config", default="cfg2", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="mixed", choices=["plain", "mixed"])
     ap.add_argument("--seed", type=int, default=20261203)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-steps", type=int, default=10,
+                    help="periods of the plain FP64 path timed beside a mixed run (0: skip)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB three times (profiler counter scale)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (slices shard across GPUs "
               "only by design, see the sharding notes); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     g = run_gpu(args)
     d = g["desc"]
     K = args.steps
@@ -198,6 +220,13 @@ def main():
                      "avg_launch_us": avg_s * 1e6},
         "kernels_ms_per_step": {k: v[0] / K for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])},
     }
+    if mixed and args.plain_steps > 0:
+        gp = run_gpu(args, precision="plain", steps=args.plain_steps, warmup=1, keep_init=False)
+        plain_ms = gp["seconds"] / args.plain_steps * 1e3
+        gp["ctx"].close()
+        out["plain_fp64"] = {"ms_per_step": plain_ms, "steps": args.plain_steps,
+                             "value": d.slice_n * args.plain_steps / gp["seconds"],
+                             "speedup_of_batched": plain_ms / out["ms_per_step"]}
     if not args.no_cpu:
         cpu = run_cpu(args, g)
         out["cpu_baseline"] = {"value": d.slice_n * args.cpu_steps / cpu["seconds"], "unit": "slice·periods/s",

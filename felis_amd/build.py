@@ -38,15 +38,36 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False) -> Path:
-    srcs = [str(CSRC / s) for s in HIP_SOURCES]
+    """Compiles every source to an object in parallel, then links the library."""
+    objdir = PKG / "_build"
+    objdir.mkdir(exist_ok=True)
+    inc = [f"-I{ROOT / 'include'}", f"-I{CSRC}"]
+    flags = [f for f in HIPFLAGS if f != "-shared"]
+    procs = []
+    objs = []
+    for s in HIP_SOURCES:
+        obj = objdir / (Path(s).stem + ".o")
+        cmd = [hipcc(), *flags, *inc, "-c", str(CSRC / s), "-o", str(obj)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((s, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        objs.append(str(obj))
+    failed = []
+    for s, p in procs:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out)
+            failed.append(s)
+    if failed:
+        raise RuntimeError(f"hipcc failed on {', '.join(failed)}")
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPFLAGS, f"-I{ROOT / 'include'}", f"-I{CSRC}", *srcs, "-o", str(tmp)]
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed ({res.returncode})")
+        raise RuntimeError(f"link failed ({res.returncode})")
     os.replace(tmp, LIB)
     return LIB
 

@@ -136,6 +136,9 @@ int dep_args(felis_ctx *ctx, const felis_lattice *lat, DepArgs *out) {
   a.xi2c = std::sqrt(2.0) * K0 / (1.0 + K0 * K0 * 0.5 * (1.0 + ue * ue));
   a.ue = ue;
   a.ka = lat->K0_alpha;
+  a.inv_dx = 1.0 / d.dx;
+  a.inv_dy = 1.0 / d.dy;
+  a.inv_dB = 1.0 / ctx->dB;
   *out = a;
   return FELIS_OK;
 }

@@ -112,6 +112,7 @@ struct DepArgs {
   double etaXr, etaXi, etaYr, etaYi;
   double xi2c;  // sqrt(2) K0 / (1 + K0^2/2 (1+ue^2))
   double ue, ka;
+  double inv_dx, inv_dy, inv_dB;  // reciprocals for the batched deposit
   int last_species_first_seg;
 };
 

@@ -26,7 +26,37 @@ struct PStepArgs {
   int drift;       // driftFlag of (it,0.5)
   int nred;        // partial-sum row length: 2*nh + 1 + 7
   double gamR;
+  double inv_dx, inv_dy, inv_dr;  // grid reciprocals
+  // per harmonic H: 1/H, H/dBessel, 1/(2 H^2) and H - H_prev (H_prev of the
+  // first harmonic is 0), the step of the power chain e^{iH psi}
+  float hinv[FELIS_MAX_HARM], hdb[FELIS_MAX_HARM], hu2[FELIS_MAX_HARM];
+  int hstep[FELIS_MAX_HARM];
 };
+
+// e^{i H_h psi} for every harmonic from e^{i psi}, by multiplying along the
+// ascending harmonic list (steps of 1 and 2 use e and e^2 directly)
+template <int NH>
+__device__ inline void harm_powers(cpx<float> e1, const int *hstep, cpx<float> (&out)[NH]) {
+  const cpx<float> e2 = e1 * e1;
+  cpx<float> cur = mk(1.f, 0.f);
+#pragma unroll
+  for (int h = 0; h < NH; ++h) {
+    const int d = hstep[h];
+    cpx<float> m;
+    if (d == 1) m = e1;
+    else if (d == 2) m = e2;
+    else {
+      m = mk(1.f, 0.f);
+      cpx<float> b = e1;
+      for (int n = d; n; n >>= 1) {
+        if (n & 1) m = m * b;
+        b = b * b;
+      }
+    }
+    cur = h == 0 && d == 1 ? e1 : cur * m;
+    out[h] = cur;
+  }
+}
 
 template <typename T>
 __device__ inline cpx<T> cpow_int(cpx<T> b, int n) {
@@ -39,12 +69,18 @@ __device__ inline cpx<T> cpow_int(cpx<T> b, int n) {
   return r;
 }
 
-// psi mod 2pi in FP64, then e^{i psi} in FP32
+// e^{i psi} in FP32 with the reduction in FP64: psi = q pi/2 + r with
+// |r| <= pi/4, then the Taylor series of sin r (to r^9) and cos r (to r^10),
+// whose truncation errors there are below 2e-9, and the quadrant's swap.
 __device__ inline cpx<float> expi_f32(double psi) {
-  psi -= 6.283185307179586 * floor(psi * 0.15915494309189535);
-  float s, c;
-  sincosf((float)psi, &s, &c);
-  return mk(c, s);
+  const double q = rint(psi * 0.6366197723675814);
+  const float r = (float)fma(-q, 1.5707963267948966, psi);
+  const float r2 = r * r;
+  const float sr = r + r * r2 * (-1.f / 6 + r2 * (1.f / 120 + r2 * (-1.f / 5040 + r2 * (1.f / 362880))));
+  const float cr = 1.f + r2 * (-0.5f + r2 * (1.f / 24 + r2 * (-1.f / 720 + r2 * (1.f / 40320 + r2 * (-1.f / 3628800)))));
+  const int qi = (int)(long long)q & 3;
+  const float s = (qi & 1) ? cr : sr, c = (qi & 1) ? sr : cr;
+  return mk((qi == 1 || qi == 2) ? -c : c, (qi & 2) ? -s : s);
 }
 
 // transversePush of one particle in FP32 (particlePush.cpp:76-111)
@@ -82,41 +118,44 @@ __device__ inline void transverse_f32(const LatArgs &l, float dz, float gam, flo
     spx += wt[m] * kpx;
     spy += wt[m] * kpy;
   }
-  x += dz / 6.0f * sx;
-  y += dz / 6.0f * sy;
-  px += dz / 6.0f * spx;
-  py += dz / 6.0f * spy;
+  const float dz6 = dz * (1.0f / 6.0f);
+  x += dz6 * sx;
+  y += dz6 * sy;
+  px += dz6 * spx;
+  py += dz6 * spy;
 }
 
 // push_theta_gamma_3D of one particle (particlePush.cpp:260-379): the field
 // coupling in FP32, the slippage phase ku - ks/(2 gamma^2) K2, gamma and
 // theta in FP64. One sincos per RK stage; harmonics by complex powers.
 template <int NH>
-__device__ inline void push_tg_f32(const GridArgs &g, const LatArgs &l, int slot, double wakeE,
+__device__ inline void push_tg_f32(const GridArgs &g, const PStepArgs &a, const LatArgs &l, int slot, double wakeE,
                                    const cpx<float> *__restrict__ Ux, const cpx<float> *__restrict__ Uy,
                                    const cpx<double> *__restrict__ Ez, const double *__restrict__ bes, float x0,
                                    float y0, float px, float py, double &th, double &gam, unsigned *err) {
   const double ku = l.ku, ks = g.ks, K0 = l.K0, ue = l.ue;
   const float ka = (float)l.K0_alpha;
   const double dz = g.dz;
-  const double xd = x0, yd = y0;
-  const int idxI = (int)((xd - g.minX) / g.dx);
-  const int idxJ = (int)((yd - g.minY) / g.dy);
+  const double fx_ = ((double)x0 - g.minX) * a.inv_dx, fy_ = ((double)y0 - g.minY) * a.inv_dy;
+  const int idxI = (int)fx_;
+  const int idxJ = (int)fy_;
   if (!(idxI >= 0 && idxI < g.nx - 1 && idxJ >= 0 && idxJ < g.ny - 1)) return;
-  const float wx1 = (float)((xd - g.minX) / g.dx - idxI), wx0 = 1.f - wx1;
-  const float wy1 = (float)((yd - g.minY) / g.dy - idxJ), wy0 = 1.f - wy1;
+  const float wx1 = (float)(fx_ - idxI), wx0 = 1.f - wx1;
+  const float wy1 = (float)(fy_ - idxJ), wy0 = 1.f - wy1;
   const float r2 = x0 * x0 + y0 * y0, pr2 = px * px + py * py;
   const double th0 = th, gam0 = gam;
   const double K2 = 1.0 + (double)pr2 + (1.0 + ue * ue) * K0 * K0 * 0.5 * (1.0 + ku * ku * 0.5 * (double)r2);
-  const float xi = (float)(ks / ku * K0 * K0 / (8.0 * gam0 * gam0) * (1.0 - ue * ue));
+  const double ig0 = 1.0 / gam0;
+  const float xi = (float)(ks / ku * K0 * K0 * 0.125 * (1.0 - ue * ue) * ig0 * ig0);
   // longitudinal space charge, first mode (sc_l = 1 in the batched path)
   cpx<float> Em = mk(0.f, 0.f);
   {
     const int F = g.F, nr = g.nr;
     const double r = sqrt((double)r2);
-    const int idxR = (int)(r / g.dr + 0.5);
+    const double rr = r * a.inv_dr;
+    const int idxR = (int)(rr + 0.5);
     if (idxR >= 0 && idxR < nr) {
-      const float wr1 = idxR > 0 ? (float)((r * r / (g.dr * g.dr) - idxR * idxR) / (2.0 * idxR)) : 1.f;
+      const float wr1 = idxR > 0 ? __fdividef((float)(rr * rr - (double)idxR * idxR), 2.f * idxR) : 1.f;
       const float wr0 = idxR > 0 ? 1.f - wr1 : 1.f;
       for (int f = 0; f < F; ++f) {
         const cpx<double> *p = Ez + (long long)slot * nr * F + nr * f + idxR;
@@ -138,9 +177,7 @@ __device__ inline void push_tg_f32(const GridArgs &g, const LatArgs &l, int slot
           sqrtf((1 + ka) * (px * px + uef * uef * py * py) + (1 - ka) * (px * px * uef * uef + py * py));
     const float B = atanf(2.0f * ka * uef * (px - py) /
                           ((1.0f + ka) * (px + uef * uef * py) + (1 - ka) * (px * uef * uef + py)));
-    float sb, cb;
-    sincosf(B - (float)(l.Phi * 0.5), &sb, &cb);
-    expP = mk(cb, sb);
+    expP = expi_f32((double)B - l.Phi * 0.5);
   }
   const cpx<float> etaX = mk((float)l.etaXr, (float)l.etaXi), etaY = mk((float)l.etaYr, (float)l.etaYi);
   cpx<float> cu[NH];
@@ -154,9 +191,10 @@ __device__ inline void push_tg_f32(const GridArgs &g, const LatArgs &l, int slot
     const cpx<float> uy =
         b0[0] * (wx0 * wy0) + b0[g.nx] * (wx0 * wy1) + b0[1] * (wx1 * wy0) + b0[g.nx + 1] * (wx1 * wy1);
     const int H = g.harm[h];
-    sumU2 += (norm2(ux) + norm2(uy)) / (2.0f * H * H);
-    const int idx = (int)((1.0f * H * xi) / (float)g.dB);
-    const float w1 = (H * xi / (float)g.dB) - idx, w0 = 1.f - w1;
+    sumU2 += (norm2(ux) + norm2(uy)) * a.hu2[h];
+    const float xb = xi * a.hdb[h];
+    const int idx = (int)xb;
+    const float w1 = xb - idx, w0 = 1.f - w1;
     cpx<float> fx = mk(0.f, 0.f), fy = mk(0.f, 0.f);
     if (idx < 0 || idx + 1 >= g.bn) {
       atomicOr(err, EB_BESSEL);
@@ -191,30 +229,33 @@ __device__ inline void push_tg_f32(const GridArgs &g, const LatArgs &l, int slot
     const double thm = th0 + dz * kth * cmv[m];
     const double gm = gam0 + dz * kg * cmv[m];
     const cpx<float> e1 = expi_f32(thm + halfPhi);
-    const float igm = (float)(0.5 / gm);
+    const double igd = 1.0 / gm;
+    const float igm = (float)(0.5 * igd);
+    cpx<float> eh[NH];
+    harm_powers<NH>(e1, a.hstep, eh);
     float sumTh = 0.f, sumG = 0.f;
 #pragma unroll
     for (int h = 0; h < NH; ++h) {
-      const int H = g.harm[h];
-      const cpx<float> t = cpow_int(e1, H) * cu[h];
-      sumTh -= t.im / (float)H;
+      const cpx<float> t = eh[h] * cu[h];
+      sumTh -= t.im * a.hinv[h];
       sumG += t.re * igm;
     }
     const cpx<float> eth = e1 * rotm;  // e^{i theta}
     const float sumEz = 2.0f * (Em.re * eth.re - Em.im * eth.im);
-    const double c2 = ks / (2.0 * gm * gm);
+    const double c2 = ks * 0.5 * igd * igd;
     kth = (ku - c2 * K2) - (double)((float)c2 * (sumU2 + sq * sumTh));
     kg = (double)(ks_f * sq * sumG + e_mc2 * sumEz);
     sth += wt[m] * kth;
     sg += wt[m] * kg;
   }
-  const double dth = dz / 6.0 * sth;
+  const double dz6 = dz * (1.0 / 6.0);
+  const double dth = dz6 * sth;
   if (dth > g.dPhi || dth < -g.dPhi) {
     atomicOr(err, EB_GUARD);
     return;
   }
   th = th0 + dth;
-  gam = gam0 + (dz / 6.0 * sg - dz * wakeE);
+  gam = gam0 + (dz6 * sg - dz * wakeE);
 }
 
 // One workgroup of kTile threads per tile, one particle per thread.
@@ -225,8 +266,8 @@ __global__ void __launch_bounds__(kTile)
             const double *__restrict__ seg_w, const cpx<float> *__restrict__ Ux, const cpx<float> *__restrict__ Uy,
             const cpx<double> *__restrict__ Ez, const double *__restrict__ bes, const double *__restrict__ wakeE,
             double *__restrict__ tile_red, unsigned *err) {
-  __shared__ double red[(2 * NH + 1 + 7) * (kTile / 64)];
-  const int t = blockIdx.x, k = threadIdx.x, lane = k & 63, wv = k >> 6;
+  __shared__ double red[(2 * NH + 1 + 7) * kTile];
+  const int t = blockIdx.x, k = threadIdx.x;
   const int seg = tl.seg[t];
   const int sl = seg % g.slice_n, slot = sl + 1;
   const bool on = k < tl.cnt[t];
@@ -246,13 +287,13 @@ __global__ void __launch_bounds__(kTile)
     transverse_f32(a.l0, dzh, gf, xx, yy, ppx, ppy);
     transverse_f32(a.l1, dzh, gf, xx, yy, ppx, ppy);
     if (!a.drift) {
-      push_tg_f32<NH>(g, a.l1, slot, wk, Ux, Uy, Ez, bes, xx, yy, ppx, ppy, th, gm, err);
+      push_tg_f32<NH>(g, a, a.l1, slot, wk, Ux, Uy, Ez, bes, xx, yy, ppx, ppy, th, gm, err);
     } else {
-      const double xd = xx, yd = yy;
-      const int idxI = (int)((xd - g.minX) / g.dx), idxJ = (int)((yd - g.minY) / g.dy);
+      const double fx_ = ((double)xx - g.minX) * a.inv_dx, fy_ = ((double)yy - g.minY) * a.inv_dy;
+      const int idxI = (int)fx_, idxJ = (int)fy_;
       if (idxI >= 0 && idxI < g.nx - 1 && idxJ >= 0 && idxJ < g.ny - 1) {
-        const float wx1 = (float)((xd - g.minX) / g.dx - idxI), wx0 = 1.f - wx1;
-        const float wy1 = (float)((yd - g.minY) / g.dy - idxJ), wy0 = 1.f - wy1;
+        const float wx1 = (float)(fx_ - idxI), wx0 = 1.f - wx1;
+        const float wy1 = (float)(fy_ - idxJ), wy0 = 1.f - wy1;
         const long long base =
             (long long)phys_slot(slot, g.ring_base, g.S) * g.N + (long long)idxJ * g.nx + idxI;
         float sumU2 = 0.f;
@@ -264,8 +305,7 @@ __global__ void __launch_bounds__(kTile)
               a0[0] * (wx0 * wy0) + a0[g.nx] * (wx0 * wy1) + a0[1] * (wx1 * wy0) + a0[g.nx + 1] * (wx1 * wy1);
           const cpx<float> uy =
               b0[0] * (wx0 * wy0) + b0[g.nx] * (wx0 * wy1) + b0[1] * (wx1 * wy0) + b0[g.nx + 1] * (wx1 * wy1);
-          const int H = g.harm[h];
-          sumU2 += (norm2(ux) + norm2(uy)) / (2.0f * H * H);
+          sumU2 += (norm2(ux) + norm2(uy)) * a.hu2[h];
         }
         const double K2 = 1.0 + (double)(ppx * ppx + ppy * ppy);
         const double tt = a.l1.ku - g.ks / (2.0 * a.gamR * a.gamR) * (K2 + (double)sumU2);
@@ -288,12 +328,12 @@ __global__ void __launch_bounds__(kTile)
     px[n] = ppx;
     py[n] = ppy;
     // moments of the next step's bFactor and twiss, accumulated in FP64
-    const cpx<float> e1 = expi_f32(th);
+    cpx<float> eh[NH];
+    harm_powers<NH>(expi_f32(th), a.hstep, eh);
 #pragma unroll
     for (int h = 0; h < NH; ++h) {
-      const cpx<float> eh = cpow_int(e1, g.harm[h]);
-      acc[2 * h] = (double)eh.re * w;
-      acc[2 * h + 1] = (double)eh.im * w;
+      acc[2 * h] = (double)eh[h].re * w;
+      acc[2 * h + 1] = (double)eh[h].im * w;
     }
     acc[2 * NH] = w;
     const double inv = 1.0 / gm;
@@ -306,17 +346,19 @@ __global__ void __launch_bounds__(kTile)
     acc[2 * NH + 6] = yv * yp;
     acc[2 * NH + 7] = 1.0;
   }
+  // tile sums in a fixed order: the moments go through LDS transposed, 16
+  // threads add 16 particles each per moment, then a 16-lane tree
 #pragma unroll
-  for (int q = 0; q < NV; ++q) {
-    double v = acc[q];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
-    if (lane == 0) red[q * (kTile / 64) + wv] = v;
-  }
+  for (int q = 0; q < NV; ++q) red[q * kTile + k] = acc[q];
   __syncthreads();
-  if (k < NV) {
-    double s = 0.0;
-    for (int u = 0; u < kTile / 64; ++u) s += red[k * (kTile / 64) + u];
-    tile_red[(long long)t * a.nred + k] = s;
+  const int part = k & 15;
+  for (int q = k >> 4; q < NV; q += kTile / 16) {
+    double v = 0.0;
+#pragma unroll
+    for (int u = 0; u < kTile / 16; ++u) v += red[q * kTile + part + 16 * u];
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 16);
+    if (part == 0) tile_red[(long long)q * gridDim.x + t] = v;
   }
 }
 
@@ -345,8 +387,8 @@ __global__ void __launch_bounds__(1024)
     const int seg = sp * g.slice_n + sl;
     const long long b = seg_off[seg], e = seg_off[seg + 1];
     for (long long n = b + tid; n < e; n += nt) {
-      const double xd = x[n], yd = y[n];
-      const int idxI = (int)((xd - g.minX) / g.dx), idxJ = (int)((yd - g.minY) / g.dy);
+      // the same cell index as the deposit below
+      const int idxI = (int)(((double)x[n] - g.minX) * d.inv_dx), idxJ = (int)(((double)y[n] - g.minY) * d.inv_dy);
       if (idxI >= 0 && idxI < g.nx - 1 && idxJ >= 0 && idxJ < g.ny - 1) {
         mi = min(mi, idxI); Mi = max(Mi, idxI); mj = min(mj, idxJ); Mj = max(Mj, idxJ);
       }
@@ -366,8 +408,42 @@ __global__ void __launch_bounds__(1024)
   bool any_even = false;
 #pragma unroll
   for (int h = 0; h < NH; ++h) any_even |= (g.harm[h] % 2 == 0);
-  const cpx<float> etaX = mk((float)d.etaXr, (float)d.etaXi), etaY = mk((float)d.etaYr, (float)d.etaYi);
   const float ka = (float)d.ka;
+  // per harmonic, from the lattice only: odd H keeps eta (J1 -+ ka J2) H as
+  // two complex factors; even H keeps J1, J2, J3 (solveField.cpp:270-300)
+  __shared__ float hfac[FELIS_MAX_HARM][4];
+  __shared__ int hbad;
+  if (tid == 0) hbad = 0;
+  __syncthreads();
+  if (tid < NH) {
+    const int H = g.harm[tid];
+    const int ib = (int)(H * d.xi * d.inv_dB);
+    const double w1 = H * d.xi * d.inv_dB - ib, w0 = 1.0 - w1;
+    if (ib < 0 || ib + 1 >= g.bn) {
+      hbad = 1;
+      hfac[tid][0] = hfac[tid][1] = hfac[tid][2] = hfac[tid][3] = 0.f;
+    } else if (H % 2 == 1) {
+      const double sign = ((H - 1) / 2 % 2 == 0) ? 1.0 : -1.0;
+      const double J1 = bessel_interp(bes, g.bo, ib, (H - 1) / 2, w0, w1);
+      const double J2 = bessel_interp(bes, g.bo, ib, (H + 1) / 2, w0, w1);
+      const double fx = sign * (J1 - d.ka * J2) * H, fy = sign * (J1 + d.ka * J2) * H;
+      hfac[tid][0] = (float)(d.etaXr * fx);
+      hfac[tid][1] = (float)(d.etaXi * fx);
+      hfac[tid][2] = (float)(d.etaYr * fy);
+      hfac[tid][3] = (float)(d.etaYi * fy);
+    } else {
+      hfac[tid][0] = (float)bessel_interp(bes, g.bo, ib, (H - 2) / 2, w0, w1);
+      hfac[tid][1] = (float)bessel_interp(bes, g.bo, ib, H / 2, w0, w1);
+      hfac[tid][2] = (float)bessel_interp(bes, g.bo, ib, (H + 2) / 2, w0, w1);
+      hfac[tid][3] = 0.f;
+    }
+  }
+  __syncthreads();
+  if (hbad) {
+    if (tid == 0) atomicOr(err, EB_BESSEL);
+    return;
+  }
+  const cpx<float> etaX = mk((float)d.etaXr, (float)d.etaXi), etaY = mk((float)d.etaYr, (float)d.etaYi);
   cpx<float> *win = reinterpret_cast<cpx<float> *>(lds_d);
   for (int h0 = 0; h0 < NH; h0 += step) {
     const int hc = min(step, NH - h0);
@@ -375,11 +451,10 @@ __global__ void __launch_bounds__(1024)
       for (int q = tid; q < hc * 2 * W; q += nt) win[q] = mk(0.f, 0.f);
       __syncthreads();
     }
-    bool bad = false;
     for (int sp = 0; sp < n_species; ++sp) {
       const int seg = sp * g.slice_n + sl;
       const long long b = seg_off[seg], e = seg_off[seg + 1];
-      const double macro = seg_w[seg];
+      const float macro_u = (float)(seg_w[seg] * d.coef_U);
       // each thread takes kRun consecutive particles; consecutive particles
       // in the same cell (the phases of a beamlet) are summed in registers
       // and reach the window with one set of atomics
@@ -415,8 +490,8 @@ __global__ void __launch_bounds__(1024)
     }                                                                                                           \
   }
       for (long long n = r0; n < r1; ++n) {
-        const double xd = x[n], yd = y[n];
-        const int idxI = (int)((xd - g.minX) / g.dx), idxJ = (int)((yd - g.minY) / g.dy);
+        const double fx_ = ((double)x[n] - g.minX) * d.inv_dx, fy_ = ((double)y[n] - g.minY) * d.inv_dy;
+        const int idxI = (int)fx_, idxJ = (int)fy_;
         if (!(idxI >= 0 && idxI < g.nx && idxJ >= 0 && idxJ < g.ny)) continue;
         const int k2 = idxJ * g.nx + idxI;
         if (k2 != key) {
@@ -431,10 +506,10 @@ __global__ void __launch_bounds__(1024)
           kin = hp > 0 && idxI < g.nx - 1 && idxJ < g.ny - 1;
         }
         float wx[2], wy[2];
-        wx[1] = (float)((xd - g.minX) / g.dx - idxI); wx[0] = 1.f - wx[1];
-        wy[1] = (float)((yd - g.minY) / g.dy - idxJ); wy[0] = 1.f - wy[1];
+        wx[1] = (float)(fx_ - idxI); wx[0] = 1.f - wx[1];
+        wy[1] = (float)(fy_ - idxJ); wy[0] = 1.f - wy[1];
         const double th = theta[n];
-        const double gm = g.gref + (double)gamma[n];
+        const float amp0 = __fdividef(macro_u, (float)(g.gref + (double)gamma[n]));
         float xi2 = 0.f;
         cpx<float> expP = mk(1.f, 0.f);
         if (any_even) {
@@ -443,41 +518,33 @@ __global__ void __launch_bounds__(1024)
                 sqrtf((1 + ka) * (ppx * ppx + ue * ue * ppy * ppy) + (1 - ka) * (ppx * ppx * ue * ue + ppy * ppy));
           const float B = atanf(2.0f * ka * ue * (ppx - ppy) /
                                 ((1.0f + ka) * (ppx + ue * ue * ppy) + (1 - ka) * (ppx * ue * ue + ppy)));
-          float sb, cb;
-          sincosf(B - (float)(d.Phi * 0.5), &sb, &cb);
-          expP = mk(cb, sb);
+          expP = expi_f32((double)B - d.Phi * 0.5);
         }
         const cpx<float> em1 = conjg(expi_f32(th + d.Phi * 0.5));  // e^{-i psi}
-        const float amp0 = (float)(macro * d.coef_U / gm);
+        const cpx<float> em2 = em1 * em1;
+        cpx<float> pw = cpow_int(em1, g.harm[h0]);
 #pragma unroll
         for (int hh = 0; hh < NH; ++hh) {
           if (hh >= hc) break;
           const int h = h0 + hh, H = g.harm[h];
-          const int ib = (int)((H * d.xi) / g.dB);
-          const float w1 = (float)((H * d.xi / g.dB) - ib), w0 = 1.f - w1;
-          if (ib < 0 || ib + 1 >= g.bn) {
-            bad = true;
-            continue;
+          if (hh > 0) {
+            const int dH = H - g.harm[h - 1];
+            pw = pw * (dH == 1 ? em1 : dH == 2 ? em2 : cpow_int(em1, dH));
           }
-          cpx<float> fx, fy;
+          const cpx<float> mc = pw * amp0;
+          cpx<float> cxv, cyv;
           if (H % 2 == 1) {
-            const float sign = ((H - 1) / 2 % 2 == 0) ? 1.f : -1.f;
-            const float J1 = (float)bessel_interp(bes, g.bo, ib, (H - 1) / 2, w0, w1);
-            const float J2 = (float)bessel_interp(bes, g.bo, ib, (H + 1) / 2, w0, w1);
-            fx = mk(sign * (J1 - ka * J2), 0.f);
-            fy = mk(sign * (J1 + ka * J2), 0.f);
+            cxv = mk(hfac[h][0], hfac[h][1]) * mc;
+            cyv = mk(hfac[h][2], hfac[h][3]) * mc;
           } else {
-            const float sign = (H / 2 % 2 == 0) ? 1.f : -1.f;
-            const float J1 = (float)bessel_interp(bes, g.bo, ib, (H - 2) / 2, w0, w1);
-            const float J2 = (float)bessel_interp(bes, g.bo, ib, H / 2, w0, w1);
-            const float J3 = (float)bessel_interp(bes, g.bo, ib, (H + 2) / 2, w0, w1);
-            const float s = sign * xi2 * H * 0.5f;
+            const float J1 = hfac[h][0], J2 = hfac[h][1], J3 = hfac[h][2];
+            const float s = ((H / 2 % 2 == 0) ? 1.f : -1.f) * xi2 * H * 0.5f * H;
             const cpx<float> expM = conjg(expP);
-            fx = (expM * (J1 - ka * J2) + expP * (J2 - ka * J3)) * s;
-            fy = (expM * (J1 + ka * J2) + expP * (J2 + ka * J3)) * s;
+            const cpx<float> fx = (expM * (J1 - ka * J2) + expP * (J2 - ka * J3)) * s;
+            const cpx<float> fy = (expM * (J1 + ka * J2) + expP * (J2 + ka * J3)) * s;
+            cxv = etaX * fx * mc;
+            cyv = etaY * fy * mc;
           }
-          const cpx<float> mc = cpow_int(em1, H) * (amp0 * (float)H);
-          const cpx<float> cxv = etaX * fx * mc, cyv = etaY * fy * mc;
 #pragma unroll
           for (int c = 0; c < 4; ++c) {
             const float wgt = -(wx[c >> 1] * wy[c & 1]);
@@ -490,7 +557,6 @@ __global__ void __launch_bounds__(1024)
 #undef FELIS_DEPOSIT_FLUSH
       }
     }
-    if (bad) atomicOr(err, EB_BESSEL);
     if (hp > 0) {
       __syncthreads();
       for (int q = tid; q < hc * 2 * W; q += nt) {
@@ -550,33 +616,48 @@ __global__ void __launch_bounds__(1024)
 
 // out[o] for o < 2*nh: energy of pol o/nh, harmonic o%nh over the slices that
 // stay after the slippage; next 2*nh+1: bFactor sums; then 7 twiss sums per
-// species. One workgroup per output, strided partial sums then a fixed tree.
+// species. tile_red holds moment q of tile t at [q * ntiles + t]. One
+// workgroup per output: strided partial sums (four independent chains per
+// thread), then a fixed tree.
 __global__ void __launch_bounds__(256)
     k_reduce_diag(GridArgs g, const double *__restrict__ slice_e, int skip_last, const double *__restrict__ tile_red,
                   int nred, const int *__restrict__ species_tile, int n_species, double *__restrict__ out) {
   __shared__ double sh[256];
   const int o = blockIdx.x, tid = threadIdx.x, nh = g.nh;
-  double s = 0.0;
+  const long long ntiles = species_tile[n_species];
+  const double *row;
+  int k0, k1;
   if (o < 2 * nh) {
-    const double *row = slice_e + (long long)o * g.slice_n;
-    const int n = g.slice_n - (skip_last ? 1 : 0);
-    for (int k = tid; k < n; k += 256) s += row[k];
+    row = slice_e + (long long)o * g.slice_n;
+    k0 = 0;
+    k1 = g.slice_n - (skip_last ? 1 : 0);
   } else if (o < 4 * nh + 1) {
-    const int q = o - 2 * nh;
-    const int n = species_tile[n_species];
-    for (int k = tid; k < n; k += 256) s += tile_red[(long long)k * nred + q];
+    row = tile_red + (long long)(o - 2 * nh) * ntiles;
+    k0 = 0;
+    k1 = (int)ntiles;
   } else {
     const int q = o - (4 * nh + 1), sp = q / 7, c = q % 7;
-    for (int k = species_tile[sp] + tid; k < species_tile[sp + 1]; k += 256)
-      s += tile_red[(long long)k * nred + 2 * nh + 1 + c];
+    row = tile_red + (long long)(2 * nh + 1 + c) * ntiles;
+    k0 = species_tile[sp];
+    k1 = species_tile[sp + 1];
   }
-  sh[tid] = s;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int k = k0 + tid;
+  for (; k + 768 < k1; k += 1024) {
+    s0 += row[k];
+    s1 += row[k + 256];
+    s2 += row[k + 512];
+    s3 += row[k + 768];
+  }
+  for (; k < k1; k += 256) s0 += row[k];
+  sh[tid] = (s0 + s1) + (s2 + s3);
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (tid < w) sh[tid] += sh[tid + w];
     __syncthreads();
   }
   if (tid == 0) out[o] = sh[0];
+  (void)nred;
 }
 
 }  // namespace felis

@@ -261,6 +261,16 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
     a.drift = l1->drift_flag;
     a.nred = nred;
     a.gamR = d.gamR;
+    a.inv_dx = 1.0 / d.dx;
+    a.inv_dy = 1.0 / d.dy;
+    a.inv_dr = 1.0 / d.dr;
+    for (int h = 0; h < nh; ++h) {
+      const int H = d.harmony[h];
+      a.hinv[h] = (float)(1.0 / H);
+      a.hdb[h] = (float)(H / ctx->dB);
+      a.hu2[h] = (float)(1.0 / (2.0 * H * H));
+      a.hstep[h] = H - (h ? d.harmony[h - 1] : 0);
+    }
     if ((err = particles(ctx, a, tile_red))) break;
     // the next solveField's deposit, with the lattice it will run under
     if (l1->current_flag) {
