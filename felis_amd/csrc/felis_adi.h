@@ -1,0 +1,357 @@
+// Batched ADI of solveField.cpp:80-185 in FP32 for every (slice, harmonic,
+// pol) system of a step, one workgroup each.
+//
+// Per system the two half steps are tridiagonal solves along rows (x) and
+// columns (y) whose Thomas recurrences have field-independent coefficients:
+//   forward   DD_i = r_i dd_i + a_i DD_{i-1}
+//   backward  U_i  = q_i DD_i + p_i U_{i+1}
+// Scaling DD' = q DD folds q into the forward map,
+//   DD'_i = (q_i r_i) dd_i + (q_i a_i / q_{i-1}) DD'_{i-1},   U_i = DD'_i + p_i U_{i+1},
+// so both sweeps are first-order affine recurrences y_i = c_i y_{i-1} + b_i
+// whose multipliers c_i depend on the harmonic only. A line of n-2 unknowns is
+// held by a 16-lane DPP row, E consecutive unknowns per lane. Each lane runs
+// its segment from zero (the particular part), a 4-level Hillis-Steele scan
+// over the lanes combines segment ends, and the homogeneous part is added
+// back. The products of multipliers that the scan and the add-back need are
+// the same for every line, slice and step, so they are tabulated on the host
+// (adi_scan_tables) and each level of the scan is one complex FMA.
+//
+// The y-implicit right-hand side of the whole system lives in LDS between the
+// sweeps, in a skewed layout whose row and column strides per lane segment are
+// odd in 8-byte words, so the row-wise writes of the x sweep and the
+// column-wise reads of the y sweep both spread over the banks.
+#pragma once
+
+#include <vector>
+
+#include "felis_common.h"
+
+namespace felis {
+
+// unknowns per lane for a 16-lane line
+__host__ __device__ inline int adi_scan_E(int nx, int ny) {
+  const int n = (nx > ny ? nx : ny) - 2;
+  return (n + 15) / 16;
+}
+
+// skewed LDS layout of the (ny-2) x (nx-2) y right-hand side
+struct DyLayout {
+  int E, P, sc, sr;
+  __host__ __device__ DyLayout(int nx, int ny, int E_) : E(E_) {
+    const int nxi = nx - 2;
+    sc = (E % 2 == 0) ? 1 : 0;
+    P = nxi + sc * ((nxi - 1) / E);
+    sr = ((E * P) % 2 == 0) ? 1 : 0;
+    (void)ny;
+  }
+  __host__ __device__ int at(int r, int c) const { return r * P + c + sc * (c / E) + sr * (r / E); }
+  __host__ __device__ int size(int ny) const { return at(ny - 3, 0) + P; }
+};
+
+// Per harmonic table block (complex FP32), e-major entries [e*16 + lane]:
+//   x sweep: ap, rp, pp, alphaP, alpha, Af, Gb
+//   y sweep: ap, pp, Af, Gb
+//   rows j:  betaP[ny], beta[ny], y-rp[ny]
+//   scan:    Cf_x[4][16], Cb_x[4][16], Cf_y[4][16], Cb_y[4][16]
+struct AdiTabOff {
+  int xap, xrp, xpp, xaP, xaM, xAf, xGb, yap, ypp, yAf, yGb, rbP, rbM, ryrp, scan, block, staged;
+  __host__ __device__ AdiTabOff(int E, int ny) {
+    const int s = 16 * E;
+    xap = 0; xrp = s; xpp = 2 * s; xaP = 3 * s; xaM = 4 * s; xAf = 5 * s; xGb = 6 * s;
+    yap = 7 * s; ypp = 8 * s; yAf = 9 * s; yGb = 10 * s;
+    rbP = 11 * s; rbM = rbP + ny; ryrp = rbM + ny;
+    scan = ryrp + ny;
+    block = scan + 4 * 4 * 16;
+    staged = scan;  // everything but the scan constants is staged in LDS
+  }
+};
+
+inline size_t adi_scan_lds(int nx, int ny) {
+  const int E = adi_scan_E(nx, ny);
+  DyLayout L(nx, ny, E);
+  AdiTabOff o(E, ny);
+  return ((size_t)L.size(ny) + o.staged) * sizeof(cpx<float>);
+}
+
+// Host: the tables above from the hoisted Thomas coefficients (r, a, q, p per
+// index, as in the plain path) and the absorbing-layer stencil.
+inline void adi_scan_tables(int nx, int ny, int nh, const std::vector<cpx<double>> &adix,
+                            const std::vector<cpx<double>> &adiy, const std::vector<cpx<double>> &abc,
+                            std::vector<cpx<float>> &out) {
+  const int E = adi_scan_E(nx, ny);
+  AdiTabOff o(E, ny);
+  out.assign((size_t)nh * o.block, mk(0.f, 0.f));
+  auto cm = [](cpx<double> a, cpx<double> b) { return mk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); };
+  auto cinv = [](cpx<double> a) {
+    const double d = a.re * a.re + a.im * a.im;
+    return mk(a.re / d, -a.im / d);
+  };
+  auto f32 = [](cpx<double> a) { return mk((float)a.re, (float)a.im); };
+  for (int h = 0; h < nh; ++h) {
+    cpx<float> *T = out.data() + (size_t)h * o.block;
+    for (int dir = 0; dir < 2; ++dir) {
+      const int n = dir ? ny : nx;
+      const cpx<double> *c = (dir ? adiy.data() : adix.data()) + (size_t)h * 4 * n;
+      const cpx<double> *r = c, *a = c + n, *q = c + 2 * n, *p = c + 3 * n;
+      std::vector<cpx<double>> ap(16 * E), rp(16 * E), pp(16 * E), Af(16 * E), Gb(16 * E);
+      std::vector<cpx<double>> At(16), Gt(16);
+      for (int sub = 0; sub < 16; ++sub) {
+        for (int e = 0; e < E; ++e) {
+          const int i = 1 + sub * E + e, k = e * 16 + sub;
+          const bool valid = i <= n - 2;
+          ap[k] = valid && i > 1 ? cm(cm(q[i], a[i]), cinv(q[i - 1])) : mk(0.0, 0.0);
+          rp[k] = valid ? cm(q[i], r[i]) : mk(0.0, 0.0);
+          pp[k] = valid ? p[i] : mk(0.0, 0.0);
+        }
+        cpx<double> acc = mk(1.0, 0.0);
+        for (int e = 0; e < E; ++e) {
+          acc = cm(acc, ap[e * 16 + sub]);
+          Af[e * 16 + sub] = acc;
+        }
+        At[sub] = acc;
+        acc = mk(1.0, 0.0);
+        for (int e = E - 1; e >= 0; --e) {
+          acc = cm(acc, pp[e * 16 + sub]);
+          Gb[e * 16 + sub] = acc;
+        }
+        Gt[sub] = acc;
+      }
+      cpx<float> *Cf = T + o.scan + (dir ? 2 : 0) * 64, *Cb = Cf + 64;
+      for (int lv = 0; lv < 4; ++lv) {
+        const int dd = 1 << lv;
+        for (int sub = 0; sub < 16; ++sub) {
+          cpx<double> f = mk(1.0, 0.0), b = mk(1.0, 0.0);
+          for (int m = (sub - dd + 1 > 0 ? sub - dd + 1 : 0); m <= sub; ++m) f = cm(f, At[m]);
+          for (int m = sub; m <= (sub + dd - 1 < 15 ? sub + dd - 1 : 15); ++m) b = cm(b, Gt[m]);
+          Cf[lv * 16 + sub] = f32(f);
+          Cb[lv * 16 + sub] = f32(b);
+        }
+      }
+      for (int k = 0; k < 16 * E; ++k) {
+        if (dir == 0) {
+          T[o.xap + k] = f32(ap[k]);
+          T[o.xrp + k] = f32(rp[k]);
+          T[o.xpp + k] = f32(pp[k]);
+          T[o.xAf + k] = f32(Af[k]);
+          T[o.xGb + k] = f32(Gb[k]);
+          const int i = 1 + (k % 16) * E + k / 16;
+          if (i <= nx - 2) {
+            T[o.xaP + k] = f32(abc[nx + i]);
+            T[o.xaM + k] = f32(abc[i]);
+          }
+        } else {
+          T[o.yap + k] = f32(ap[k]);
+          T[o.ypp + k] = f32(pp[k]);
+          T[o.yAf + k] = f32(Af[k]);
+          T[o.yGb + k] = f32(Gb[k]);
+        }
+      }
+      if (dir == 1)
+        for (int j = 1; j <= ny - 2; ++j) T[o.ryrp + j] = f32(cm(q[j], r[j]));
+    }
+    for (int j = 0; j < ny; ++j) {
+      T[o.rbP + j] = f32(abc[2 * nx + ny + j]);
+      T[o.rbM + j] = f32(abc[2 * nx + j]);
+    }
+  }
+}
+
+#ifdef __HIPCC__
+
+// lane l of each 16-lane row reads lane l-D (l+D); lanes without a source read 0
+template <int D>
+__device__ inline float dpp_shr(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x110 + D, 0xF, 0xF, true));
+}
+template <int D>
+__device__ inline float dpp_shl(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x100 + D, 0xF, 0xF, true));
+}
+
+__device__ inline cpx<float> cfma(cpx<float> c, cpx<float> x, cpx<float> b) {  // c*x + b
+  return mk(fmaf(c.re, x.re, fmaf(-c.im, x.im, b.re)), fmaf(c.re, x.im, fmaf(c.im, x.re, b.im)));
+}
+
+// forward scan of lane-segment ends: after it, B of lane l is the recurrence
+// value at the end of lane l's segment
+__device__ inline cpx<float> scan_fwd(cpx<float> B, const cpx<float> (&C)[4]) {
+  B = cfma(C[0], mk(dpp_shr<1>(B.re), dpp_shr<1>(B.im)), B);
+  B = cfma(C[1], mk(dpp_shr<2>(B.re), dpp_shr<2>(B.im)), B);
+  B = cfma(C[2], mk(dpp_shr<4>(B.re), dpp_shr<4>(B.im)), B);
+  B = cfma(C[3], mk(dpp_shr<8>(B.re), dpp_shr<8>(B.im)), B);
+  return mk(dpp_shr<1>(B.re), dpp_shr<1>(B.im));  // value entering the lane
+}
+__device__ inline cpx<float> scan_bwd(cpx<float> B, const cpx<float> (&C)[4]) {
+  B = cfma(C[0], mk(dpp_shl<1>(B.re), dpp_shl<1>(B.im)), B);
+  B = cfma(C[1], mk(dpp_shl<2>(B.re), dpp_shl<2>(B.im)), B);
+  B = cfma(C[2], mk(dpp_shl<4>(B.re), dpp_shl<4>(B.im)), B);
+  B = cfma(C[3], mk(dpp_shl<8>(B.re), dpp_shl<8>(B.im)), B);
+  return mk(dpp_shl<1>(B.re), dpp_shl<1>(B.im));
+}
+
+// One workgroup (512 threads) per system sys = (pol*nh + h)*slice_n + sl.
+template <int E>
+__global__ void __launch_bounds__(512)
+    k_adi_scan(GridArgs g, const cpx<float> *__restrict__ tab, int cf, cpx<float> *__restrict__ U,
+               cpx<float> *__restrict__ Sc, const int4 *__restrict__ win, double *__restrict__ slice_e) {
+  extern __shared__ double lds_d[];
+  __shared__ double red[8];
+  const int nx = g.nx, ny = g.ny, nxi = nx - 2, nyi = ny - 2;
+  const DyLayout L(nx, ny, E);
+  const AdiTabOff o(E, ny);
+  cpx<float> *DY = reinterpret_cast<cpx<float> *>(lds_d);
+  cpx<float> *T = DY + L.size(ny);
+  const int sys = blockIdx.x;
+  const int sl = sys % g.slice_n, ph = sys / g.slice_n, h = ph % g.nh, slot = sl + 1;
+  const float H = (float)g.harm[h];
+  cpx<float> *Un = U + ((long long)ph * g.S + phys_slot(slot, g.ring_base, g.S)) * g.N;
+  cpx<float> *Sn = Sc + ((long long)ph * g.S + slot) * g.N;
+  const cpx<float> *Th = tab + (long long)h * o.block;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+  for (int k = tid; k < o.staged; k += blockDim.x) T[k] = Th[k];
+  const int grp = lane >> 4, sub = lane & 15;
+  cpx<float> Cf[4], Cb[4];
+#pragma unroll
+  for (int lv = 0; lv < 4; ++lv) {
+    Cf[lv] = Th[o.scan + lv * 16 + sub];
+    Cb[lv] = Th[o.scan + 64 + lv * 16 + sub];
+  }
+  __syncthreads();
+  const int i0 = 1 + sub * E;
+  // Sc is nonzero only inside this slot's deposit window (written by the
+  // deposit kernel); cells outside it are neither read nor cleared.
+  const int4 wn = cf ? win[slot] : make_int4(1, 0, 1, 0);
+  // ---- x-implicit half step, 4 rows per wave per pass ----
+  cpx<float> qm[E], q0[E], qp[E], qs[E];
+#define FELIS_ADI_LOADX(JB)                                                                   \
+  {                                                                                           \
+    const int jl = 1 + (JB) + wv * 4 + grp;                                                   \
+    _Pragma("unroll") for (int e = 0; e < E; ++e) {                                           \
+      const int i = i0 + e;                                                                   \
+      if (jl <= ny - 2 && i <= nx - 2) {                                                      \
+        const long long idx = (long long)jl * nx + i;                                         \
+        qm[e] = Un[idx - nx];                                                                 \
+        q0[e] = Un[idx];                                                                      \
+        qp[e] = Un[idx + nx];                                                                 \
+        qs[e] = (i >= wn.x && i <= wn.y && jl >= wn.z && jl <= wn.w) ? Sn[idx] : mk(0.f, 0.f); \
+      } else {                                                                                \
+        qm[e] = q0[e] = qp[e] = qs[e] = mk(0.f, 0.f);                                         \
+      }                                                                                       \
+    }                                                                                         \
+  }
+  FELIS_ADI_LOADX(0);
+  for (int jb = 0; jb < nyi; jb += nw * 4) {
+    const int j = 1 + jb + wv * 4 + grp;
+    const bool rowok = j <= ny - 2;
+    const int jj = rowok ? j : 1;
+    const cpx<float> bP = T[o.rbP + jj], bM = T[o.rbM + jj], yrp = T[o.ryrp + jj];
+    const cpx<float> diag = mk(H + bP.re + bM.re, bP.im + bM.im);
+    cpx<float> y[E], sc[E];
+    cpx<float> B = mk(0.f, 0.f);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int k = e * 16 + sub;
+      sc[e] = qs[e];
+      // x-implicit right-hand side: y-explicit stencil of U plus the source
+      cpx<float> dd = diag * q0[e] - bP * qp[e] - bM * qm[e] + sc[e];
+      B = cfma(T[o.xap + k], B, T[o.xrp + k] * dd);
+      y[e] = B;
+    }
+    if (jb + nw * 4 < nyi) FELIS_ADI_LOADX(jb + nw * 4);
+    const cpx<float> din = scan_fwd(B, Cf);
+    B = mk(0.f, 0.f);
+#pragma unroll
+    for (int e = E - 1; e >= 0; --e) {
+      const int k = e * 16 + sub;
+      const cpx<float> dp = cfma(T[o.xAf + k], din, y[e]);  // DD'
+      B = cfma(T[o.xpp + k], B, dp);
+      y[e] = B;
+    }
+    const cpx<float> uin = scan_bwd(B, Cb);
+#pragma unroll
+    for (int e = 0; e < E; ++e) y[e] = cfma(T[o.xGb + e * 16 + sub], uin, y[e]);  // Uc
+    // y-implicit right-hand side: x-explicit stencil of Uc (zero on the ring),
+    // times the y sweep's first multiplier for this row
+    const cpx<float> left = mk(dpp_shr<1>(y[E - 1].re), dpp_shr<1>(y[E - 1].im));
+    const cpx<float> right = mk(dpp_shl<1>(y[0].re), dpp_shl<1>(y[0].im));
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = i0 + e, k = e * 16 + sub;
+      if (!rowok || i > nx - 2) continue;
+      const cpx<float> um = e == 0 ? left : y[e - 1];
+      const cpx<float> up = e == E - 1 ? right : y[e + 1];
+      const cpx<float> v = y[e] * H + T[o.xaP + k] * (y[e] - up) + T[o.xaM + k] * (y[e] - um) + sc[e];
+      DY[L.at(j - 1, i - 1)] = yrp * v;
+      if (i >= wn.x && i <= wn.y && j >= wn.z && j <= wn.w) Sn[(long long)j * nx + i] = mk(0.f, 0.f);
+    }
+  }
+#undef FELIS_ADI_LOADX
+  __syncthreads();
+  // ---- y-implicit half step: columns, in place ----
+#pragma unroll
+  for (int lv = 0; lv < 4; ++lv) {
+    Cf[lv] = Th[o.scan + 128 + lv * 16 + sub];
+    Cb[lv] = Th[o.scan + 192 + lv * 16 + sub];
+  }
+  for (int cb = 0; cb < nxi; cb += nw * 4) {
+    const int col = cb + wv * 4 + grp;  // 0-based interior column
+    const bool colok = col < nxi;
+    cpx<float> y[E];
+    cpx<float> B = mk(0.f, 0.f);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int r = i0 + e - 1;  // 0-based interior row
+      const cpx<float> b = (colok && r < nyi) ? DY[L.at(r, col)] : mk(0.f, 0.f);
+      B = cfma(T[o.yap + e * 16 + sub], B, b);
+      y[e] = B;
+    }
+    const cpx<float> din = scan_fwd(B, Cf);
+    B = mk(0.f, 0.f);
+#pragma unroll
+    for (int e = E - 1; e >= 0; --e) {
+      const int k = e * 16 + sub;
+      B = cfma(T[o.ypp + k], B, cfma(T[o.yAf + k], din, y[e]));
+      y[e] = B;
+    }
+    const cpx<float> uin = scan_bwd(B, Cb);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int r = i0 + e - 1;
+      if (colok && r < nyi) DY[L.at(r, col)] = cfma(T[o.yGb + e * 16 + sub], uin, y[e]);
+    }
+  }
+  __syncthreads();
+  // ---- coalesced store of the interior and the slice energy (FP64) ----
+  double es = 0.0;
+  for (int q = tid; q < nxi * nyi; q += blockDim.x) {
+    const int jr = q / nxi, ic = q - jr * nxi;
+    const cpx<float> v = DY[L.at(jr, ic)];
+    Un[(long long)(jr + 1) * nx + ic + 1] = v;
+    es += (double)v.re * v.re + (double)v.im * v.im;
+  }
+  // the boundary ring is untouched by the ADI but counts in the energy
+  const int nring = 2 * nx + 2 * (ny - 2);
+  for (int k = tid; k < nring; k += blockDim.x) {
+    long long idx;
+    if (k < nx) idx = k;
+    else if (k < 2 * nx) idx = (long long)(ny - 1) * nx + (k - nx);
+    else {
+      int r = k - 2 * nx;
+      idx = (long long)(1 + (r >> 1)) * nx + ((r & 1) ? nx - 1 : 0);
+    }
+    cpx<float> v = Un[idx];
+    es += (double)v.re * v.re + (double)v.im * v.im;
+  }
+  for (int off = 32; off > 0; off >>= 1) es += __shfl_down(es, off);
+  if (lane == 0) red[wv] = es;
+  __syncthreads();
+  if (tid == 0) {
+    double s = 0.0;
+    for (int k = 0; k < nw; ++k) s += red[k];
+    slice_e[sys] = s;
+  }
+}
+
+#endif  // __HIPCC__
+
+}  // namespace felis

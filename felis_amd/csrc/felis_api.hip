@@ -862,6 +862,10 @@ static int launch_adi(felis_ctx *ctx, int cf) {
   return FELIS_OK;
 }
 
+int felis_ctx::adi_only(int current_flag) {
+  return mixed() ? launch_adi<float>(this, current_flag) : launch_adi<double>(this, current_flag);
+}
+
 int felis_ctx::solve_field(const felis_lattice *lat) {
   felis_ctx *ctx = this;
   // solveField.cpp:54-58 zeroes only the first slice_n+2 entries of each Sc;

@@ -97,6 +97,7 @@ struct felis_ctx {
 
   // per-phase operators (enqueue only; the C-ABI wrappers check errors)
   int solve_field(const felis_lattice *lat);
+  int adi_only(int current_flag);  // the two ADI half steps on U with Sc as source
   int transverse(const felis_lattice *lat);
   int push_theta_gamma(const felis_lattice *lat);
   int drift(const felis_lattice *lat);

@@ -14,7 +14,7 @@
 #include <vector>
 
 #include "felis_ctx.h"
-#include "felis_fused.h"
+#include "felis_adi.h"
 #include "felis_pstep.h"
 
 using namespace felis;
@@ -55,49 +55,56 @@ int enqueue_step(felis_ctx *ctx, const felis_lattice *lat, double *e_out, double
   return FELIS_OK;
 }
 
-template <typename T>
-int launch_adi_fused(felis_ctx *ctx, int cf) {
+// The batched ADI over every (slice, harmonic, pol) system with the scan
+// tables of adi_scan_tables (device copy in tab).
+int launch_adi_scan(felis_ctx *ctx, const cpx<float> *tab, int cf) {
   GridArgs g = ctx->grid();
-  AdiCoef c{ctx->adix, ctx->adiy, ctx->abc};
   const felis_desc &d = ctx->d;
   const int nsys = 2 * d.n_harm * d.slice_n;
-  const size_t lds = adi_fused_lds<T>(d.nx, d.ny);
-  // 16-lane scans along rows (x) and columns (y): E elements per lane
-  const int E = (std::max(d.nx, d.ny) - 2 + 15) / 16;
-  cpx<T> *U = (cpx<T> *)ctx->U, *Sc = (cpx<T> *)ctx->Sc;
+  const size_t lds = adi_scan_lds(d.nx, d.ny);
+  const int E = adi_scan_E(d.nx, d.ny);
+  cpx<float> *U = (cpx<float> *)ctx->U, *Sc = (cpx<float> *)ctx->Sc;
   KTimer kt(ctx, FELIS_K_ADI_FUSED);
-#define ADI_CASE(e)                                                                                     \
-  case e:                                                                                               \
-    hipLaunchKernelGGL((k_adi_fused<T, e>), dim3(nsys), dim3(512), lds, ctx->stream, g, c, cf, U, Sc, \
-                       ctx->win, ctx->slice_e);                                                          \
+#define ADI_CASE(e)                                                                                       \
+  case e:                                                                                                 \
+    hipLaunchKernelGGL((k_adi_scan<e>), dim3(nsys), dim3(512), lds, ctx->stream, g, tab, cf, U, Sc, ctx->win, \
+                       ctx->slice_e);                                                                      \
     break;
   switch (E) {
-    ADI_CASE(1) ADI_CASE(2) ADI_CASE(3) ADI_CASE(4) ADI_CASE(5) ADI_CASE(6) ADI_CASE(7) ADI_CASE(8)
-    ADI_CASE(9) ADI_CASE(10) ADI_CASE(11) ADI_CASE(12) ADI_CASE(13) ADI_CASE(14) ADI_CASE(15) ADI_CASE(16)
+    ADI_CASE(1) ADI_CASE(2) ADI_CASE(3) ADI_CASE(4) ADI_CASE(5) ADI_CASE(6) ADI_CASE(7) ADI_CASE(8) ADI_CASE(9)
     default:
-      ctx->err = "grid too wide for the fused ADI";
+      ctx->err = "grid too wide for the LDS-resident ADI";
       return FELIS_ERR_UNSUPPORTED;
   }
 #undef ADI_CASE
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
-    ctx->err = std::string("adi_fused launch: ") + hipGetErrorString(e);
+    ctx->err = std::string("adi launch: ") + hipGetErrorString(e);
     return FELIS_ERR_HIP;
   }
   return FELIS_OK;
 }
 
-template <typename T>
-hipError_t set_adi_lds(size_t lds) {
-  hipError_t e = hipSuccess;
-#define ADI_ATTR(x)                                                                                     \
-  if (e == hipSuccess)                                                                                  \
-    e = hipFuncSetAttribute((const void *)k_adi_fused<T, x>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                            (int)lds);
-  ADI_ATTR(1) ADI_ATTR(2) ADI_ATTR(3) ADI_ATTR(4) ADI_ATTR(5) ADI_ATTR(6) ADI_ATTR(7) ADI_ATTR(8)
-  ADI_ATTR(9) ADI_ATTR(10) ADI_ATTR(11) ADI_ATTR(12) ADI_ATTR(13) ADI_ATTR(14) ADI_ATTR(15) ADI_ATTR(16)
-#undef ADI_ATTR
-  return e;
+// true when the system's right-hand side and tables fit the LDS of one
+// workgroup; sets the kernel's dynamic LDS limit
+bool adi_scan_fits(felis_ctx *ctx) {
+  const felis_desc &d = ctx->d;
+  const size_t lds = adi_scan_lds(d.nx, d.ny);
+  const int E = adi_scan_E(d.nx, d.ny);
+  if (E > 9 || lds + 256 > (size_t)ctx->lds_max + 1024) return false;
+  const void *fn = nullptr;
+  switch (E) {
+    case 1: fn = (const void *)k_adi_scan<1>; break;
+    case 2: fn = (const void *)k_adi_scan<2>; break;
+    case 3: fn = (const void *)k_adi_scan<3>; break;
+    case 4: fn = (const void *)k_adi_scan<4>; break;
+    case 5: fn = (const void *)k_adi_scan<5>; break;
+    case 6: fn = (const void *)k_adi_scan<6>; break;
+    case 7: fn = (const void *)k_adi_scan<7>; break;
+    case 8: fn = (const void *)k_adi_scan<8>; break;
+    default: fn = (const void *)k_adi_scan<9>; break;
+  }
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess;
 }
 
 template <int NH>
@@ -179,11 +186,9 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
     ctx->err = "the batched path models one longitudinal space-charge mode (number_longitudinal_mode = 1)";
     return FELIS_ERR_UNSUPPORTED;
   }
-  const size_t lds_adi = adi_fused_lds<float>(d.nx, d.ny);
-  if (lds_adi + 512 > (size_t)ctx->lds_max + 1024 || set_adi_lds<float>(lds_adi) != hipSuccess) {
-    ctx->err = "grid too large for the LDS-resident ADI";
-    return FELIS_ERR_UNSUPPORTED;
-  }
+  // LDS-resident scan ADI when the system fits, else the two-pass ADI with a
+  // full clear of Sc after each use
+  const bool fused = adi_scan_fits(ctx);
   const int dep_lds = ctx->lds_max;
   if (deposit_lds_attr(d.n_harm, dep_lds) != hipSuccess) {
     ctx->err = "hipFuncSetAttribute(k_deposit_slot)";
@@ -191,19 +196,27 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
   }
   const int nh = d.n_harm, ns = d.n_species;
   const int nred = 2 * nh + 1 + 7;
-  double *tile_red = nullptr;
+  double *tile_red = nullptr, *scratch_e = nullptr;
   int *species_tile = nullptr;
-  RCHK(hipMalloc((void **)&tile_red, (size_t)std::max(1, ctx->ntiles) * nred * sizeof(double)));
-  if (hipMalloc((void **)&species_tile, (ns + 1) * sizeof(int)) != hipSuccess) {
-    (void)hipFree(tile_red);
-    ctx->err = "hipMalloc species_tile";
-    return FELIS_ERR_HIP;
-  }
+  cpx<float> *adi_tab = nullptr;
+  std::vector<cpx<float>> h_tab;
+  adi_scan_tables(d.nx, d.ny, nh, ctx->h_adix, ctx->h_adiy, ctx->h_abc, h_tab);
   auto cleanup = [&]() {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(tile_red);
     (void)hipFree(species_tile);
+    (void)hipFree(adi_tab);
+    (void)hipFree(scratch_e);
   };
+  if (hipMalloc((void **)&tile_red, (size_t)std::max(1, ctx->ntiles) * nred * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&species_tile, (ns + 1) * sizeof(int)) != hipSuccess ||
+      hipMalloc((void **)&adi_tab, h_tab.size() * sizeof(cpx<float>)) != hipSuccess ||
+      hipMalloc((void **)&scratch_e, 2 * nh * sizeof(double)) != hipSuccess ||
+      hipMemcpy(adi_tab, h_tab.data(), h_tab.size() * sizeof(cpx<float>), hipMemcpyHostToDevice) != hipSuccess) {
+    cleanup();
+    ctx->err = "batched step buffers";
+    return FELIS_ERR_HIP;
+  }
   if (hipMemcpy(species_tile, ctx->species_tile.data(), (ns + 1) * sizeof(int), hipMemcpyHostToDevice) !=
       hipSuccess) {
     cleanup();
@@ -241,7 +254,15 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
     }
     // solveField: the radial Ez from the last deposit, then the ADI
     if (ls->current_flag && (err = ctx->ez_solve(ls))) break;
-    if ((err = launch_adi_fused<float>(ctx, ls->current_flag))) break;
+    if (fused) {
+      if ((err = launch_adi_scan(ctx, adi_tab, ls->current_flag))) break;
+    } else {
+      if ((err = ctx->adi_only(ls->current_flag)) || (err = ctx->energy(scratch_e))) break;
+      if (ls->current_flag) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(4096), dim3(256), 0, ctx->stream, (unsigned *)ctx->Sc,
+                           (long long)(fsz / sizeof(unsigned)), 0u);
+      }
+    }
     // particles: transverse x2, theta/gamma or drift, wrap, diagnostic moments
     PStepArgs a{};
     if (l0->und_type != 1 && l0->und_type != 2) {
