@@ -189,6 +189,8 @@ __device__ inline cpx<float> scan_bwd(cpx<float> B, const cpx<float> (&C)[4]) {
   return mk(dpp_shl<1>(B.re), dpp_shl<1>(B.im));
 }
 
+__device__ const cpx<float> kZeroCell = {0.f, 0.f};
+
 // One workgroup (512 threads) per system sys = (pol*nh + h)*slice_n + sl.
 template <int E>
 __global__ void __launch_bounds__(512)
@@ -222,22 +224,23 @@ __global__ void __launch_bounds__(512)
   // deposit kernel); cells outside it are neither read nor cleared.
   const int4 wn = cf ? win[slot] : make_int4(1, 0, 1, 0);
   // ---- x-implicit half step, 4 rows per wave per pass ----
+  // Loads are unconditional: out-of-line elements read a clamped in-slot
+  // address and their multipliers in the tables are zero; Sc outside the
+  // window reads a zero cell.
   cpx<float> qm[E], q0[E], qp[E], qs[E];
-#define FELIS_ADI_LOADX(JB)                                                                   \
-  {                                                                                           \
-    const int jl = 1 + (JB) + wv * 4 + grp;                                                   \
-    _Pragma("unroll") for (int e = 0; e < E; ++e) {                                           \
-      const int i = i0 + e;                                                                   \
-      if (jl <= ny - 2 && i <= nx - 2) {                                                      \
-        const long long idx = (long long)jl * nx + i;                                         \
-        qm[e] = Un[idx - nx];                                                                 \
-        q0[e] = Un[idx];                                                                      \
-        qp[e] = Un[idx + nx];                                                                 \
-        qs[e] = (i >= wn.x && i <= wn.y && jl >= wn.z && jl <= wn.w) ? Sn[idx] : mk(0.f, 0.f); \
-      } else {                                                                                \
-        qm[e] = q0[e] = qp[e] = qs[e] = mk(0.f, 0.f);                                         \
-      }                                                                                       \
-    }                                                                                         \
+#define FELIS_ADI_LOADX(JB)                                                        \
+  {                                                                                \
+    const int jl = 1 + (JB) + wv * 4 + grp;                                        \
+    const int jc = jl <= ny - 2 ? jl : ny - 2;                                     \
+    const bool jw = jl >= wn.z && jl <= wn.w;                                      \
+    _Pragma("unroll") for (int e = 0; e < E; ++e) {                                \
+      const int i = i0 + e;                                                        \
+      const int idx = jc * nx + (i <= nx - 2 ? i : nx - 2);                        \
+      qm[e] = Un[idx - nx];                                                        \
+      q0[e] = Un[idx];                                                             \
+      qp[e] = Un[idx + nx];                                                        \
+      qs[e] = *((jw && i >= wn.x && i <= wn.y) ? Sn + idx : &kZeroCell);           \
+    }                                                                              \
   }
   FELIS_ADI_LOADX(0);
   for (int jb = 0; jb < nyi; jb += nw * 4) {
@@ -274,34 +277,43 @@ __global__ void __launch_bounds__(512)
     // times the y sweep's first multiplier for this row
     const cpx<float> left = mk(dpp_shr<1>(y[E - 1].re), dpp_shr<1>(y[E - 1].im));
     const cpx<float> right = mk(dpp_shl<1>(y[0].re), dpp_shl<1>(y[0].im));
+    const bool jwin = j >= wn.z && j <= wn.w;
+    // row j-1 of the layout, columns i0-1 .. i0-1+E-1 are consecutive
+    cpx<float> *dst = DY + L.at(j - 1, i0 - 1);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = i0 + e, k = e * 16 + sub;
-      if (!rowok || i > nx - 2) continue;
       const cpx<float> um = e == 0 ? left : y[e - 1];
       const cpx<float> up = e == E - 1 ? right : y[e + 1];
       const cpx<float> v = y[e] * H + T[o.xaP + k] * (y[e] - up) + T[o.xaM + k] * (y[e] - um) + sc[e];
-      DY[L.at(j - 1, i - 1)] = yrp * v;
-      if (i >= wn.x && i <= wn.y && j >= wn.z && j <= wn.w) Sn[(long long)j * nx + i] = mk(0.f, 0.f);
+      if (rowok && i <= nx - 2) {
+        dst[e] = yrp * v;
+        if (jwin && i >= wn.x && i <= wn.y) Sn[j * nx + i] = mk(0.f, 0.f);
+      }
     }
   }
 #undef FELIS_ADI_LOADX
   __syncthreads();
-  // ---- y-implicit half step: columns, in place ----
+  // ---- y-implicit half step down the columns, stored straight to U with
+  // the slice energy accumulated in FP64 ----
 #pragma unroll
   for (int lv = 0; lv < 4; ++lv) {
     Cf[lv] = Th[o.scan + 128 + lv * 16 + sub];
     Cb[lv] = Th[o.scan + 192 + lv * 16 + sub];
   }
+  double es = 0.0;
   for (int cb = 0; cb < nxi; cb += nw * 4) {
     const int col = cb + wv * 4 + grp;  // 0-based interior column
     const bool colok = col < nxi;
+    const int cc = colok ? col : nxi - 1;
+    // rows sub*E .. sub*E+E-1 of column cc are P apart in the layout
+    const cpx<float> *src = DY + L.at(sub * E, cc);
     cpx<float> y[E];
     cpx<float> B = mk(0.f, 0.f);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const int r = i0 + e - 1;  // 0-based interior row
-      const cpx<float> b = (colok && r < nyi) ? DY[L.at(r, col)] : mk(0.f, 0.f);
+      const int r = sub * E + e;
+      const cpx<float> b = r < nyi ? src[e * L.P] : mk(0.f, 0.f);
       B = cfma(T[o.yap + e * 16 + sub], B, b);
       y[e] = B;
     }
@@ -314,30 +326,26 @@ __global__ void __launch_bounds__(512)
       y[e] = B;
     }
     const cpx<float> uin = scan_bwd(B, Cb);
+    cpx<float> *out = Un + (1 + sub * E) * nx + cc + 1;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const int r = i0 + e - 1;
-      if (colok && r < nyi) DY[L.at(r, col)] = cfma(T[o.yGb + e * 16 + sub], uin, y[e]);
+      const int r = sub * E + e;
+      if (colok && r < nyi) {
+        const cpx<float> v = cfma(T[o.yGb + e * 16 + sub], uin, y[e]);
+        out[e * nx] = v;
+        es += (double)v.re * v.re + (double)v.im * v.im;
+      }
     }
-  }
-  __syncthreads();
-  // ---- coalesced store of the interior and the slice energy (FP64) ----
-  double es = 0.0;
-  for (int q = tid; q < nxi * nyi; q += blockDim.x) {
-    const int jr = q / nxi, ic = q - jr * nxi;
-    const cpx<float> v = DY[L.at(jr, ic)];
-    Un[(long long)(jr + 1) * nx + ic + 1] = v;
-    es += (double)v.re * v.re + (double)v.im * v.im;
   }
   // the boundary ring is untouched by the ADI but counts in the energy
   const int nring = 2 * nx + 2 * (ny - 2);
   for (int k = tid; k < nring; k += blockDim.x) {
-    long long idx;
+    int idx;
     if (k < nx) idx = k;
-    else if (k < 2 * nx) idx = (long long)(ny - 1) * nx + (k - nx);
+    else if (k < 2 * nx) idx = (ny - 1) * nx + (k - nx);
     else {
       int r = k - 2 * nx;
-      idx = (long long)(1 + (r >> 1)) * nx + ((r & 1) ? nx - 1 : 0);
+      idx = (1 + (r >> 1)) * nx + ((r & 1) ? nx - 1 : 0);
     }
     cpx<float> v = Un[idx];
     es += (double)v.re * v.re + (double)v.im * v.im;
