@@ -32,9 +32,11 @@ std::vector<std::string> read_lines(const char *path, bool *ok) {
   std::ifstream f(path);
   *ok = f.good();
   std::string line;
+  // like fgets, a line keeps its terminator unless a "//" comment cut it
   while (std::getline(f, line)) {
     size_t c = line.find("//");
     if (c != std::string::npos) line.resize(c);
+    else if (!f.eof()) line.push_back('\n');
     out.push_back(line);
   }
   return out;

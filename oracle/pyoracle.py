@@ -127,7 +127,7 @@ def lib() -> C.CDLL:
                                         C.c_double, _dp, _dp]
         L.orc_wake_function.restype = C.c_int
         L.orc_update_wake.argtypes = [C.POINTER(State), _dp, _dp, _dp, _dp]
-        L.orc_load_seed_3d.argtypes = [C.POINTER(Params), _dp, _dp, C.c_int] + [C.c_double] * 10
+        L.orc_load_seed_3d.argtypes = [C.POINTER(Params), _dp, _dp, C.c_int] + [C.c_double] * 9
         L.orc_run.argtypes = [C.POINTER(State), C.POINTER(Lattice), C.c_int, C.c_int, C.c_int, C.c_int,
                               _dp, _dp, _dp, _dp, _dp]
         L.orc_run.restype = C.c_int
