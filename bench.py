@@ -97,33 +97,9 @@ def run_cpu(args, g):
     d = g["desc"]
     threads = int(args.cpu_threads or min(16, os.cpu_count() or 1))
     po.set_threads(threads)
-    p = po.Params()
-    p.nx, p.ny, p.slice_n, p.n_harm = d.nx, d.ny, d.slice_n, d.n_harm
-    for h in range(d.n_harm):
-        p.harmony[h] = d.harmony[h]
-    maxh = d.harmony[d.n_harm - 1]
-    p.num_slice, p.n_species, p.nr, p.sc_l, p.sc_f = d.num_slice, d.n_species, d.nr, d.sc_l, d.sc_f
-    p.bn, p.b_orders, p.wake_on, p.dimension = d.bessel_n, maxh // 2 + 2, d.wake_on, 3
-    p.dx, p.dy, p.dz, p.min_x, p.min_y = d.dx, d.dy, d.dz, d.min_x, d.min_y
-    p.ks, p.lambda0, p.gamR, p.dr = d.ks, d.lambda0, d.gamR, d.dr
-    p.dBessel = maxh * 2.0 * np.pi / d.bessel_n
-    al, alP, be, beP = po.abc_setup(d.nx, d.ny, d.abc_n, d.abc_sig, d.dx, d.dy, d.dz, d.ks)
-    bes = po.bessel_table(d.bessel_n, maxh)
     seg_off, seg_w = g["deck"].beam_layout(d.n_species)
-    part = g["init"]["particles"]
-    S = d.slice_n + 2
-    st = po.OracleState(params=p, alpha=al, alphaP=alP, beta=be, betaP=beP, bessel=bes, seg_off=seg_off,
-                        seg_weight=seg_w, x=part["x"].copy(), y=part["y"].copy(), px=part["px"].copy(),
-                        py=part["py"].copy(), theta=part["theta"].copy(), gamma=part["gamma"].copy(),
-                        Ux=g["init"]["Ux"].copy(), Uy=g["init"]["Uy"].copy(),
-                        ScUx=np.zeros((d.n_harm, S, d.ny, d.nx), np.complex128),
-                        ScUy=np.zeros((d.n_harm, S, d.ny, d.nx), np.complex128),
-                        Ez=np.zeros((d.sc_l, S * d.sc_f * d.nr), np.complex128), wakeE=np.zeros(d.slice_n))
-    tab = g["tab"]
-    lat = [po.Lattice() for _ in range(2 * args.cpu_steps + 1)]
-    for k in range(2 * args.cpu_steps + 1):
-        for name, _ in po.Lattice._fields_:
-            setattr(lat[k], name, getattr(tab[k], name))
+    st = po.state_from_desc(d, seg_off, seg_w, g["init"]["particles"], g["init"]["Ux"], g["init"]["Uy"])
+    lat = po.lattice_table(g["tab"][:2 * args.cpu_steps + 1])
     t0 = time.perf_counter()
     rc, energy, _, _ = st.run(lat, 0, args.cpu_steps, d.mode == 2, g["deck"].wake_step, g["wakeF"])
     t1 = time.perf_counter()

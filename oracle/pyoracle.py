@@ -27,7 +27,7 @@ MAX_HARM = 8
 def build(force: bool = False) -> Path:
     """Compiles liboracle.so with make (gcc)."""
     if force or not LIB_PATH.exists():
-        subprocess.run(["make", "-s", "-C", str(HERE), "liboracle.so"], check=True)
+        subprocess.run(["make", "-s", "-B", "-C", str(HERE), "liboracle.so"], check=True)
     return LIB_PATH
 
 
@@ -300,3 +300,43 @@ def ref():
         R = C.CDLL(str(REF_PATH))
         _ref = R
     return _ref
+
+
+def state_from_desc(d, seg_off, seg_w, particles, Ux, Uy) -> "OracleState":
+    """An oracle state for a felis_desc-like object `d` (fields of
+    include/felis.h) with the given beam and field (copied)."""
+    p = Params()
+    p.nx, p.ny, p.slice_n, p.n_harm = d.nx, d.ny, d.slice_n, d.n_harm
+    for h in range(d.n_harm):
+        p.harmony[h] = d.harmony[h]
+    maxh = d.harmony[d.n_harm - 1]
+    p.num_slice, p.n_species, p.nr, p.sc_l, p.sc_f = d.num_slice, d.n_species, d.nr, d.sc_l, d.sc_f
+    p.bn, p.b_orders, p.wake_on, p.dimension = d.bessel_n, maxh // 2 + 2, d.wake_on, d.dimension
+    p.dx, p.dy, p.dz, p.min_x, p.min_y = d.dx, d.dy, d.dz, d.min_x, d.min_y
+    p.ks, p.lambda0, p.gamR, p.dr = d.ks, d.lambda0, d.gamR, d.dr
+    p.dBessel = maxh * 2.0 * np.pi / d.bessel_n
+    al, alP, be, beP = abc_setup(d.nx, d.ny, d.abc_n, d.abc_sig, d.dx, d.dy, d.dz, d.ks)
+    bes = bessel_table(d.bessel_n, maxh)
+    S = d.slice_n + 2
+    return OracleState(params=p, alpha=al, alphaP=alP, beta=be, betaP=beP, bessel=bes,
+                       seg_off=np.ascontiguousarray(seg_off, np.int64),
+                       seg_weight=np.ascontiguousarray(seg_w, np.float64),
+                       x=particles["x"].astype(np.float64), y=particles["y"].astype(np.float64),
+                       px=particles["px"].astype(np.float64), py=particles["py"].astype(np.float64),
+                       theta=particles["theta"].astype(np.float64), gamma=particles["gamma"].astype(np.float64),
+                       Ux=np.ascontiguousarray(Ux, np.complex128).copy(),
+                       Uy=np.ascontiguousarray(Uy, np.complex128).copy(),
+                       ScUx=np.zeros((d.n_harm, S, d.ny, d.nx), np.complex128),
+                       ScUy=np.zeros((d.n_harm, S, d.ny, d.nx), np.complex128),
+                       Ez=np.zeros((d.sc_l, S * d.sc_f * d.nr), np.complex128), wakeE=np.zeros(d.slice_n))
+
+
+def lattice_table(tab) -> list:
+    """Oracle lattices from any objects with the felis_lattice fields."""
+    out = []
+    for t in tab:
+        lat = Lattice()
+        for name, _ in Lattice._fields_:
+            setattr(lat, name, getattr(t, name))
+        out.append(lat)
+    return out
