@@ -61,6 +61,26 @@ def kernel_bytes(name: str, d, np_: int, mixed: bool) -> float:
     return float(table.get(name, 0.0))
 
 
+# kernel-timer name -> rocprofv3 kernel name, for the measured HBM traffic
+PROFILED = {("cfg2", "mixed"): {"adi_fused": "felis::k_adi_scan<8>", "particle_step": "felis::k_pstep<3>",
+                                "deposit": "felis::k_deposit_slot<3>"}}
+
+
+def measured_traffic(config: str, precision: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE
+    / WRITE_SIZE summary of this workload (scaled by a copy of known size), or
+    None when that workload was not profiled."""
+    names = PROFILED.get((config, precision), {})
+    path = ROOT / "profiles" / f"hbm_traffic_{config}_batched.md"
+    if kernel not in names or not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic" and parts[1] == names[kernel]:
+            return float(parts[2])
+    return None
+
+
 def run_gpu(args, precision=None, steps=None, warmup=None, keep_init=None):
     from felis_amd import _lib as fl
     deck = fl.Deck(ROOT / "felis_amd" / "decks" / f"{args.config}.inp")
@@ -192,7 +212,8 @@ def main():
                    "slices": d.slice_n, "particles": g["np"], "grid": f"{d.nx}x{d.ny}",
                    "harmonics": [d.harmony[i] for i in range(d.n_harm)]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None, "bytes_per_launch": bpl,
+                     "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": measured_traffic(args.config, args.precision, dom), "bytes_per_launch": bpl,
                      "avg_launch_us": avg_s * 1e6},
         "kernels_ms_per_step": {k: v[0] / K for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])},
     }
