@@ -29,7 +29,7 @@ EXPORTS = [
     "felis_run", "felis_find_parameter", "felis_deck_open", "felis_deck_close", "felis_deck_error",
     "felis_deck_desc", "felis_deck_steps", "felis_deck_lattice", "felis_deck_wake_function",
     "felis_deck_beam_layout", "felis_deck_load_beam", "felis_deck_load_seed", "felis_profile",
-    "felis_profile_read", "felis_device_info",
+    "felis_profile_read", "felis_device_info", "felis_set_beam_sigma", "felis_deck_beam_sigma",
 ]
 
 
@@ -116,6 +116,8 @@ def load() -> C.CDLL:
         L.felis_deck_desc.argtypes = [C.c_void_p, C.c_int, C.POINTER(Desc)]
         L.felis_deck_steps.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.felis_deck_lattice.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(Lattice)]
+        L.felis_set_beam_sigma.argtypes = [C.c_void_p, _dp]
+        L.felis_deck_beam_sigma.argtypes = [C.c_void_p, _dp]
         L.felis_deck_wake_function.argtypes = [C.c_void_p, _dp, _dp]
         L.felis_deck_beam_layout.argtypes = [C.c_void_p, _i64p, _dp]
         L.felis_deck_load_beam.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64]
@@ -203,6 +205,10 @@ class Context:
 
     def set_wake(self, wakeE):
         self._chk(self.L.felis_set_wake(self.h, _p(np.ascontiguousarray(wakeE, np.float64))))
+
+    def set_beam_sigma(self, sig):
+        """dimension = 1: rms beam sizes [n_species][2] of the source area."""
+        self._chk(self.L.felis_set_beam_sigma(self.h, _p(np.ascontiguousarray(sig, np.float64).ravel())))
 
     # -- operators ---------------------------------------------------------------
     def solve_field(self, lat: Lattice):
@@ -331,6 +337,11 @@ class Deck:
         if rc != OK:
             raise FelisError(rc, "wake function")
         return wf, imp
+
+    def beam_sigma(self, n_species: int):
+        sig = np.zeros(2 * n_species)
+        self.L.felis_deck_beam_sigma(self.h, _p(sig))
+        return sig.reshape(n_species, 2)
 
     def beam_layout(self, n_species: int):
         so = np.zeros(n_species * self.slice_n + 1, np.int64)

@@ -182,8 +182,14 @@ int felis_ctx::check_err() {
 
 // boundary.cpp:73-106 and the hoisted Thomas coefficients of
 // solveField.cpp:137-152 / 165-180 (field independent: H and the ABC only).
+static void build_bessel(felis_ctx *ctx);
+
 static void build_tables(felis_ctx *ctx) {
   const felis_desc &d = ctx->d;
+  if (d.dimension == 1) {  // one cell per slice: no transverse solver tables
+    build_bessel(ctx);
+    return;
+  }
   using C = std::complex<double>;
   const C I(0.0, 1.0);
   int nx = d.nx, ny = d.ny, L = d.abc_n;
@@ -243,7 +249,12 @@ static void build_tables(felis_ctx *ctx) {
     coef(nx, al, alP, (double)d.harmony[h], &ctx->h_adix[(size_t)h * 4 * nx]);
     coef(ny, be, beP, (double)d.harmony[h], &ctx->h_adiy[(size_t)h * 4 * ny]);
   }
-  // Bessel table, boundary.cpp:109-123 (J_n via the C++17 special functions)
+  build_bessel(ctx);
+}
+
+// Bessel table, boundary.cpp:109-123 (J_n via the C++17 special functions)
+static void build_bessel(felis_ctx *ctx) {
+  const felis_desc &d = ctx->d;
   int maxH = d.harmony[d.n_harm - 1];
   double bmax = maxH * 2.0 * M_PI;
   ctx->dB = bmax / (double)d.bessel_n;
@@ -261,11 +272,20 @@ extern "C" int felis_create(const felis_desc *desc, felis_ctx **out) {
   if (!desc || !out) return FELIS_ERR_ARG;
   *out = nullptr;
   const felis_desc &d = *desc;
-  if (d.dimension != 3) return FELIS_ERR_UNSUPPORTED;
-  if (d.nx < 6 || d.ny < 6 || d.slice_n < 1 || d.n_harm < 1 || d.n_harm > FELIS_MAX_HARM ||
-      d.num_slice < 1 || d.n_species < 1 || d.nr < 3 || d.nr > 256 || d.sc_l < 1 || d.sc_l > 4 || d.sc_f < 1 ||
-      d.bessel_n < 2 || d.abc_n < 1 || d.nx > 64 * 4 + 2 || d.ny > 64 * 4 + 2)
+  if (d.dimension == 1) {
+    // solveField.cpp:409-549 / particlePush.cpp:388-524: one cell per slice,
+    // FP64 only (parameterSetting.cpp:112, 196-200 fix nx = ny = nr = 1)
+    if (d.nx != 1 || d.ny != 1 || d.nr != 1 || d.sc_f != 1 || d.sc_l < 1 || d.sc_l > 16 || d.slice_n < 1 ||
+        d.n_harm < 1 || d.n_harm > FELIS_MAX_HARM || d.num_slice < 1 || d.n_species < 1 || d.bessel_n < 2)
+      return FELIS_ERR_ARG;
+    if (d.precision != FELIS_PLAIN_FP64) return FELIS_ERR_UNSUPPORTED;
+  } else if (d.dimension != 3) {
+    return FELIS_ERR_UNSUPPORTED;
+  } else if (d.nx < 6 || d.ny < 6 || d.slice_n < 1 || d.n_harm < 1 || d.n_harm > FELIS_MAX_HARM ||
+             d.num_slice < 1 || d.n_species < 1 || d.nr < 3 || d.nr > 256 || d.sc_l < 1 || d.sc_l > 4 ||
+             d.sc_f < 1 || d.bessel_n < 2 || d.abc_n < 1 || d.nx > 64 * 4 + 2 || d.ny > 64 * 4 + 2) {
     return FELIS_ERR_ARG;
+  }
   if (d.precision != FELIS_PLAIN_FP64 && d.precision != FELIS_MIXED_FP32) return FELIS_ERR_ARG;
   for (int h = 0; h < d.n_harm; ++h)
     if (d.harmony[h] < 1) return FELIS_ERR_ARG;
@@ -314,6 +334,7 @@ extern "C" int felis_create(const felis_desc *desc, felis_ctx **out) {
       (e = dalloc(ctx, &ctx->den, (size_t)d.slice_n * sizeof(double))) ||
       (e = dalloc(ctx, &ctx->slice_e, (size_t)nsys * sizeof(double))) ||
       (e = dalloc(ctx, &ctx->win, (size_t)ctx->S * sizeof(int4))) ||
+      (e = dalloc(ctx, &ctx->beam_sig, (size_t)2 * d.n_species * sizeof(double))) ||
       (e = dalloc(ctx, &ctx->red_out, (size_t)1024 * sizeof(double))) ||
       (e = dalloc(ctx, &ctx->errw, sizeof(unsigned)))) {
     felis_destroy(ctx);
@@ -667,6 +688,22 @@ extern "C" int felis_set_wake(felis_ctx *ctx, const double *wakeE) {
   return FELIS_OK;
 }
 
+// solveField.cpp:463-468: the rms beam sizes whose Gaussian area normalises
+// the 1D source (sigma = sqrt(emittance / gamma0 / gamma_twiss) per species).
+extern "C" int felis_set_beam_sigma(felis_ctx *ctx, const double *sig) {
+  if (!ctx || !sig) return FELIS_ERR_ARG;
+  for (int s = 0; s < 2 * ctx->d.n_species; ++s)
+    if (!(sig[s] > 0.0)) {
+      ctx->err = "beam sigma must be positive";
+      return FELIS_ERR_ARG;
+    }
+  HIPCHK(hipMemcpyAsync(ctx->beam_sig, sig, 2 * ctx->d.n_species * sizeof(double), hipMemcpyHostToDevice,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  ctx->sig_set = true;
+  return FELIS_OK;
+}
+
 // ---------------------------------------------------------------- operators
 
 template <typename R, typename T>
@@ -713,6 +750,17 @@ int felis_ctx::push_theta_gamma(const felis_lattice *lat) {
   LatArgs l;
   int e = lat_args(this, lat, &l);
   if (e) return e;
+  if (d.dimension == 1) {
+    if (!ntiles) return FELIS_OK;
+    GridArgs g = grid();
+    felis_ctx *ctx = this;
+    KTimer kt(ctx, FELIS_K_PUSH);
+    hipLaunchKernelGGL(k_push_tg_1d, dim3(ntiles), dim3(kTile), 0, stream, g, l, tiles(), theta, (double *)gamma,
+                       (const cpx<double> *)U, (const cpx<double> *)((char *)U + pol_bytes()),
+                       (const cpx<double> *)Ez, bessel, wakeE, errw);
+    LAUNCHCHK();
+    return FELIS_OK;
+  }
   return mixed() ? launch_push<float, float>(this, l) : launch_push<double, double>(this, l);
 }
 
@@ -868,6 +916,31 @@ int felis_ctx::adi_only(int current_flag) {
 
 int felis_ctx::solve_field(const felis_lattice *lat) {
   felis_ctx *ctx = this;
+  if (d.dimension == 1) {
+    // solveField.cpp:49-66: Sc cleared, the 1D source if current, U += Sc
+    int e = zero_async(ctx, Sc, (size_t)2 * d.n_harm * S * esize());
+    if (e) return e;
+    GridArgs g = grid();
+    if (lat->current_flag) {
+      if (!sig_set) {
+        err = "1D source needs the beam sizes (felis_set_beam_sigma)";
+        return FELIS_ERR_STATE;
+      }
+      LatArgs l;
+      if ((e = lat_args(this, lat, &l))) return e;
+      KTimer kt(ctx, FELIS_K_DEPOSIT);
+      hipLaunchKernelGGL(k_sc_1d, dim3(d.slice_n), dim3(256), 0, stream, g, l, d.n_species, seg_off, seg_w,
+                         beam_sig, theta, (const double *)gamma, bessel, d.lambda0 * d.num_slice,
+                         (cpx<double> *)Sc, (cpx<double> *)((char *)Sc + pol_bytes()), Ez, errw);
+      LAUNCHCHK();
+    }
+    KTimer kt(ctx, FELIS_K_ADI_X);
+    const int n = 2 * d.n_harm * d.slice_n;
+    hipLaunchKernelGGL(k_field_1d, dim3((n + 255) / 256), dim3(256), 0, stream, g, (cpx<double> *)U,
+                       (const cpx<double> *)Sc);
+    LAUNCHCHK();
+    return FELIS_OK;
+  }
   // solveField.cpp:54-58 zeroes only the first slice_n+2 entries of each Sc;
   // the deposit below clears everything when it runs.
   if (!lat->current_flag) {

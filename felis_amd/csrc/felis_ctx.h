@@ -41,6 +41,8 @@ struct felis_ctx {
   double ez_A = -1.0;  // A of the coefficients held in ez_b (negative: none)
   // per slot: interior cells [x0..x1] x [y0..y1] the last batched deposit wrote
   int4 *win = nullptr;
+  double *beam_sig = nullptr;  // [n_species][2] rms beam sizes (1D source area)
+  bool sig_set = false;
   felis::cpx<double> *abc = nullptr, *adix = nullptr, *adiy = nullptr;
   double *bessel = nullptr, *wakeE = nullptr, *wakeF = nullptr, *den = nullptr;
   double *slice_e = nullptr, *red_out = nullptr;

@@ -993,4 +993,162 @@ static __global__ void k_wake(int sn, double dZ, double coef, const double *__re
   wakeE[ii] = acc;
 }
 
+// ---------------------------------------------------------------- 1D (nx = ny = 1)
+
+// push_theta_gamma_1D (particlePush.cpp:388-524) for one particle per thread:
+// the slice's field and its L longitudinal space-charge modes drive an RK4 of
+// theta and gamma; even harmonics do not couple (494-497).
+static __global__ void __launch_bounds__(kTile)
+    k_push_tg_1d(GridArgs g, LatArgs l, Tiles tl, double *__restrict__ theta, double *__restrict__ gamma,
+                 const cpx<double> *__restrict__ Ux, const cpx<double> *__restrict__ Uy,
+                 const cpx<double> *__restrict__ Ez, const double *__restrict__ bes,
+                 const double *__restrict__ wakeE, unsigned *err) {
+  const int t = blockIdx.x, k = threadIdx.x;
+  if (k >= tl.cnt[t]) return;
+  const int seg = tl.seg[t], sl = seg % g.slice_n, slot = sl + 1;
+  const long long n = tl.start[t] + k;
+  const int phys = phys_slot(slot, g.ring_base, g.S);
+  const double ue = l.ue, ks = g.ks, ku = l.ku, K0 = l.K0, dz = g.dz, ka = l.K0_alpha;
+  const double dPhi = g.dPhi;
+  const double e_mc2 = kECharge / kEMass / kC / kC;
+  const double K2 = 1.0 + K0 * K0 * 0.5 * (1.0 + ue * ue);
+  const double xi = 0.25 * K0 * K0 * (1.0 - ue * ue) / (1.0 + (1.0 + ue * ue) * K0 * K0 * 0.5);
+  const double wk = g.wake_on ? wakeE[sl] / (kMc2 * 1.0e6) : 0.0;
+  const cpx<double> etaX = mk(l.etaXr, l.etaXi), etaY = mk(l.etaYr, l.etaYi);
+  double sumU2 = 0.0;
+  for (int h = 0; h < g.nh; ++h) {
+    const cpx<double> ux = Ux[(long long)h * g.S + phys], uy = Uy[(long long)h * g.S + phys];
+    const int H = g.harm[h];
+    sumU2 += (norm2(ux) + norm2(uy)) / (2.0 * H * H);
+  }
+  const double th0 = theta[n], gam0 = gamma[n];
+  const double cm[5] = {0, 0, 0.5, 0.5, 1};
+  double kth[5] = {0, 0, 0, 0, 0}, kg[5] = {0, 0, 0, 0, 0};
+  for (int m = 1; m < 5; ++m) {
+    const double th = th0 + dz * kth[m - 1] * cm[m];
+    const double gam = gam0 + dz * kg[m - 1] * cm[m];
+    double sumTh = 0.0, sumG = 0.0, sumEz = 0.0;
+    for (int q = 0; q < g.L; ++q) {
+      double sn, cs;
+      sincos((q + 1.0) * th, &sn, &cs);
+      const cpx<double> em = Ez[(long long)q * g.S + slot];
+      sumEz += 2.0 * (em.re * cs - em.im * sn);
+    }
+    for (int h = 0; h < g.nh; ++h) {
+      const int H = g.harm[h];
+      if (H % 2 == 0) continue;
+      const double sign = ((H - 1) / 2 % 2 == 0) ? 1.0 : -1.0;
+      const int idx = (int)((1.0 * H * xi) / g.dB);
+      const double w1 = (H * xi / g.dB) - idx, w0 = 1.0 - w1;
+      if (idx < 0 || idx + 1 >= g.bn) {
+        atomicOr(err, EB_BESSEL);
+        return;
+      }
+      const double J1 = bessel_interp(bes, g.bo, idx, (H - 1) / 2, w0, w1);
+      const double J2 = bessel_interp(bes, g.bo, idx, (H + 1) / 2, w0, w1);
+      const double fx = sign * (ka * J1 - J2), fy = sign * (ka * J1 + J2);
+      double sn, cs;
+      sincos((1.0 * H) * (th + l.Phi * 0.5), &sn, &cs);
+      const cpx<double> ux = Ux[(long long)h * g.S + phys], uy = Uy[(long long)h * g.S + phys];
+      const cpx<double> c = etaX * ux * fx - etaY * uy * fy;
+      const cpx<double> tt = mk(cs, sn) * c;
+      sumTh += -tt.im / (1.0 * H);
+      sumG += tt.re / (2.0 * gam);
+    }
+    kth[m] = ku - ks / (2.0 * gam * gam) * (K2 + sumU2 + K0 * sqrt(ue * ue + 1.0) * sumTh);
+    kg[m] = ks * K0 * sqrt(ue * ue + 1.0) * sumG + e_mc2 * sumEz;
+  }
+  const double dth = dz / 6.0 * (kth[1] + 2.0 * kth[2] + 2.0 * kth[3] + kth[4]);
+  if (dth > dPhi || dth <= -dPhi) {
+    atomicOr(err, EB_GUARD);
+    return;
+  }
+  theta[n] = th0 + dth;
+  gamma[n] = gam0 + dz / 6.0 * (kg[1] + 2 * kg[2] + 2 * kg[3] + kg[4]) - dz * wk;
+}
+
+// solve_Sc_1D (solveField.cpp:422-549) for one slot per workgroup: the
+// bunching source of every harmonic and the L longitudinal modes of Ez, each
+// a sum over the slot's particles of every species. sig[2s], sig[2s+1] are
+// the species' rms beam sizes that set the source area (463-468).
+static __global__ void __launch_bounds__(256)
+    k_sc_1d(GridArgs g, LatArgs l, int n_species, const long long *__restrict__ seg_off,
+            const double *__restrict__ seg_w, const double *__restrict__ sig, const double *__restrict__ theta,
+            const double *__restrict__ gamma, const double *__restrict__ bes, double bucket,
+            cpx<double> *__restrict__ Scx, cpx<double> *__restrict__ Scy, cpx<double> *__restrict__ Ez,
+            unsigned *err) {
+  __shared__ double sh[256];
+  const int sl = blockIdx.x, slot = sl + 1, tid = threadIdx.x;
+  const double ue = l.ue, K0 = l.K0, ka = l.K0_alpha;
+  const double xi = 0.25 * K0 * K0 * (1.0 - ue * ue) / (1.0 + (1.0 + ue * ue) * K0 * K0 * 0.5);
+  // per harmonic: sum of macro coef e^{-iH(theta+Phi/2)}/gamma; per mode: sum
+  // of macro coef2 e^{-i(l+1) theta}
+  for (int q = 0; q < g.nh + g.L; ++q) {
+    double re = 0.0, im = 0.0;
+    for (int s = 0; s < n_species; ++s) {
+      const double area = 2.0 * M_PI * sig[2 * s] * sig[2 * s + 1];
+      const int seg = s * g.slice_n + sl;
+      const double macro = seg_w[seg];
+      double c;
+      if (q < g.nh)
+        c = g.dz * kECharge * kECharge * kMu0 * K0 * sqrt(1.0 + ue * ue) / (2.0 * g.ks * kEMass * bucket * area);
+      else
+        c = kECharge * kC * kC * kMu0 / g.ks / area / bucket / (1.0 + (q - g.nh));
+      for (long long n = seg_off[seg] + tid; n < seg_off[seg + 1]; n += blockDim.x) {
+        double sn, cs;
+        if (q < g.nh) {
+          sincos(-(1.0 * g.harm[q]) * (theta[n] + l.Phi * 0.5), &sn, &cs);
+          const double a = macro * c / gamma[n];
+          re += cs * a;
+          im += sn * a;
+        } else {
+          sincos(-((q - g.nh) + 1.0) * theta[n], &sn, &cs);
+          re += cs * macro * c;
+          im += sn * macro * c;
+        }
+      }
+    }
+    re = block_sum<256>(re, sh);
+    im = block_sum<256>(im, sh);
+    if (tid == 0) {
+      if (q < g.nh) {
+        const int H = g.harm[q];
+        double fx = 0.0, fy = 0.0;
+        if (H % 2 == 1) {
+          const double sign = ((H - 1) / 2 % 2 == 0) ? 1.0 : -1.0;
+          const int idx = (int)(H * xi / g.dB);
+          const double w1 = (H * xi / g.dB) - idx, w0 = 1.0 - w1;
+          if (idx < 0 || idx + 1 >= g.bn) {
+            atomicOr(err, EB_BESSEL);
+          } else {
+            const double J1 = bessel_interp(bes, g.bo, idx, (H - 1) / 2, w0, w1);
+            const double J2 = bessel_interp(bes, g.bo, idx, (H + 1) / 2, w0, w1);
+            fx = sign * (J1 - ka * J2);
+            fy = sign * (J1 + ka * J2);
+          }
+        }
+        const cpx<double> m = mk(re, im);
+        Scx[(long long)q * g.S + slot] = (mk(l.etaXr, l.etaXi) * m) * (-fx);
+        Scy[(long long)q * g.S + slot] = (mk(l.etaYr, l.etaYi) * m) * (-fy);
+      } else {
+        // -i * sum
+        Ez[(long long)(q - g.nh) * g.S + slot] = mk(im, -re);
+      }
+    }
+  }
+  if (sl == 0 && tid < g.L) {
+    Ez[(long long)tid * g.S] = mk(0.0, 0.0);
+    Ez[(long long)tid * g.S + g.S - 1] = mk(0.0, 0.0);
+  }
+}
+
+// solve_Field_U_1D (solveField.cpp:409-420): U += Sc on slots 1..slice_n.
+static __global__ void k_field_1d(GridArgs g, cpx<double> *__restrict__ U, const cpx<double> *__restrict__ Sc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;  // over pol, h, slice
+  if (t >= 2 * g.nh * g.slice_n) return;
+  const int ph = t / g.slice_n, slot = t % g.slice_n + 1;
+  cpx<double> &u = U[(long long)ph * g.S + phys_slot(slot, g.ring_base, g.S)];
+  u = u + Sc[(long long)ph * g.S + slot];
+}
+
 }  // namespace felis

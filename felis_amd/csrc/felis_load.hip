@@ -233,6 +233,19 @@ extern "C" int felis_deck_beam_layout(const felis_deck *D, int64_t *seg_off, dou
   return FELIS_OK;
 }
 
+extern "C" int felis_deck_beam_sigma(const felis_deck *D, double *sig) {
+  if (!D || !sig) return FELIS_ERR_ARG;
+  size_t s = 0;
+  for (const BeamBlock &LL : D->beams) {
+    const double ex = LL.emit_x / D->gamma0, ey = LL.emit_y / D->gamma0;
+    const double gx = (1 + LL.alpha_x * LL.alpha_x) / LL.beta_x, gy = (1 + LL.alpha_y * LL.alpha_y) / LL.beta_y;
+    sig[2 * s] = std::sqrt(ex / gx);
+    sig[2 * s + 1] = std::sqrt(ey / gy);
+    ++s;
+  }
+  return FELIS_OK;
+}
+
 extern "C" int felis_deck_load_beam(const felis_deck *D, felis_ctx *ctx, uint64_t seed) {
   if (!D || !ctx) return FELIS_ERR_ARG;
   const felis_desc &d = ctx->d;

@@ -45,8 +45,9 @@ int enqueue_step(felis_ctx *ctx, const felis_lattice *lat, double *e_out, double
   if (t_out && (e = ctx->twiss(t_out))) return e;
   if (b_out && (e = ctx->bfactor(b_out))) return e;
   if ((e = ctx->solve_field(&lat[0]))) return e;
-  if ((e = ctx->transverse(&lat[1]))) return e;
-  if ((e = ctx->transverse(&lat[2]))) return e;
+  // main.cpp:133-140: no transverse motion in 1D
+  if (d.dimension == 3 && (e = ctx->transverse(&lat[1]))) return e;
+  if (d.dimension == 3 && (e = ctx->transverse(&lat[2]))) return e;
   if (!lat[2].drift_flag) e = ctx->push_theta_gamma(&lat[2]);
   else e = ctx->drift(&lat[2]);
   if (e) return e;

@@ -127,6 +127,9 @@ int felis_update_wake(felis_ctx *ctx, const double *wakeF, double *wakeE, double
                       double *total_cnt);
 /* Sets the per-slice wake field directly (restart path). */
 int felis_set_wake(felis_ctx *ctx, const double *wakeE);
+/* dimension = 1 only: sig[2s], sig[2s+1] = rms beam sizes of species s, the
+ * source area of solve_Sc_1D (solveField.cpp:463-468). */
+int felis_set_beam_sigma(felis_ctx *ctx, const double *sig);
 
 /* -------- per-step diagnostics (main.cpp:123-125) ----------------------- */
 /* updateTotalEnergy.cpp:25-36: raw sums of |Ux|^2, |Uy|^2 per harmonic. */
@@ -188,6 +191,9 @@ int felis_deck_beam_layout(const felis_deck *deck, int64_t *seg_off, double *seg
 int felis_deck_load_beam(const felis_deck *deck, felis_ctx *ctx, uint64_t seed);
 /* loadSeed.cpp:25-80 generated on the device. */
 int felis_deck_load_seed(const felis_deck *deck, felis_ctx *ctx);
+/* solveField.cpp:463-468: rms beam sizes sig[2s], sig[2s+1] of each species
+ * from the deck's emittance and Twiss parameters. */
+int felis_deck_beam_sigma(const felis_deck *deck, double *sig);
 
 #ifdef __cplusplus
 }

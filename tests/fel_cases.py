@@ -175,6 +175,8 @@ def gpu_context(case: Case, precision: int = 0):
     ctx.upload_field(fl.UY, s["Uy"])
     ctx.upload_field(fl.EZ, s["Ez"])
     ctx.set_wake(s["wakeE"])
+    if case.desc_kw["dimension"] == 1:
+        ctx.set_beam_sigma(s["beam_sig"])
     return ctx
 
 
@@ -184,3 +186,65 @@ def rel_err(a, b) -> float:
     if den == 0:
         return float(np.max(np.abs(a)))
     return float(np.max(np.abs(a - b)) / den)
+
+
+def make_case_1d(slice_n=6, harmonics=(1, 3), ppslice=64, sc_l=2, n_species=1, seed=5, K0=1.87,
+                 lambdaU=0.026, gamma0=8000.0, ue=0.0, K0_alpha=1, field_amp=2e-6, in_phases=4,
+                 sigma=(30e-6, 25e-6), wake=True, bn=2001) -> Case:
+    """A dimension = 1 state: one field cell per slot (parameterSetting.cpp:112,
+    196-200), beam sizes for the source area (solveField.cpp:463-468)."""
+    rng = np.random.default_rng(seed)
+    maxh = max(harmonics)
+    photon = resonant_photon_energy(gamma0, K0, lambdaU)
+    ks = photon * 2 * math.pi / (PLANK_H * C_LIGHT)
+    lambda0 = 2 * math.pi / ks
+    ku = 2 * math.pi / lambdaU
+    dz = lambdaU
+    gamR = math.sqrt(0.5 * ks / ku * (1 + 0.5 * (1 + ue * ue) * K0 * K0))
+    p = po.Params()
+    p.nx, p.ny, p.slice_n, p.n_harm = 1, 1, slice_n, len(harmonics)
+    for i, h in enumerate(harmonics):
+        p.harmony[i] = h
+    p.num_slice, p.n_species, p.nr, p.sc_l, p.sc_f = 1, n_species, 1, sc_l, 1
+    p.bn, p.b_orders, p.wake_on, p.dimension = bn, maxh // 2 + 2, int(wake), 1
+    p.dx, p.dy, p.dz, p.min_x, p.min_y = 0.0, 0.0, dz, -300e-6, -300e-6
+    p.ks, p.lambda0, p.gamR, p.dr = ks, lambda0, gamR, 0.0
+    p.dBessel = maxh * 2.0 * math.pi / bn
+    bes = po.bessel_table(bn, maxh)
+    counts = [ppslice] * (n_species * slice_n)
+    seg_off = np.zeros(len(counts) + 1, np.int64)
+    seg_off[1:] = np.cumsum(counts)
+    npart = int(seg_off[-1])
+    nb = npart // in_phases
+    rep = lambda a: np.repeat(a, in_phases)  # noqa: E731
+    gamma = rep(gamma0 * (1 + 1e-4 * rng.normal(size=nb)))
+    theta = rep(rng.uniform(0, 2 * math.pi, nb)) + np.tile(np.arange(in_phases) * 2 * math.pi / in_phases, nb)
+    theta = np.mod(theta + 1e-3 * rng.normal(size=npart), 2 * math.pi)
+    x = rep(rng.normal(0, 30e-6, nb)); y = rep(rng.normal(0, 30e-6, nb))
+    px = rep(rng.normal(0, 0.5, nb)); py = rep(rng.normal(0, 0.5, nb))
+    seg_weight = 1e3 * (1 + 0.1 * rng.random(len(counts)))
+    S = slice_n + 2
+    nh = len(harmonics)
+    Ux = np.zeros((nh, S, 1, 1), np.complex128)
+    Uy = np.zeros_like(Ux)
+    for h in range(nh):
+        for slot in range(1, slice_n + 1):
+            Ux[h, slot] = field_amp * np.exp(1j * rng.uniform(0, 2 * math.pi))
+            Uy[h, slot] = 0.3 * field_amp * np.exp(1j * rng.uniform(0, 2 * math.pi))
+    Ez = np.zeros((sc_l, S), np.complex128)
+    Ez[:, 1:slice_n + 1] = 1e3 * (rng.normal(size=(sc_l, slice_n)) + 1j * rng.normal(size=(sc_l, slice_n)))
+    wakeE = 1e4 * rng.normal(size=slice_n)
+    lat = po.Lattice()
+    lat.K0, lat.ue, lat.K0_alpha, lat.ku, lat.g, lat.lambdaU = K0, ue, float(K0_alpha), ku, 0.0, lambdaU
+    lat.und_type, lat.current_flag, lat.drift_flag = 1, 1, 0
+    state = dict(seg_off=seg_off, seg_weight=seg_weight, x=x, y=y, px=px, py=py, theta=theta, gamma=gamma,
+                 Ux=Ux, Uy=Uy, ScUx=np.zeros_like(Ux), ScUy=np.zeros_like(Ux), Ez=Ez, wakeE=wakeE,
+                 beam_sig=np.tile(np.array(sigma, np.float64), n_species))
+    tables = dict(alpha=np.zeros(1, np.complex128), alphaP=np.zeros(1, np.complex128),
+                  beta=np.zeros(1, np.complex128), betaP=np.zeros(1, np.complex128), bessel=bes)
+    desc_kw = dict(dimension=1, mode=2, nx=1, ny=1, slice_n=slice_n, n_harm=nh, harmony=list(harmonics),
+                   num_slice=1, n_species=n_species, nr=1, sc_l=sc_l, sc_f=1, bessel_n=bn, abc_n=1,
+                   wake_on=int(wake), abc_sig=0.0, dx=0.0, dy=0.0, dz=dz, min_x=-300e-6, min_y=-300e-6,
+                   min_z=0.0, max_z=slice_n * lambda0, ks=ks, lambda0=lambda0, gamR=gamR, dr=0.0,
+                   gamma_ref=gamma0)
+    return Case(1, 1, slice_n, tuple(harmonics), n_species, p, lat, state, tables, desc_kw)

@@ -197,3 +197,57 @@ def test_run_plain_matches_oracle_loop():
     bg = np.abs(b_g[:, 0:2 * nh:2] + 1j * b_g[:, 1:2 * nh:2]) / b_g[:, 2 * nh:2 * nh + 1]
     assert fc.rel_err(bg, b_o[:, :nh]) < 1e-8
     assert fc.rel_err(t_g, t_o) < 1e-9
+
+
+# ---------------------------------------------------------------- 1D path
+
+@pytest.mark.parametrize("harm,L", [((1,), 1), ((1, 3), 2), ((1, 2, 3), 3)])
+def test_1d_push(harm, L):
+    """push_theta_gamma_1D (particlePush.cpp:388-524) in FP64: increments of
+    theta and gamma to 1e-12 of their size."""
+    c = fc.make_case_1d(harmonics=harm, sc_l=L)
+    o = c.oracle_state()
+    ctx = fc.gpu_context(c)
+    lat = c.lattice()
+    assert o.push_theta_gamma(lat) == 0
+    ctx.push_theta_gamma(fc.to_lib_lattice(lat))
+    p = ctx.download_particles()
+    th0, g0 = c.state["theta"], c.state["gamma"]
+    assert fc.rel_err(p["theta"] - th0, o.theta - th0) < 1e-11
+    assert fc.rel_err(p["gamma"] - g0, o.gamma - g0) < 1e-11
+    ctx.close()
+
+
+@pytest.mark.parametrize("harm,L,ns", [((1,), 1, 1), ((1, 3), 2, 2), ((1, 2, 3), 3, 1)])
+def test_1d_solve_field(harm, L, ns):
+    """solve_Sc_1D + solve_Field_U_1D (solveField.cpp:409-549): sources,
+    fields and the space-charge modes."""
+    c = fc.make_case_1d(harmonics=harm, sc_l=L, n_species=ns)
+    o = c.oracle_state()
+    ctx = fc.gpu_context(c)
+    lat = c.lattice()
+    assert o.solve_field(lat) == 0
+    ctx.solve_field(fc.to_lib_lattice(lat))
+    dU = o.Ux - c.state["Ux"]
+    assert fc.rel_err(ctx.download_field(fl.UX) - c.state["Ux"], dU) < 1e-12
+    assert fc.rel_err(ctx.download_field(fl.UY) - c.state["Uy"], o.Uy - c.state["Uy"]) < 1e-12
+    assert fc.rel_err(ctx.download_field(fl.EZ), o.Ez) < 1e-12
+    ctx.close()
+
+
+def test_1d_run_matches_oracle_loop():
+    """The 1D loop (main.cpp:92-156 with no transverse push) for 12 steps with
+    slippage, the wake loss and the space-charge modes."""
+    c = fc.make_case_1d(harmonics=(1, 3), sc_l=2, n_species=2)
+    o = c.oracle_state()
+    ctx = fc.gpu_context(c)
+    nsteps = 12
+    tab = [c.lattice(current_flag=0)] + [c.lattice() for _ in range(2 * nsteps)]
+    rc, e_o, b_o, _ = o.run(tab, 0, nsteps, True)
+    assert rc == 0
+    e_g, b_g, _ = ctx.run([fc.to_lib_lattice(l) for l in tab], 0, nsteps)
+    assert fc.rel_err(e_g, e_o) < 1e-10
+    nh = 2
+    bg = np.abs(b_g[:, 0:2 * nh:2] + 1j * b_g[:, 1:2 * nh:2]) / b_g[:, 2 * nh:2 * nh + 1]
+    assert np.max(np.abs(bg - b_o[:, :nh])) < 1e-10
+    ctx.close()
