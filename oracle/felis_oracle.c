@@ -24,8 +24,9 @@
  *       quiet-start cancellation, energy balance with the push; Ez radial
  *       solve against a dense solve.
  *   orc_push_theta_gamma_1d, orc_solve_sc_1d, orc_solve_field_u_1d
- *                                    -> parity unpinned (no reference build,
- *       only the zero-field closed form checks them).
+ *                                    -> zero-field slippage, quiet-start
+ *       cancellation and U += Sc closed forms; their coupling and source
+ *       coefficients are parity unpinned (no reference build covers them).
  */
 #include "felis_oracle.h"
 
