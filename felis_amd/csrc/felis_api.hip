@@ -305,6 +305,9 @@ extern "C" int felis_create(const felis_desc *desc, felis_ctx **out) {
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
       ctx->lds_max = lds - 1024;  // room for the kernel's static LDS
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+      ctx->n_cu = ncu;
     if (ctx->lds_max > 65536 - 1024) {
       // opt in to more than 64 KiB of dynamic LDS where the device allows it
       if (hipFuncSetAttribute((const void *)k_deposit_win<double, double>,

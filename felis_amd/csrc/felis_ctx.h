@@ -63,6 +63,7 @@ struct felis_ctx {
   int *chunk_seg = nullptr, *chunk_cnt = nullptr;
   long long *chunk_start = nullptr;
   int lds_max = 65536;  // dynamic LDS bytes a workgroup may use
+  int n_cu = 256;       // compute units of the device
   felis::Tiles chunks() const { return felis::Tiles{chunk_seg, chunk_start, chunk_cnt}; }
 
   // per-kernel timing (felis_profile)

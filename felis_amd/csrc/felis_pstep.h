@@ -394,7 +394,17 @@ __global__ void __launch_bounds__(1024)
       }
     }
   }
-  atomicMin(&bb[0], mi); atomicMax(&bb[1], Mi); atomicMin(&bb[2], mj); atomicMax(&bb[3], Mj);
+  // wave-wide min/max first: one LDS atomic per wave instead of per thread
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    mi = min(mi, __shfl_xor(mi, o));
+    Mi = max(Mi, __shfl_xor(Mi, o));
+    mj = min(mj, __shfl_xor(mj, o));
+    Mj = max(Mj, __shfl_xor(Mj, o));
+  }
+  if ((tid & 63) == 0) {
+    atomicMin(&bb[0], mi); atomicMax(&bb[1], Mi); atomicMin(&bb[2], mj); atomicMax(&bb[3], Mj);
+  }
   __syncthreads();
   const int wi0 = bb[0], wj0 = bb[2];
   const bool empty = bb[1] < 0;
@@ -575,43 +585,92 @@ __global__ void __launch_bounds__(1024)
       __syncthreads();
     }
   }
-  // radial source of the last species, stored whole for this slot
+  // radial source of the last species, stored whole for this slot. Each wave
+  // accumulates into its own copy of the row (the beam core fills a handful
+  // of radial bins, which one shared copy would serialise); consecutive
+  // particles at one radius (a beamlet) are summed in registers first. The
+  // copies are added in a fixed order.
   const int F = g.F, nr = g.nr, row = F * nr + 1;
+  const int nwv = nt >> 6, wid = tid >> 6;
   cpx<double> *hist = reinterpret_cast<cpx<double> *>(lds_d);
-  for (int q = tid; q < row; q += nt) hist[q] = mk(0.0, 0.0);
+  const bool priv = (long long)nwv * row * (long long)sizeof(cpx<double>) <= lds_bytes;
+  cpx<double> *mine = priv ? hist + (long long)wid * row : hist;
+  for (int q = tid; q < (priv ? nwv : 1) * row; q += nt) hist[q] = mk(0.0, 0.0);
   __syncthreads();
   {
     const int seg = (n_species - 1) * g.slice_n + sl;
     const long long b = seg_off[seg], e = seg_off[seg + 1];
     const double macro = seg_w[seg];
-    for (long long n = b + tid; n < e; n += nt) {
-      const double xx = x[n], yy = y[n], th = theta[n];
-      const double phi = (xx == 0.0) ? 0.0 : atan2(yy, xx);
-      const double r = sqrt(xx * xx + yy * yy);
-      const int idxR = (int)(r / g.dr + 0.5);
-      if (idxR > 0 && idxR < nr) {
-        const double wr1 = (r * r / (g.dr * g.dr) - idxR * idxR) / (2.0 * idxR), wr0 = 1.0 - wr1;
-        for (int f = 0; f < F; ++f) {
-          double s, c;
-          sincos(-(th + f * phi), &s, &c);
+    const double inv_dr = 1.0 / g.dr;
+    for (long long r0 = b + (long long)tid * kRun; r0 < e; r0 += (long long)nt * kRun) {
+      const long long r1 = min(r0 + (long long)kRun, e);
+      int key = -1;
+      double wr0 = 0.0, wr1 = 0.0;
+      cpx<double> sum[2] = {mk(0.0, 0.0), mk(0.0, 0.0)};  // F = 1 and F > 1 use f = 0 here
+#define FELIS_RADIAL_FLUSH()                                                                 \
+  if (key > 0) {                                                                             \
+    cpx<double> *hp2 = mine + key;                                                           \
+    atomicAdd(&hp2[0].re, sum[0].re * wr0);                                                  \
+    atomicAdd(&hp2[0].im, sum[0].im * wr0);                                                  \
+    atomicAdd(&hp2[1].re, sum[0].re * wr1);                                                  \
+    atomicAdd(&hp2[1].im, sum[0].im * wr1);                                                  \
+  } else if (key == 0) {                                                                     \
+    atomicAdd(&mine[0].re, sum[1].re);                                                       \
+    atomicAdd(&mine[0].im, sum[1].im);                                                       \
+  }
+      for (long long n = r0; n < r1; ++n) {
+        const double xx = x[n], yy = y[n];
+        const double rr = sqrt(xx * xx + yy * yy);
+        const int idxR = (int)(rr * inv_dr + 0.5);
+        const cpx<float> em = conjg(expi_f32(theta[n]));  // e^{-i theta}
+        if (idxR > 0 && idxR < nr) {
           const double amp = d.coef_Ez * macro / (1.0 * idxR);
-          cpx<double> *hp2 = hist + f * nr + idxR;
-          atomicAdd(&hp2[0].re, -s * amp * wr0);
-          atomicAdd(&hp2[0].im, c * amp * wr0);
-          atomicAdd(&hp2[1].re, -s * amp * wr1);
-          atomicAdd(&hp2[1].im, c * amp * wr1);
+          if (F > 1) {
+            // azimuthal modes need the angle of every particle: no combining
+            FELIS_RADIAL_FLUSH();
+            key = -1;
+            const double ph = (xx == 0.0) ? 0.0 : atan2(yy, xx);
+            const double w1 = (rr * rr * inv_dr * inv_dr - idxR * idxR) / (2.0 * idxR), w0 = 1.0 - w1;
+            for (int f = 0; f < F; ++f) {
+              double sn, cs;
+              sincos(-(theta[n] + f * ph), &sn, &cs);
+              cpx<double> *hp2 = mine + f * nr + idxR;
+              atomicAdd(&hp2[0].re, -sn * amp * w0);
+              atomicAdd(&hp2[0].im, cs * amp * w0);
+              atomicAdd(&hp2[1].re, -sn * amp * w1);
+              atomicAdd(&hp2[1].im, cs * amp * w1);
+            }
+            continue;
+          }
+          if (idxR != key) {
+            FELIS_RADIAL_FLUSH();
+            key = idxR;
+            sum[0] = mk(0.0, 0.0);
+            wr1 = (rr * rr * inv_dr * inv_dr - idxR * idxR) / (2.0 * idxR);
+            wr0 = 1.0 - wr1;
+          }
+          // i e^{-i theta} = (sin theta, cos theta) = (-em.im, em.re)
+          sum[0] = sum[0] + mk(-(double)em.im, (double)em.re) * amp;
+        } else if (idxR == 0) {
+          if (key != 0) {
+            FELIS_RADIAL_FLUSH();
+            key = 0;
+            sum[1] = mk(0.0, 0.0);
+          }
+          sum[1] = sum[1] + mk(-(double)em.im, (double)em.re) * (d.coef_Ez * macro);
         }
-      } else if (idxR == 0) {
-        double s, c;
-        sincos(-th, &s, &c);
-        const double amp = d.coef_Ez * macro;
-        atomicAdd(&hist[0].re, -s * amp);
-        atomicAdd(&hist[0].im, c * amp);
       }
+      FELIS_RADIAL_FLUSH();
+#undef FELIS_RADIAL_FLUSH
     }
   }
   __syncthreads();
-  for (int q = tid; q < row; q += nt) ScR[(long long)slot * row + q] = hist[q];
+  for (int q = tid; q < row; q += nt) {
+    cpx<double> v = hist[q];
+    if (priv)
+      for (int w = 1; w < nwv; ++w) v = v + hist[(long long)w * row + q];
+    ScR[(long long)slot * row + q] = v;
+  }
 }
 
 // out[o] for o < 2*nh: energy of pol o/nh, harmonic o%nh over the slices that

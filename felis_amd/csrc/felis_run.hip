@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -190,7 +191,10 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
   // LDS-resident scan ADI when the system fits, else the two-pass ADI with a
   // full clear of Sc after each use
   const bool fused = adi_scan_fits(ctx);
-  const int dep_lds = ctx->lds_max;
+  // the batched deposit adapts to any LDS budget (harmonics in chunks, else
+  // global atomics); FELIS_DEPOSIT_LDS overrides the default for tuning
+  int dep_lds = ctx->lds_max;
+  if (const char *v = std::getenv("FELIS_DEPOSIT_LDS")) dep_lds = std::max(4096, std::min(ctx->lds_max, std::atoi(v)));
   if (deposit_lds_attr(d.n_harm, dep_lds) != hipSuccess) {
     ctx->err = "hipFuncSetAttribute(k_deposit_slot)";
     return FELIS_ERR_HIP;
