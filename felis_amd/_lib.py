@@ -30,6 +30,8 @@ EXPORTS = [
     "felis_deck_desc", "felis_deck_steps", "felis_deck_lattice", "felis_deck_wake_function",
     "felis_deck_beam_layout", "felis_deck_load_beam", "felis_deck_load_seed", "felis_profile",
     "felis_profile_read", "felis_device_info", "felis_set_beam_sigma", "felis_deck_beam_sigma",
+    "felis_mlp_create", "felis_mlp_destroy", "felis_mlp_last_error", "felis_mlp_get_layer", "felis_mlp_set_layer",
+    "felis_mlp_forward", "felis_mlp_train_step", "felis_mlp_get_grad", "felis_gemm_bf16",
 ]
 
 
@@ -117,6 +119,17 @@ def load() -> C.CDLL:
         L.felis_deck_steps.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.felis_deck_lattice.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(Lattice)]
         L.felis_set_beam_sigma.argtypes = [C.c_void_p, _dp]
+        _fp = C.POINTER(C.c_float)
+        L.felis_mlp_create.argtypes = [C.c_int, C.POINTER(C.c_int), C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]
+        L.felis_mlp_destroy.argtypes = [C.c_void_p]
+        L.felis_mlp_last_error.argtypes = [C.c_void_p]
+        L.felis_mlp_last_error.restype = C.c_char_p
+        L.felis_mlp_get_layer.argtypes = [C.c_void_p, C.c_int, _fp, _fp]
+        L.felis_mlp_set_layer.argtypes = [C.c_void_p, C.c_int, _fp, _fp]
+        L.felis_mlp_forward.argtypes = [C.c_void_p, C.c_int, _fp, _fp]
+        L.felis_mlp_train_step.argtypes = [C.c_void_p, C.c_int, _fp, _fp, _fp, C.c_float, _dp]
+        L.felis_mlp_get_grad.argtypes = [C.c_void_p, C.c_int, _fp, _fp]
+        L.felis_gemm_bf16.argtypes = [C.c_int, C.c_int, C.c_int, _fp, _fp, _fp]
         L.felis_deck_beam_sigma.argtypes = [C.c_void_p, _dp]
         L.felis_deck_wake_function.argtypes = [C.c_void_p, _dp, _dp]
         L.felis_deck_beam_layout.argtypes = [C.c_void_p, _i64p, _dp]
@@ -358,3 +371,85 @@ class Deck:
         rc = self.L.felis_deck_load_seed(self.h, ctx.h)
         if rc != OK:
             raise FelisError(rc, self.L.felis_last_error(ctx.h).decode())
+
+
+def _f32(a):
+    a = np.ascontiguousarray(a, np.float32)
+    return a, a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def gemm_bf16(A, B):
+    """C = A B on the matrix cores with BF16-rounded operands (FP32 result)."""
+    A, pa = _f32(A)
+    B, pb = _f32(B)
+    M, K = A.shape
+    K2, N = B.shape
+    assert K == K2
+    Cm = np.zeros((M, N), np.float32)
+    rc = load().felis_gemm_bf16(M, N, K, pa, pb, Cm.ctypes.data_as(C.POINTER(C.c_float)))
+    if rc != OK:
+        raise FelisError(rc, "felis_gemm_bf16")
+    return Cm
+
+
+class MLP:
+    """The matrix-core perceptron of felis_mlp.hip (BF16 operands, FP32
+    accumulation, GELU hidden layers, linear output, Adam)."""
+
+    def __init__(self, dims, capacity: int, seed: int = 1):
+        self.L = load()
+        self.dims = [int(d) for d in dims]
+        arr = (C.c_int * len(self.dims))(*self.dims)
+        h = C.c_void_p()
+        rc = self.L.felis_mlp_create(len(self.dims) - 1, arr, int(capacity), int(seed), C.byref(h))
+        if rc != OK:
+            raise FelisError(rc, "felis_mlp_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.L.felis_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != OK:
+            raise FelisError(rc, self.L.felis_mlp_last_error(self.h).decode())
+
+    def get_layer(self, l):
+        W = np.zeros((self.dims[l], self.dims[l + 1]), np.float32)
+        b = np.zeros(self.dims[l + 1], np.float32)
+        self._chk(self.L.felis_mlp_get_layer(self.h, l, _f32(W)[1], _f32(b)[1]))
+        return W, b
+
+    def set_layer(self, l, W, b):
+        W, pw = _f32(W)
+        b, pb = _f32(b)
+        self._chk(self.L.felis_mlp_set_layer(self.h, l, pw, pb))
+
+    def forward(self, X):
+        X, px = _f32(X)
+        Y = np.zeros((X.shape[0], self.dims[-1]), np.float32)
+        self._chk(self.L.felis_mlp_forward(self.h, X.shape[0], px, _f32(Y)[1]))
+        return Y
+
+    def train_step(self, X, T, lr: float, wrow=None) -> float:
+        X, px = _f32(X)
+        T, pt = _f32(T)
+        pw = None
+        if wrow is not None:
+            wrow, pw = _f32(wrow)
+        loss = C.c_double()
+        self._chk(self.L.felis_mlp_train_step(self.h, X.shape[0], px, pt, pw, float(lr), C.byref(loss)))
+        return float(loss.value)
+
+    def get_grad(self, l):
+        gW = np.zeros((self.dims[l], self.dims[l + 1]), np.float32)
+        gb = np.zeros(self.dims[l + 1], np.float32)
+        self._chk(self.L.felis_mlp_get_grad(self.h, l, _f32(gW)[1], _f32(gb)[1]))
+        return gW, gb

@@ -1,0 +1,9 @@
+// Device-side entry of the matrix-core perceptron (felis_mlp.hip) for the
+// batched step's surrogate (felis_sur.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "felis.h"
+
+extern "C" int felis_mlp_forward_dev(felis_mlp *n, int rows, const float *X, float *Y, hipStream_t after);

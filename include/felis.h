@@ -195,6 +195,25 @@ int felis_deck_load_seed(const felis_deck *deck, felis_ctx *ctx);
  * from the deck's emittance and Twiss parameters. */
 int felis_deck_beam_sigma(const felis_deck *deck, double *sig);
 
+/* -------- learned surrogate: a matrix-core perceptron (no counterpart in
+ * the reference; BF16 operands, FP32 accumulation and FP32 master weights).
+ * dims[0..layers] are the widths; hidden layers use GELU, the output is
+ * linear. Host arrays, row-major, rows <= capacity. */
+typedef struct felis_mlp felis_mlp;
+int felis_mlp_create(int layers, const int *dims, int capacity, uint64_t seed, felis_mlp **out);
+int felis_mlp_destroy(felis_mlp *net);
+const char *felis_mlp_last_error(const felis_mlp *net);
+int felis_mlp_get_layer(felis_mlp *net, int layer, float *W /* [in][out] */, float *b /* [out] */);
+int felis_mlp_set_layer(felis_mlp *net, int layer, const float *W, const float *b);
+int felis_mlp_forward(felis_mlp *net, int rows, const float *X, float *Y);
+/* one Adam step on the (row-weighted, wrow may be NULL) mean squared error;
+ * lr = 0 only computes the gradients (felis_mlp_get_grad) */
+int felis_mlp_train_step(felis_mlp *net, int rows, const float *X, const float *T, const float *wrow, float lr,
+                         double *loss);
+int felis_mlp_get_grad(felis_mlp *net, int layer, float *gW, float *gb);
+/* C[M][N] = A[M][K] B[K][N] with BF16-rounded operands on the matrix cores */
+int felis_gemm_bf16(int M, int N, int K, const float *A, const float *B, float *C);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,79 @@
+"""The matrix-core perceptron of the surrogate against its CPU reference
+(oracle/mlp_ref.py): the BF16 product's lane layout, forward outputs and
+gradients, and that Adam training reduces the loss."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+fl = pytest.importorskip("felis_amd._lib")
+from oracle import mlp_ref  # noqa: E402
+
+
+@pytest.mark.parametrize("M,N,K", [(16, 16, 16), (37, 29, 53), (64, 48, 200)])
+def test_gemm_bf16_layout(M, N, K):
+    """C = A B on v_mfma_f32_16x16x16_bf16 tiles, ragged edges included;
+    operands BF16-rounded, so only the FP32 summation order differs."""
+    rng = np.random.default_rng(M * 1000 + N)
+    A = rng.normal(size=(M, K)).astype(np.float32)
+    B = rng.normal(size=(K, N)).astype(np.float32)
+    got = fl.gemm_bf16(A, B)
+    want = mlp_ref.bf16(A).astype(np.float64) @ mlp_ref.bf16(B).astype(np.float64)
+    assert np.max(np.abs(got - want)) < 1e-4 * np.max(np.abs(want))
+
+
+def _net(dims, rows, seed=3):
+    net = fl.MLP(dims, capacity=rows, seed=seed)
+    rng = np.random.default_rng(seed)
+    Ws, bs = [], []
+    for l in range(len(dims) - 1):
+        W = (rng.normal(size=(dims[l], dims[l + 1])) / np.sqrt(dims[l])).astype(np.float32)
+        b = (0.1 * rng.normal(size=dims[l + 1])).astype(np.float32)
+        net.set_layer(l, W, b)
+        Ws.append(W)
+        bs.append(b)
+    return net, Ws, bs
+
+
+def test_mlp_forward_matches_reference():
+    dims = [12, 48, 32, 6]
+    net, Ws, bs = _net(dims, 70)
+    X = np.random.default_rng(7).normal(size=(70, 12)).astype(np.float32)
+    Y = net.forward(X)
+    Yr, _ = mlp_ref.forward(Ws, bs, X)
+    assert np.max(np.abs(Y - Yr)) < 2e-3 * np.max(np.abs(Yr))
+    net.close()
+
+
+def test_mlp_gradients_match_reference():
+    """dW and db of every layer (lr = 0 leaves the weights unchanged)."""
+    dims = [12, 48, 32, 6]
+    net, Ws, bs = _net(dims, 90)
+    rng = np.random.default_rng(8)
+    X = rng.normal(size=(90, 12)).astype(np.float32)
+    T = rng.normal(size=(90, 6)).astype(np.float32)
+    wrow = rng.uniform(0.5, 2.0, 90).astype(np.float32)
+    loss = net.train_step(X, T, 0.0, wrow)
+    lr_, gW, gb = mlp_ref.gradients(Ws, bs, X, T, wrow)
+    assert loss == pytest.approx(lr_, rel=2e-3)
+    for l in range(3):
+        g, gbb = net.get_grad(l)
+        scale = np.max(np.abs(gW[l]))
+        assert np.max(np.abs(g - gW[l])) < 1e-2 * scale, l
+        assert np.max(np.abs(gbb - gb[l])) < 1e-2 * np.max(np.abs(gb[l])), l
+    W0, _ = net.get_layer(0)
+    assert np.array_equal(W0, Ws[0])
+    net.close()
+
+
+def test_mlp_training_reduces_loss():
+    """Adam on the GPU fits a smooth map of 4 inputs to 2 outputs."""
+    rng = np.random.default_rng(11)
+    X = rng.uniform(-1, 1, size=(512, 4)).astype(np.float32)
+    T = np.stack([np.sin(2 * X[:, 0]) * X[:, 1], X[:, 2] ** 2 - 0.5 * X[:, 3]], 1).astype(np.float32)
+    net = fl.MLP([4, 64, 64, 2], capacity=512, seed=5)
+    first = net.train_step(X, T, 3e-3)
+    for _ in range(400):
+        last = net.train_step(X, T, 3e-3)
+    assert last < 0.05 * first
+    net.close()
