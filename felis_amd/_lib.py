@@ -32,6 +32,9 @@ EXPORTS = [
     "felis_profile_read", "felis_device_info", "felis_set_beam_sigma", "felis_deck_beam_sigma",
     "felis_mlp_create", "felis_mlp_destroy", "felis_mlp_last_error", "felis_mlp_get_layer", "felis_mlp_set_layer",
     "felis_mlp_forward", "felis_mlp_train_step", "felis_mlp_get_grad", "felis_gemm_bf16",
+    "felis_run_surrogate", "felis_state_save", "felis_state_restore", "felis_slice_moments",
+    "felis_jump_residual", "felis_slice_features", "felis_mlp_shape", "felis_mlp_set_input_affine",
+    "felis_mlp_get_input_affine",
 ]
 
 
@@ -45,7 +48,7 @@ def device_info() -> dict:
 
 # felis_kernel_id order
 KERNELS = ["transverse", "push", "drift", "periodic", "deposit", "ez_deposit", "ez_solve", "adi_x", "adi_y",
-           "shift", "energy", "bfactor", "twiss", "wake", "zero", "particle_step", "adi_fused"]
+           "shift", "energy", "bfactor", "twiss", "wake", "zero", "particle_step", "adi_fused", "jump_window"]
 
 
 class FelisError(RuntimeError):
@@ -74,6 +77,11 @@ class Lattice(C.Structure):
         ("g", C.c_double), ("lambdaU", C.c_double),
         ("und_type", C.c_int), ("current_flag", C.c_int), ("drift_flag", C.c_int), ("pad", C.c_int),
     ]
+
+
+class JumpOpts(C.Structure):
+    _fields_ = [("periods", C.c_int), ("max_jumps", C.c_int), ("bunching_max", C.c_double),
+                ("spread_max", C.c_double), ("net", C.c_void_p)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -110,6 +118,16 @@ def load() -> C.CDLL:
         L.felis_update_bfactor.argtypes = [C.c_void_p, _dp, _dp]
         L.felis_calculate_twiss.argtypes = [C.c_void_p, _dp]
         L.felis_run.argtypes = [C.c_void_p, C.POINTER(Lattice), C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+        L.felis_state_save.argtypes = [C.c_void_p, C.c_int]
+        L.felis_state_restore.argtypes = [C.c_void_p, C.c_int]
+        L.felis_slice_moments.argtypes = [C.c_void_p, _dp]
+        L.felis_jump_residual.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.felis_slice_features.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
+        L.felis_mlp_shape.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.felis_mlp_set_input_affine.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.felis_mlp_get_input_affine.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.felis_run_surrogate.argtypes = [C.c_void_p, C.POINTER(Lattice), C.c_int, C.c_int, C.c_int, _dp,
+                                          C.POINTER(JumpOpts), _dp, _dp, _dp, C.POINTER(C.c_int)]
         L.felis_find_parameter.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]
         L.felis_deck_open.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
         L.felis_deck_close.argtypes = [C.c_void_p]
@@ -283,6 +301,53 @@ class Context:
         self._chk(self.L.felis_run(self.h, arr, it0, n_steps, wake_step, _p(wf), _p(e), _p(b), _p(t)))
         return e, b, t
 
+    def run_surrogate(self, lat_tab, it0: int, n_steps: int, periods: int, bunching_max: float = 0.05,
+                      max_jumps: int = -1, wake_step: int = 0, wakeF=None, net=None, spread_max: float = 0.0):
+        """felis_run with multi-period jumps while |b1| < bunching_max (and
+        the rms relative energy spread < spread_max when that is > 0); net is
+        the learned field correction (an MLP) or None.
+        Returns energy, bfac, twiss (NaN rows inside jumps) and the step kinds."""
+        d = self.desc
+        arr = (Lattice * len(lat_tab))(*lat_tab)
+        e = np.zeros((n_steps, 2 * d.n_harm))
+        b = np.zeros((n_steps, 2 * d.n_harm + 1))
+        t = np.zeros((n_steps, 7 * d.n_species))
+        kind = np.zeros(max(1, n_steps), np.int32)
+        o = JumpOpts(int(periods), int(max_jumps), float(bunching_max), float(spread_max),
+                     net.h if net is not None else None)
+        wf = None if wakeF is None else np.ascontiguousarray(wakeF, np.float64)
+        self._chk(self.L.felis_run_surrogate(self.h, arr, it0, n_steps, wake_step, _p(wf), C.byref(o), _p(e), _p(b),
+                                             _p(t), kind.ctypes.data_as(C.POINTER(C.c_int))))
+        return e, b, t, kind[:n_steps]
+
+    def save_state(self, slot: int = 0):
+        self._chk(self.L.felis_state_save(self.h, slot))
+
+    def restore_state(self, slot: int = 0):
+        self._chk(self.L.felis_state_restore(self.h, slot))
+
+    def slice_moments(self):
+        d = self.desc
+        out = np.zeros((d.slice_n, 5 + 8 * d.n_harm))
+        self._chk(self.L.felis_slice_moments(self.h, _p(out)))
+        return out
+
+    def slice_features(self):
+        """The correction network's inputs of every slice (float32)."""
+        d = self.desc
+        out = np.zeros((d.slice_n, 5 + 5 * d.n_harm), np.float32)
+        self._chk(self.L.felis_slice_features(self.h, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def jump_residual(self, slot: int):
+        """Current state (a) against snapshot `slot` (b): field sums
+        [2*n_harm, slice_n, 4] and particle least-squares sums [slice_n, 12]."""
+        d = self.desc
+        f = np.zeros((2 * d.n_harm, d.slice_n, 4))
+        p = np.zeros((d.slice_n, 12))
+        self._chk(self.L.felis_jump_residual(self.h, slot, _p(f), _p(p)))
+        return f, p
+
     def sync(self):
         self._chk(self.L.felis_sync(self.h))
 
@@ -431,6 +496,17 @@ class MLP:
         W, pw = _f32(W)
         b, pb = _f32(b)
         self._chk(self.L.felis_mlp_set_layer(self.h, l, pw, pb))
+
+    def set_input_affine(self, mu, inv_sd):
+        mu, pm = _f32(mu)
+        inv_sd, ps = _f32(inv_sd)
+        self._chk(self.L.felis_mlp_set_input_affine(self.h, pm, ps))
+
+    def get_input_affine(self):
+        mu = np.zeros(self.dims[0], np.float32)
+        isd = np.zeros(self.dims[0], np.float32)
+        self._chk(self.L.felis_mlp_get_input_affine(self.h, _f32(mu)[1], _f32(isd)[1]))
+        return mu, isd
 
     def forward(self, X):
         X, px = _f32(X)

@@ -195,7 +195,8 @@ __device__ const cpx<float> kZeroCell = {0.f, 0.f};
 template <int E>
 __global__ void __launch_bounds__(512)
     k_adi_scan(GridArgs g, const cpx<float> *__restrict__ tab, int cf, cpx<float> *__restrict__ U,
-               cpx<float> *__restrict__ Sc, const int4 *__restrict__ win, double *__restrict__ slice_e) {
+               cpx<float> *__restrict__ Sc, const int4 *__restrict__ win, double *__restrict__ slice_e,
+               const cpx<float> *__restrict__ gain) {
   extern __shared__ double lds_d[];
   __shared__ double red[8];
   const int nx = g.nx, ny = g.ny, nxi = nx - 2, nyi = ny - 2;
@@ -206,6 +207,10 @@ __global__ void __launch_bounds__(512)
   const int sys = blockIdx.x;
   const int sl = sys % g.slice_n, ph = sys / g.slice_n, h = ph % g.nh, slot = sl + 1;
   const float H = (float)g.harm[h];
+  // optional complex gain on the stored field (the learned correction of a
+  // multi-period jump); null on every plain step
+  const bool has_gain = gain != nullptr;
+  const cpx<float> gs = has_gain ? gain[sys] : mk(1.f, 0.f);
   cpx<float> *Un = U + ((long long)ph * g.S + phys_slot(slot, g.ring_base, g.S)) * g.N;
   cpx<float> *Sn = Sc + ((long long)ph * g.S + slot) * g.N;
   const cpx<float> *Th = tab + (long long)h * o.block;
@@ -331,7 +336,8 @@ __global__ void __launch_bounds__(512)
     for (int e = 0; e < E; ++e) {
       const int r = sub * E + e;
       if (colok && r < nyi) {
-        const cpx<float> v = cfma(T[o.yGb + e * 16 + sub], uin, y[e]);
+        cpx<float> v = cfma(T[o.yGb + e * 16 + sub], uin, y[e]);
+        if (has_gain) v = v * gs;
         out[e * nx] = v;
         es += (double)v.re * v.re + (double)v.im * v.im;
       }

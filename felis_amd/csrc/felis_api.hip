@@ -82,7 +82,7 @@ GridArgs felis_ctx::grid() const {
   g.dr = d.dr;
   g.minX = d.min_x;
   g.minY = d.min_y;
-  g.dz = d.dz;
+  g.dz = d.dz * dz_mul;
   g.ks = d.ks;
   g.dB = dB;
   g.dPhi = 2 * M_PI * d.num_slice;
@@ -184,12 +184,10 @@ int felis_ctx::check_err() {
 // solveField.cpp:137-152 / 165-180 (field independent: H and the ABC only).
 static void build_bessel(felis_ctx *ctx);
 
-static void build_tables(felis_ctx *ctx) {
-  const felis_desc &d = ctx->d;
-  if (d.dimension == 1) {  // one cell per slice: no transverse solver tables
-    build_bessel(ctx);
-    return;
-  }
+// The ABC coefficients alpha, alphaP, beta, betaP (abc) and the hoisted
+// Thomas factors per harmonic (adix, adiy) for a field step of length dz.
+void abc_tables(const felis_desc &d, double dz, std::vector<cpx<double>> &abc, std::vector<cpx<double>> &adix,
+                std::vector<cpx<double>> &adiy) {
   using C = std::complex<double>;
   const C I(0.0, 1.0);
   int nx = d.nx, ny = d.ny, L = d.abc_n;
@@ -203,21 +201,21 @@ static void build_tables(felis_ctx *ctx) {
   for (int j = 0; j < ny; ++j) cy[j] = 1.0 + I * sy[j];
   std::vector<C> al(nx, 0.0), alP(nx, 0.0), be(ny, 0.0), beP(ny, 0.0);
   for (int i = 1; i < nx - 1; ++i) {
-    al[i] = -I * d.dz / (2.0 * d.ks * d.dx * d.dx * cx[i] * (cx[i] + cx[i - 1]));
-    alP[i] = -I * d.dz / (2.0 * d.ks * d.dx * d.dx * cx[i] * (cx[i] + cx[i + 1]));
+    al[i] = -I * dz / (2.0 * d.ks * d.dx * d.dx * cx[i] * (cx[i] + cx[i - 1]));
+    alP[i] = -I * dz / (2.0 * d.ks * d.dx * d.dx * cx[i] * (cx[i] + cx[i + 1]));
   }
   for (int j = 1; j < ny - 1; ++j) {
-    be[j] = -I * d.dz / (2.0 * d.ks * d.dy * d.dy * cy[j] * (cy[j] + cy[j - 1]));
-    beP[j] = -I * d.dz / (2.0 * d.ks * d.dy * d.dy * cy[j] * (cy[j] + cy[j + 1]));
+    be[j] = -I * dz / (2.0 * d.ks * d.dy * d.dy * cy[j] * (cy[j] + cy[j - 1]));
+    beP[j] = -I * dz / (2.0 * d.ks * d.dy * d.dy * cy[j] * (cy[j] + cy[j + 1]));
   }
-  ctx->h_abc.assign(2 * nx + 2 * ny, mk(0.0, 0.0));
+  abc.assign(2 * nx + 2 * ny, mk(0.0, 0.0));
   for (int i = 0; i < nx; ++i) {
-    ctx->h_abc[i] = mk(al[i].real(), al[i].imag());
-    ctx->h_abc[nx + i] = mk(alP[i].real(), alP[i].imag());
+    abc[i] = mk(al[i].real(), al[i].imag());
+    abc[nx + i] = mk(alP[i].real(), alP[i].imag());
   }
   for (int j = 0; j < ny; ++j) {
-    ctx->h_abc[2 * nx + j] = mk(be[j].real(), be[j].imag());
-    ctx->h_abc[2 * nx + ny + j] = mk(beP[j].real(), beP[j].imag());
+    abc[2 * nx + j] = mk(be[j].real(), be[j].imag());
+    abc[2 * nx + ny + j] = mk(beP[j].real(), beP[j].imag());
   }
   auto coef = [&](int n, const std::vector<C> &a, const std::vector<C> &aP, double H,
                    cpx<double> *out) {
@@ -243,12 +241,16 @@ static void build_tables(felis_ctx *ctx) {
       out[3 * n + i] = mk(p.real(), p.imag());
     }
   };
-  ctx->h_adix.assign((size_t)d.n_harm * 4 * nx, mk(0.0, 0.0));
-  ctx->h_adiy.assign((size_t)d.n_harm * 4 * ny, mk(0.0, 0.0));
+  adix.assign((size_t)d.n_harm * 4 * nx, mk(0.0, 0.0));
+  adiy.assign((size_t)d.n_harm * 4 * ny, mk(0.0, 0.0));
   for (int h = 0; h < d.n_harm; ++h) {
-    coef(nx, al, alP, (double)d.harmony[h], &ctx->h_adix[(size_t)h * 4 * nx]);
-    coef(ny, be, beP, (double)d.harmony[h], &ctx->h_adiy[(size_t)h * 4 * ny]);
+    coef(nx, al, alP, (double)d.harmony[h], &adix[(size_t)h * 4 * nx]);
+    coef(ny, be, beP, (double)d.harmony[h], &adiy[(size_t)h * 4 * ny]);
   }
+}
+
+static void build_tables(felis_ctx *ctx) {
+  if (ctx->d.dimension == 3) abc_tables(ctx->d, ctx->d.dz, ctx->h_abc, ctx->h_adix, ctx->h_adiy);
   build_bessel(ctx);
 }
 

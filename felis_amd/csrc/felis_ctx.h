@@ -76,6 +76,23 @@ struct felis_ctx {
   hipEvent_t ev_get();
   void ev_flush();
 
+  // multi-period jumps (felis_run_surrogate): the field step is dz * dz_mul
+  // while a jump is enqueued; the window passes' buffers are allocated on
+  // the first jump and kept (Sw: windowed source, Ub: windowed push field,
+  // both shaped like Sc and U; win2: windows of Sw; box: beam box)
+  double dz_mul = 1.0;
+  void *Sw = nullptr, *Ub = nullptr;
+  int4 *win2 = nullptr, *box = nullptr;
+  // device-side copies of the evolving state (felis_state_save/restore):
+  // field with its ring position, particles, per-slice wake
+  struct Snap {
+    void *U = nullptr, *part = nullptr;
+    double *wake = nullptr;
+    int ring_base = 0;
+    long long np = -1;
+    bool valid = false;
+  } snap[2];
+
   // batched step (felis_run) resources
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -136,4 +153,13 @@ struct KTimer {
 };
 
 int lat_args(felis_ctx *ctx, const felis_lattice *lat, felis::LatArgs *out);
+void abc_tables(const felis_desc &d, double dz, std::vector<felis::cpx<double>> &abc,
+                std::vector<felis::cpx<double>> &adix, std::vector<felis::cpx<double>> &adiy);
 int dep_args(felis_ctx *ctx, const felis_lattice *lat, felis::DepArgs *out);
+// learned jump correction (felis_sur.hip), enqueued on ctx->stream:
+// moments M and network inputs X of every slice; network outputs Y into the
+// per-system field gain and the per-slice particle correction; sum w,
+// sum w dg, sum w dg^2 over the slices
+int felis_slice_features_dev(felis_ctx *ctx, double *M, float *X);
+int felis_decode_corr_dev(felis_ctx *ctx, const float *Y, const double *M, void *gain, float *corr);
+int felis_moment_totals_dev(felis_ctx *ctx, const double *M, double *out3);

@@ -1,0 +1,188 @@
+// Kernels of the multi-period jump of the batched step (felis_run_surrogate).
+//
+// A jump advances every slice J undulator periods with one field solve and
+// one particle push of step J*dz, and slips the field J slots. One coarse
+// step of the reference's equations would let each field slot take J times
+// the source of a single slice, and push each slice in a single field slot:
+// the shot-noise part of the bunching then adds coherently over the J
+// periods, which overstates the spontaneous power J-fold. The two window
+// passes here keep the slippage of the J fine steps instead:
+//
+//   source   Sw[s] = sum_{i<J} Sc[s+i]     the slices field slot s passes
+//                                           while it slips to slot s+J
+//   field    Ub[s] = 1/J sum_{i<J} U[s-i]  the field slots that pass slice s
+//                                           (logical slots < 1 hold no field)
+//
+// Both run over slots in chunks with a halo, one thread per (cell, pol x
+// harmonic, chunk), with the running sum in FP64; only the bounding box of
+// the beam's deposit windows is visited (Sc is zero outside the windows,
+// and the push reads the field only around its particles).
+#pragma once
+
+#include "felis_common.h"
+
+namespace felis {
+
+constexpr int kJumpChunk = 64;  // slots per thread of the window passes
+constexpr int kJumpMargin = 2;  // cells the push-field box extends the beam box by
+
+// bbox = union of win[1..slice_n] (non-empty windows), widened by margin and
+// clamped to the grid; an empty union gives an empty box (x0 > x1).
+static __global__ void __launch_bounds__(256) k_win_bbox(const int4 *__restrict__ win, int slice_n, int nx, int ny,
+                                                  int margin, int4 *__restrict__ bbox) {
+  __shared__ int sh[4][256];
+  int x0 = 1 << 30, x1 = -1, y0 = 1 << 30, y1 = -1;
+  for (int s = 1 + threadIdx.x; s <= slice_n; s += blockDim.x) {
+    const int4 w = win[s];
+    if (w.x <= w.y && w.z <= w.w) {
+      x0 = min(x0, w.x); x1 = max(x1, w.y); y0 = min(y0, w.z); y1 = max(y1, w.w);
+    }
+  }
+  const int t = threadIdx.x;
+  sh[0][t] = x0; sh[1][t] = x1; sh[2][t] = y0; sh[3][t] = y1;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) {
+      sh[0][t] = min(sh[0][t], sh[0][t + o]);
+      sh[1][t] = max(sh[1][t], sh[1][t + o]);
+      sh[2][t] = min(sh[2][t], sh[2][t + o]);
+      sh[3][t] = max(sh[3][t], sh[3][t + o]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    if (sh[1][0] < 0) {
+      *bbox = make_int4(1, 0, 1, 0);
+    } else {
+      *bbox = make_int4(max(0, sh[0][0] - margin), min(nx - 1, sh[1][0] + margin), max(0, sh[2][0] - margin),
+                        min(ny - 1, sh[3][0] + margin));
+    }
+  }
+}
+
+// win2[s] = union of win[s..min(s+J-1, slice_n)], the cells Sw[s] can be
+// nonzero in; slots 0 and slice_n+1 get an empty window.
+static __global__ void k_win_union(const int4 *__restrict__ win, int slice_n, int J, int4 *__restrict__ win2) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > slice_n + 1) return;
+  int4 u = make_int4(1 << 30, -1, 1 << 30, -1);
+  if (s >= 1 && s <= slice_n) {
+    for (int i = 0; i < J && s + i <= slice_n; ++i) {
+      const int4 w = win[s + i];
+      if (w.x <= w.y && w.z <= w.w) {
+        u.x = min(u.x, w.x); u.y = max(u.y, w.y); u.z = min(u.z, w.z); u.w = max(u.w, w.w);
+      }
+    }
+  }
+  win2[s] = u.y < 0 ? make_int4(1, 0, 1, 0) : u;
+}
+
+__device__ inline bool in_box(int4 b, int i, int j) { return i >= b.x && i <= b.y && j >= b.z && j <= b.w; }
+
+// Sw[ph][s] = sum_{i<J, s+i<=slice_n} Sc[ph][s+i] for the chunk's slots, at
+// one cell of the beam box. Sc and Sw are indexed by logical slot.
+// grid: x = cells / 256, y = pol x harmonic, z = slot chunks.
+static __global__ void __launch_bounds__(256) k_src_window(int nx, int ny, int S, int slice_n, int J,
+                                                    const int4 *__restrict__ bbox,
+                                                    const cpx<float> *__restrict__ Sc,
+                                                    cpx<float> *__restrict__ Sw) {
+  const long long N = (long long)nx * ny;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const int i = (int)(c % nx), j = (int)(c / nx);
+  if (!in_box(*bbox, i, j)) return;
+  const int s0 = 1 + blockIdx.z * kJumpChunk;
+  if (s0 > slice_n) return;
+  const int s1 = min(slice_n, s0 + kJumpChunk - 1);
+  const cpx<float> *src = Sc + (long long)blockIdx.y * S * N + c;
+  cpx<float> *dst = Sw + (long long)blockIdx.y * S * N + c;
+  double sr = 0.0, si = 0.0;
+  for (int s = s0; s < s0 + J - 1 && s <= slice_n; ++s) {
+    const cpx<float> v = src[(long long)s * N];
+    sr += v.re;
+    si += v.im;
+  }
+  for (int s = s0; s <= s1; ++s) {
+    const int sa = s + J - 1;
+    if (sa <= slice_n) {
+      const cpx<float> v = src[(long long)sa * N];
+      sr += v.re;
+      si += v.im;
+    }
+    dst[(long long)s * N] = mk((float)sr, (float)si);
+    const cpx<float> v = src[(long long)s * N];
+    sr -= v.re;
+    si -= v.im;
+  }
+}
+
+// Sc[ph][s] = 0 on the beam box for s = 1..slice_n (after the source window
+// has read it), so that Sc is zero outside the next deposit's windows.
+static __global__ void __launch_bounds__(256) k_clear_box(int nx, int ny, int S, int slice_n, const int4 *__restrict__ bbox,
+                                                   cpx<float> *__restrict__ Sc) {
+  const long long N = (long long)nx * ny;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const int i = (int)(c % nx), j = (int)(c / nx);
+  if (!in_box(*bbox, i, j)) return;
+  const int s0 = 1 + blockIdx.z * kJumpChunk;
+  const int s1 = min(slice_n, s0 + kJumpChunk - 1);
+  cpx<float> *p = Sc + (long long)blockIdx.y * S * N + c;
+  for (int s = s0; s <= s1; ++s) p[(long long)s * N] = mk(0.f, 0.f);
+}
+
+// Ub[ph][phys(s)] = 1/J sum_{i<J, s-i>=1} U[ph][phys(s-i)] / g[ph][s-i] for
+// the chunk's slots, at one cell of the widened beam box; U and Ub share the
+// ring. g is the learned gain the field solve applied to slot s-i (null: 1):
+// the push sees the jump's own field, with which the particles follow the
+// fine steps (the gain corrects only the stored field).
+static __global__ void __launch_bounds__(256) k_field_window(int nx, int ny, int S, int slice_n, int ring_base, int J,
+                                                      const int4 *__restrict__ bbox,
+                                                      const cpx<float> *__restrict__ U,
+                                                      const cpx<float> *__restrict__ gain,
+                                                      cpx<float> *__restrict__ Ub) {
+  const long long N = (long long)nx * ny;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const int i = (int)(c % nx), j = (int)(c / nx);
+  if (!in_box(*bbox, i, j)) return;
+  const int s0 = 1 + blockIdx.z * kJumpChunk;
+  if (s0 > slice_n) return;
+  const int s1 = min(slice_n, s0 + kJumpChunk - 1);
+  const cpx<float> *src = U + (long long)blockIdx.y * S * N + c;
+  const cpx<float> *gp = gain ? gain + (long long)blockIdx.y * slice_n - 1 : nullptr;  // gp[s]: slot s
+  cpx<float> *dst = Ub + (long long)blockIdx.y * S * N + c;
+  auto at = [&](int s) {
+    const cpx<float> v = src[(long long)phys_slot(s, ring_base, S) * N];
+    return gp ? cdiv(v, gp[s]) : v;
+  };
+  const double inv = 1.0 / J;
+  double sr = 0.0, si = 0.0;
+  for (int s = max(1, s0 - J + 1); s < s0; ++s) {
+    const cpx<float> v = at(s);
+    sr += v.re;
+    si += v.im;
+  }
+  for (int s = s0; s <= s1; ++s) {
+    const cpx<float> v = at(s);
+    sr += v.re;
+    si += v.im;
+    dst[(long long)phys_slot(s, ring_base, S) * N] = mk((float)(sr * inv), (float)(si * inv));
+    const int so = s - J + 1;
+    if (so >= 1) {
+      const cpx<float> w = at(so);
+      sr -= w.re;
+      si -= w.im;
+    }
+  }
+}
+
+// zero the physical slots of logical slots 0..J-1 after a J-slot rotation
+// (slippage of J slices: the reference's slots 1..J take its empty slot 0)
+static __global__ void k_zero_slots(cpx<float> *U, int ring_base, int S, long long N, int J) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int ph = blockIdx.y, l = blockIdx.z;
+  if (k < N && l < J) U[((long long)ph * S + phys_slot(l, ring_base, S)) * N + k] = mk(0.f, 0.f);
+}
+
+}  // namespace felis

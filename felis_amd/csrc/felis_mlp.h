@@ -6,4 +6,6 @@
 
 #include "felis.h"
 
-extern "C" int felis_mlp_forward_dev(felis_mlp *n, int rows, const float *X, float *Y, hipStream_t after);
+// Y[rows][out] = net(X[rows][in]) for device arrays, enqueued on stream s
+// (the network's own stream when s is null)
+extern "C" int felis_mlp_forward_dev(felis_mlp *n, int rows, const float *X, float *Y, hipStream_t s);

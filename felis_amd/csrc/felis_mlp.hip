@@ -148,9 +148,14 @@ __global__ void k_adam(long long n, float *__restrict__ w, const float *__restri
   if (wb) wb[i] = f2bf(wi);
 }
 
-__global__ void k_to_bf16(long long n, const float *__restrict__ x, uint16_t *__restrict__ y) {
+// network input: (x - mu) * isd per column in FP32, then BF16
+__global__ void k_input_bf16(long long n, int cols, const float *__restrict__ x, const float *__restrict__ mu,
+                             const float *__restrict__ isd, uint16_t *__restrict__ y) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = f2bf(x[i]);
+  if (i < n) {
+    const int c = (int)(i % cols);
+    y[i] = f2bf((x[i] - mu[c]) * isd[c]);
+  }
 }
 
 inline unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
@@ -168,6 +173,7 @@ struct felis_mlp {
   std::vector<float *> Z, Hf;                         // per layer pre-/post-activation FP32
   std::vector<uint16_t *> Hb;                         // per layer activation BF16 (index 0 = input)
   float *dHf = nullptr, *dZf = nullptr, *Tf = nullptr, *wrow = nullptr;
+  float *in_mu = nullptr, *in_isd = nullptr;  // input affine, applied in FP32 before BF16
   uint16_t *dZb = nullptr;
   double *loss = nullptr;
   long long step = 0;
@@ -246,13 +252,19 @@ int felis_mlp_create(int layers, const int *dims, int capacity, uint64_t seed, f
              (e = mlp_alloc(n, &n->dZf, (size_t)capacity * maxw * 4)) ||
              (e = mlp_alloc(n, &n->dZb, (size_t)capacity * maxw * 2)) ||
              (e = mlp_alloc(n, &n->Tf, (size_t)capacity * dims[layers] * 4)) ||
-             (e = mlp_alloc(n, &n->wrow, (size_t)capacity * 4)) || (e = mlp_alloc(n, &n->loss, 8)))) {
+             (e = mlp_alloc(n, &n->wrow, (size_t)capacity * 4)) || (e = mlp_alloc(n, &n->loss, 8)) ||
+             (e = mlp_alloc(n, &n->in_mu, (size_t)dims[0] * 4)) || (e = mlp_alloc(n, &n->in_isd, (size_t)dims[0] * 4)))) {
   }
   if (e) {
     for (void *p : n->allocs) (void)hipFree(p);
     (void)hipStreamDestroy(n->stream);
     delete n;
     return e;
+  }
+  {
+    std::vector<float> z(dims[0], 0.f), o(dims[0], 1.f);
+    (void)hipMemcpy(n->in_mu, z.data(), dims[0] * 4, hipMemcpyHostToDevice);
+    (void)hipMemcpy(n->in_isd, o.data(), dims[0] * 4, hipMemcpyHostToDevice);
   }
   // Glorot-uniform weights from a 64-bit LCG, zero biases and moments
   uint64_t s = seed * 6364136223846793005ull + 1442695040888963407ull;
@@ -289,6 +301,14 @@ int felis_mlp_destroy(felis_mlp *n) {
 
 const char *felis_mlp_last_error(const felis_mlp *n) { return n ? n->err.c_str() : "null network"; }
 
+int felis_mlp_shape(const felis_mlp *n, int *in, int *out, int *capacity) {
+  if (!n) return FELIS_ERR_ARG;
+  if (in) *in = n->dims[0];
+  if (out) *out = n->dims[n->L];
+  if (capacity) *capacity = n->cap;
+  return FELIS_OK;
+}
+
 // Weights and biases of layer l as FP32 (host arrays, W row-major in x out).
 int felis_mlp_get_layer(felis_mlp *n, int l, float *W, float *b) {
   if (!n || l < 0 || l >= n->L) return FELIS_ERR_ARG;
@@ -320,7 +340,8 @@ int felis_mlp_forward_dev(felis_mlp *n, int rows, const float *X, float *Y, hipS
   hipStream_t keep = n->stream;
   n->stream = s;
   const long long ni = (long long)rows * n->dims[0];
-  hipLaunchKernelGGL(k_to_bf16, dim3(nblk(ni)), dim3(256), 0, s, ni, X, n->Hb[0]);
+  hipLaunchKernelGGL(k_input_bf16, dim3(nblk(ni)), dim3(256), 0, s, ni, n->dims[0], X, n->in_mu, n->in_isd,
+                     n->Hb[0]);
   forward(n, rows);
   if (Y)
     (void)hipMemcpyAsync(Y, n->Hf[n->L], (size_t)rows * n->dims[n->L] * 4, hipMemcpyDeviceToDevice, s);
@@ -334,7 +355,8 @@ int felis_mlp_forward(felis_mlp *n, int rows, const float *X, float *Y) {
   if (hipMemcpyAsync(n->Hf[0], X, (size_t)rows * n->dims[0] * 4, hipMemcpyHostToDevice, n->stream) != hipSuccess)
     return FELIS_ERR_HIP;
   const long long ni = (long long)rows * n->dims[0];
-  hipLaunchKernelGGL(k_to_bf16, dim3(nblk(ni)), dim3(256), 0, n->stream, ni, n->Hf[0], n->Hb[0]);
+  hipLaunchKernelGGL(k_input_bf16, dim3(nblk(ni)), dim3(256), 0, n->stream, ni, n->dims[0], n->Hf[0], n->in_mu,
+                     n->in_isd, n->Hb[0]);
   forward(n, rows);
   if (hipMemcpyAsync(Y, n->Hf[n->L], (size_t)rows * n->dims[n->L] * 4, hipMemcpyDeviceToHost, n->stream) !=
           hipSuccess ||
@@ -356,7 +378,8 @@ int felis_mlp_train_step(felis_mlp *n, int rows, const float *X, const float *T,
       hipMemsetAsync(n->loss, 0, 8, s) != hipSuccess)
     return FELIS_ERR_HIP;
   const long long ni = (long long)rows * n->dims[0];
-  hipLaunchKernelGGL(k_to_bf16, dim3(nblk(ni)), dim3(256), 0, s, ni, n->Hf[0], n->Hb[0]);
+  hipLaunchKernelGGL(k_input_bf16, dim3(nblk(ni)), dim3(256), 0, s, ni, n->dims[0], n->Hf[0], n->in_mu, n->in_isd,
+                     n->Hb[0]);
   forward(n, rows);
   // output layer: dZ = dY
   const long long no = (long long)rows * dout;
@@ -391,6 +414,23 @@ int felis_mlp_train_step(felis_mlp *n, int rows, const float *X, const float *T,
   if (hipMemcpyAsync(&lh, n->loss, 8, hipMemcpyDeviceToHost, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
     return FELIS_ERR_HIP;
   if (loss_out) *loss_out = lh;
+  return FELIS_OK;
+}
+
+// Input affine: the network reads (x - mu) * inv_sd (FP32) per input column.
+int felis_mlp_set_input_affine(felis_mlp *n, const float *mu, const float *inv_sd) {
+  if (!n || !mu || !inv_sd) return FELIS_ERR_ARG;
+  if (hipMemcpy(n->in_mu, mu, n->dims[0] * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(n->in_isd, inv_sd, n->dims[0] * 4, hipMemcpyHostToDevice) != hipSuccess)
+    return FELIS_ERR_HIP;
+  return FELIS_OK;
+}
+
+int felis_mlp_get_input_affine(felis_mlp *n, float *mu, float *inv_sd) {
+  if (!n) return FELIS_ERR_ARG;
+  if ((mu && hipMemcpy(mu, n->in_mu, n->dims[0] * 4, hipMemcpyDeviceToHost) != hipSuccess) ||
+      (inv_sd && hipMemcpy(inv_sd, n->in_isd, n->dims[0] * 4, hipMemcpyDeviceToHost) != hipSuccess))
+    return FELIS_ERR_HIP;
   return FELIS_OK;
 }
 

@@ -265,7 +265,7 @@ __global__ void __launch_bounds__(kTile)
             float *__restrict__ x, float *__restrict__ y, float *__restrict__ px, float *__restrict__ py,
             const double *__restrict__ seg_w, const cpx<float> *__restrict__ Ux, const cpx<float> *__restrict__ Uy,
             const cpx<double> *__restrict__ Ez, const double *__restrict__ bes, const double *__restrict__ wakeE,
-            double *__restrict__ tile_red, unsigned *err) {
+            double *__restrict__ tile_red, unsigned *err, const float *__restrict__ corr) {
   __shared__ double red[(2 * NH + 1 + 7) * kTile];
   const int t = blockIdx.x, k = threadIdx.x;
   const int seg = tl.seg[t];
@@ -312,6 +312,14 @@ __global__ void __launch_bounds__(kTile)
         th += tt * g.dz;
         th -= g.dz * wk;
       }
+    }
+    // learned correction of a multi-period jump (null on every plain step):
+    // per slice, theta and gamma move by c0 + c1 cos(theta) + c2 sin(theta)
+    if (corr) {
+      const float *cp = corr + (long long)sl * 6;
+      const cpx<float> e1 = expi_f32(th);
+      th += (double)(cp[0] + cp[1] * e1.re + cp[2] * e1.im);
+      gm += (double)(cp[3] + cp[4] * e1.re + cp[5] * e1.im);
     }
     // theta wrap (rearrangeParticles.cpp:84-108)
     const double dTh = g.dPhi;
@@ -678,8 +686,8 @@ __global__ void __launch_bounds__(1024)
 // species. tile_red holds moment q of tile t at [q * ntiles + t]. One
 // workgroup per output: strided partial sums (four independent chains per
 // thread), then a fixed tree.
-__global__ void __launch_bounds__(256)
-    k_reduce_diag(GridArgs g, const double *__restrict__ slice_e, int skip_last, const double *__restrict__ tile_red,
+static __global__ void __launch_bounds__(256)
+    k_reduce_diag(GridArgs g, const double *__restrict__ slice_e, int skip, const double *__restrict__ tile_red,
                   int nred, const int *__restrict__ species_tile, int n_species, double *__restrict__ out) {
   __shared__ double sh[256];
   const int o = blockIdx.x, tid = threadIdx.x, nh = g.nh;
@@ -689,7 +697,7 @@ __global__ void __launch_bounds__(256)
   if (o < 2 * nh) {
     row = slice_e + (long long)o * g.slice_n;
     k0 = 0;
-    k1 = g.slice_n - (skip_last ? 1 : 0);
+    k1 = g.slice_n - skip;  // slices shifted out since the energies were summed
   } else if (o < 4 * nh + 1) {
     row = tile_red + (long long)(o - 2 * nh) * ntiles;
     k0 = 0;

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -17,6 +18,9 @@
 #include "felis_ctx.h"
 #include "felis_adi.h"
 #include "felis_pstep.h"
+#include "felis_jump.h"
+#include "felis_mlp.h"
+#include "felis_sur.h"
 
 using namespace felis;
 
@@ -59,18 +63,19 @@ int enqueue_step(felis_ctx *ctx, const felis_lattice *lat, double *e_out, double
 
 // The batched ADI over every (slice, harmonic, pol) system with the scan
 // tables of adi_scan_tables (device copy in tab).
-int launch_adi_scan(felis_ctx *ctx, const cpx<float> *tab, int cf) {
+int launch_adi_scan(felis_ctx *ctx, const cpx<float> *tab, int cf, cpx<float> *Sc, const int4 *win,
+                    const cpx<float> *gain = nullptr) {
   GridArgs g = ctx->grid();
   const felis_desc &d = ctx->d;
   const int nsys = 2 * d.n_harm * d.slice_n;
   const size_t lds = adi_scan_lds(d.nx, d.ny);
   const int E = adi_scan_E(d.nx, d.ny);
-  cpx<float> *U = (cpx<float> *)ctx->U, *Sc = (cpx<float> *)ctx->Sc;
+  cpx<float> *U = (cpx<float> *)ctx->U;
   KTimer kt(ctx, FELIS_K_ADI_FUSED);
 #define ADI_CASE(e)                                                                                       \
   case e:                                                                                                 \
-    hipLaunchKernelGGL((k_adi_scan<e>), dim3(nsys), dim3(512), lds, ctx->stream, g, tab, cf, U, Sc, ctx->win, \
-                       ctx->slice_e);                                                                      \
+    hipLaunchKernelGGL((k_adi_scan<e>), dim3(nsys), dim3(512), lds, ctx->stream, g, tab, cf, U, Sc, win,      \
+                       ctx->slice_e, gain);                                                                \
     break;
   switch (E) {
     ADI_CASE(1) ADI_CASE(2) ADI_CASE(3) ADI_CASE(4) ADI_CASE(5) ADI_CASE(6) ADI_CASE(7) ADI_CASE(8) ADI_CASE(9)
@@ -110,13 +115,13 @@ bool adi_scan_fits(felis_ctx *ctx) {
 }
 
 template <int NH>
-int launch_particles(felis_ctx *ctx, const PStepArgs &a, double *tile_red) {
+int launch_particles(felis_ctx *ctx, const PStepArgs &a, double *tile_red, void *Ufield, const float *corr) {
   GridArgs g = ctx->grid();
   KTimer kt(ctx, FELIS_K_PARTICLE_STEP);
   hipLaunchKernelGGL((k_pstep<NH>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, g, a, ctx->tiles(), ctx->theta,
                      (float *)ctx->gamma, (float *)ctx->x, (float *)ctx->y, (float *)ctx->px, (float *)ctx->py,
-                     ctx->seg_w, (const cpx<float> *)ctx->U, (const cpx<float> *)((char *)ctx->U + ctx->pol_bytes()),
-                     (const cpx<double> *)ctx->Ez, ctx->bessel, ctx->wakeE, tile_red, ctx->errw);
+                     ctx->seg_w, (const cpx<float> *)Ufield, (const cpx<float> *)((char *)Ufield + ctx->pol_bytes()),
+                     (const cpx<double> *)ctx->Ez, ctx->bessel, ctx->wakeE, tile_red, ctx->errw, corr);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ctx->err = std::string("particle step launch: ") + hipGetErrorString(e);
@@ -155,8 +160,8 @@ int launch_deposit_slot(felis_ctx *ctx, const DepArgs &dep, int lds) {
     default: return FELIS_ERR_UNSUPPORTED;       \
   }
 
-int particles(felis_ctx *ctx, const PStepArgs &a, double *tile_red) {
-  NH_DISPATCH(launch_particles, ctx->d.n_harm, ctx, a, tile_red)
+int particles(felis_ctx *ctx, const PStepArgs &a, double *tile_red, void *Ufield, const float *corr = nullptr) {
+  NH_DISPATCH(launch_particles, ctx->d.n_harm, ctx, a, tile_red, Ufield, corr)
 }
 int deposit_slot(felis_ctx *ctx, const DepArgs &dep, int lds) {
   NH_DISPATCH(launch_deposit_slot, ctx->d.n_harm, ctx, dep, lds)
@@ -180,9 +185,67 @@ hipError_t deposit_lds_attr(int nh, int lds) {
   }
 }
 
-// The batched FP32/FP64 loop.
+// Multi-period jump settings of felis_run_surrogate (null for felis_run).
+struct JumpCtl {
+  felis_jump_opts o;
+  int *kind = nullptr;          // per step: 0 fine, 1 jump start, 2 inside a jump
+  cpx<float> *tab = nullptr;    // ADI scan tables for a field step of J * dz
+  int jumps = 0;
+  // per-slice moments, network inputs and outputs, per-system gain, totals
+  double *M = nullptr, *tot = nullptr;
+  float *X = nullptr, *Y = nullptr, *corr = nullptr;
+  cpx<float> *gain = nullptr;
+};
+
+// the lattice of steps k..k+J-1 is one undulator section with the field
+// solve's deposit on (a jump takes one lattice for all J periods)
+bool uniform_section(const felis_lattice *lat_tab, int k, int J) {
+  const felis_lattice &a = lat_tab[2 * k + 2];
+  if (a.drift_flag || !a.current_flag) return false;
+  for (int q = 2 * k; q <= 2 * (k + J); ++q) {
+    const felis_lattice &b = lat_tab[q];
+    if (b.K0 != a.K0 || b.ue != a.ue || b.K0_alpha != a.K0_alpha || b.ku != a.ku || b.g != a.g ||
+        b.und_type != a.und_type || b.drift_flag || !b.current_flag)
+      return false;
+  }
+  return true;
+}
+
+// buffers of the jump's window passes, allocated on first use and kept
+int jump_buffers(felis_ctx *ctx) {
+  if (ctx->Sw) return FELIS_OK;
+  const size_t fsz = (size_t)2 * ctx->d.n_harm * ctx->S * ctx->N * ctx->esize();
+  void *sw = nullptr, *ub = nullptr;
+  int4 *w2 = nullptr, *bx = nullptr;
+  if (hipMalloc(&sw, fsz) != hipSuccess || hipMalloc(&ub, fsz) != hipSuccess ||
+      hipMalloc((void **)&w2, (size_t)ctx->S * sizeof(int4)) != hipSuccess ||
+      hipMalloc((void **)&bx, 2 * sizeof(int4)) != hipSuccess) {
+    (void)hipFree(sw);
+    (void)hipFree(ub);
+    (void)hipFree(w2);
+    (void)hipFree(bx);
+    ctx->err = "jump buffers";
+    return FELIS_ERR_HIP;
+  }
+  ctx->allocs.push_back(sw);
+  ctx->allocs.push_back(ub);
+  ctx->allocs.push_back(w2);
+  ctx->allocs.push_back(bx);
+  ctx->Sw = sw;
+  ctx->Ub = ub;
+  ctx->win2 = w2;
+  ctx->box = bx;
+  hipLaunchKernelGGL(k_fill_u32, dim3(4096), dim3(256), 0, ctx->stream, (unsigned *)sw,
+                     (long long)(fsz / sizeof(unsigned)), 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(4096), dim3(256), 0, ctx->stream, (unsigned *)ub,
+                     (long long)(fsz / sizeof(unsigned)), 0u);
+  return FELIS_OK;
+}
+
+// The batched FP32/FP64 loop; with jc, iterations whose gate is open advance
+// J periods at once (felis_jump.h).
 int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps, int wake_step,
-              const double *wakeF, double *dev_out, size_t per) {
+              const double *wakeF, double *dev_out, size_t per, JumpCtl *jc) {
   const felis_desc &d = ctx->d;
   if (d.sc_l != 1) {
     ctx->err = "the batched path models one longitudinal space-charge mode (number_longitudinal_mode = 1)";
@@ -191,6 +254,10 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
   // LDS-resident scan ADI when the system fits, else the two-pass ADI with a
   // full clear of Sc after each use
   const bool fused = adi_scan_fits(ctx);
+  if (jc && !fused) {
+    ctx->err = "jumps need the LDS-resident ADI (grid too wide)";
+    return FELIS_ERR_UNSUPPORTED;
+  }
   // the batched deposit adapts to any LDS budget (harmonics in chunks, else
   // global atomics); FELIS_DEPOSIT_LDS overrides the default for tuning
   int dep_lds = ctx->lds_max;
@@ -212,6 +279,13 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
     (void)hipFree(species_tile);
     (void)hipFree(adi_tab);
     (void)hipFree(scratch_e);
+    if (jc) {
+      (void)hipFree(jc->tab);
+      (void)hipFree(jc->M);
+      jc->tab = nullptr;
+      jc->M = nullptr;
+    }
+    ctx->dz_mul = 1.0;
   };
   if (hipMalloc((void **)&tile_red, (size_t)std::max(1, ctx->ntiles) * nred * sizeof(double)) != hipSuccess ||
       hipMalloc((void **)&species_tile, (ns + 1) * sizeof(int)) != hipSuccess ||
@@ -228,6 +302,41 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
     ctx->err = "species_tile upload";
     return FELIS_ERR_HIP;
   }
+  const int J = jc ? jc->o.periods : 1;
+  if (jc) {
+    // the field solve of a jump: ABC and Thomas tables for a step of J dz
+    std::vector<cpx<double>> abcJ, adixJ, adiyJ;
+    std::vector<cpx<float>> tabJ;
+    abc_tables(d, d.dz * J, abcJ, adixJ, adiyJ);
+    adi_scan_tables(d.nx, d.ny, nh, adixJ, adiyJ, abcJ, tabJ);
+    int e = jump_buffers(ctx);
+    if (!e && (hipMalloc((void **)&jc->tab, tabJ.size() * sizeof(cpx<float>)) != hipSuccess ||
+               hipMemcpy(jc->tab, tabJ.data(), tabJ.size() * sizeof(cpx<float>), hipMemcpyHostToDevice) !=
+                   hipSuccess)) {
+      ctx->err = "jump tables";
+      e = FELIS_ERR_HIP;
+    }
+    // one buffer: moments, totals, inputs, outputs, gain
+    const size_t nm = (size_t)d.slice_n * moments_per_slice(nh), nx_ = (size_t)d.slice_n * features_per_slice(nh),
+                 ny_ = (size_t)d.slice_n * outputs_per_slice(nh), ng = (size_t)2 * nh * d.slice_n,
+                 nc = (size_t)d.slice_n * 6;
+    if (!e && hipMalloc((void **)&jc->M, (nm + 4) * sizeof(double) + ng * sizeof(cpx<float>) +
+                                             (nx_ + ny_ + nc) * sizeof(float)) != hipSuccess) {
+      ctx->err = "jump correction buffers";
+      e = FELIS_ERR_HIP;
+    }
+    if (!e) {
+      jc->tot = jc->M + nm;
+      jc->gain = (cpx<float> *)(jc->tot + 4);
+      jc->X = (float *)(jc->gain + ng);
+      jc->Y = jc->X + nx_;
+      jc->corr = jc->Y + ny_;
+    }
+    if (e) {
+      cleanup();
+      return e;
+    }
+  }
   const size_t ne = 2 * (size_t)nh, nbf = 2 * (size_t)nh + 1;
   // Sc starts clear; the fused ADI clears every interior cell it reads
   const size_t fsz = (size_t)2 * nh * ctx->S * ctx->N * ctx->esize();
@@ -235,9 +344,10 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
                      (long long)(fsz / sizeof(unsigned)), 0u);
   hipLaunchKernelGGL(k_win_empty, dim3((unsigned)((ctx->S + 255) / 256)), dim3(256), 0, ctx->stream, ctx->win,
                      ctx->S);
-  bool prev_shift = false;
+  int shifted = 0;  // slots the last iteration slipped the field by
+  bool gate = jc != nullptr;
   int err = FELIS_OK;
-  for (int k = 0; k < n_steps && !err; ++k) {
+  for (int k = 0; k < n_steps && !err;) {
     const int it = it0 + k;
     const felis_lattice *ls = &lat_tab[2 * k], *l0 = &lat_tab[2 * k + 1], *l1 = &lat_tab[2 * k + 2];
     if (d.mode == 2 && wakeF && wake_step > 0 && it % wake_step == 0) {
@@ -255,12 +365,91 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
       GridArgs g = ctx->grid();
       KTimer kt(ctx, FELIS_K_ENERGY);
       hipLaunchKernelGGL(k_reduce_diag, dim3((unsigned)(ne + nbf + 7 * ns)), dim3(256), 0, ctx->stream, g,
-                         ctx->slice_e, prev_shift ? 1 : 0, tile_red, nred, species_tile, ns, row);
+                         ctx->slice_e, shifted, tile_red, nred, species_tile, ns, row);
+    }
+    // jump gate: J periods left in one undulator section with no wake update
+    // inside, the jump budget not spent, and the fundamental bunching of the
+    // state below the threshold (read back: the host decides the launches)
+    bool jump = false;
+    if (gate) {
+      const felis_jump_opts &o = jc->o;
+      bool ok = k + J <= n_steps && (o.max_jumps < 0 || jc->jumps < o.max_jumps) &&
+                uniform_section(lat_tab, k, J) && d.mode == 2;
+      if (ok && wakeF && wake_step > 0) {
+        for (int q = 1; q < J && ok; ++q) ok = (it + q) % wake_step != 0;
+      }
+      if (ok) {
+        // the network's inputs (and the energy spread) of the state before the jump
+        const bool moments = o.net || o.spread_max > 0;
+        if (moments && ((err = felis_slice_features_dev(ctx, jc->M, jc->X)) ||
+                        (err = felis_moment_totals_dev(ctx, jc->M, jc->tot))))
+          break;
+        double b[6] = {0, 0, 0, 0, 0, 0};
+        double *bsrc = row + ne;
+        if (hipMemcpyAsync(b, bsrc, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipMemcpyAsync(b + 2, bsrc + 2 * nh, sizeof(double), hipMemcpyDeviceToHost, ctx->stream) !=
+                hipSuccess ||
+            (moments &&
+             hipMemcpyAsync(b + 3, jc->tot, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess) {
+          ctx->err = "jump gate read";
+          err = FELIS_ERR_HIP;
+          break;
+        }
+        const double b1 = b[2] > 0 ? std::sqrt(b[0] * b[0] + b[1] * b[1]) / b[2] : 0.0;
+        if (!(b1 < o.bunching_max)) gate = false;  // the linear regime has ended: no more jumps
+        if (o.spread_max > 0 && b[3] > 0) {
+          const double mu = b[4] / b[3], var = std::max(b[5] / b[3] - mu * mu, 0.0);
+          if (!(std::sqrt(var) / d.gamma_ref < o.spread_max)) gate = false;
+        }
+        ok = gate;
+      }
+      jump = ok;
+    }
+    if (jc && jc->kind) {
+      jc->kind[k] = jump ? 1 : 0;
+      for (int q = 1; jump && q < J; ++q) jc->kind[k + q] = 2;
     }
     // solveField: the radial Ez from the last deposit, then the ADI
     if (ls->current_flag && (err = ctx->ez_solve(ls))) break;
-    if (fused) {
-      if ((err = launch_adi_scan(ctx, adi_tab, ls->current_flag))) break;
+    GridArgs gw = ctx->grid();
+    const unsigned ncb = (unsigned)((ctx->N + 255) / 256);
+    const unsigned nchunk = (unsigned)((d.slice_n + kJumpChunk - 1) / kJumpChunk);
+    if (jump) {
+      // source of the J slices each field slot passes, then the ADI over J dz
+      hipLaunchKernelGGL(k_win_bbox, dim3(1), dim3(256), 0, ctx->stream, ctx->win, d.slice_n, d.nx, d.ny, 0,
+                         ctx->box);
+      hipLaunchKernelGGL(k_win_union, dim3((unsigned)((ctx->S + 255) / 256)), dim3(256), 0, ctx->stream, ctx->win,
+                         d.slice_n, J, ctx->win2);
+      {
+        KTimer kt(ctx, FELIS_K_JUMP_WINDOW);
+        hipLaunchKernelGGL(k_src_window, dim3(ncb, 2 * nh, nchunk), dim3(256), 0, ctx->stream, d.nx, d.ny, ctx->S,
+                           d.slice_n, J, ctx->box, (const cpx<float> *)ctx->Sc, (cpx<float> *)ctx->Sw);
+        hipLaunchKernelGGL(k_clear_box, dim3(ncb, 2 * nh, nchunk), dim3(256), 0, ctx->stream, d.nx, d.ny, ctx->S,
+                           d.slice_n, ctx->box, (cpx<float> *)ctx->Sc);
+      }
+      // the learned correction: per-slice inputs -> network -> field gain
+      // and particle correction
+      const cpx<float> *gain = nullptr;
+      if (jc->o.net) {
+        if ((err = felis_mlp_forward_dev(jc->o.net, d.slice_n, jc->X, jc->Y, ctx->stream)) ||
+            (err = felis_decode_corr_dev(ctx, jc->Y, jc->M, jc->gain, jc->corr)))
+          break;
+        gain = jc->gain;
+      }
+      if ((err = launch_adi_scan(ctx, jc->tab, 1, (cpx<float> *)ctx->Sw, ctx->win2, gain))) break;
+      // the push field: the J field slots that pass each slice (the jump's
+      // own field, without the learned gain)
+      hipLaunchKernelGGL(k_win_bbox, dim3(1), dim3(256), 0, ctx->stream, ctx->win, d.slice_n, d.nx, d.ny,
+                         kJumpMargin, ctx->box + 1);
+      {
+        KTimer kt(ctx, FELIS_K_JUMP_WINDOW);
+        hipLaunchKernelGGL(k_field_window, dim3(ncb, 2 * nh, nchunk), dim3(256), 0, ctx->stream, d.nx, d.ny,
+                           ctx->S, d.slice_n, gw.ring_base, J, ctx->box + 1, (const cpx<float> *)ctx->U, gain,
+                           (cpx<float> *)ctx->Ub);
+      }
+    } else if (fused) {
+      if ((err = launch_adi_scan(ctx, adi_tab, ls->current_flag, (cpx<float> *)ctx->Sc, ctx->win))) break;
     } else {
       if ((err = ctx->adi_only(ls->current_flag)) || (err = ctx->energy(scratch_e))) break;
       if (ls->current_flag) {
@@ -297,25 +486,41 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
       a.hu2[h] = (float)(1.0 / (2.0 * H * H));
       a.hstep[h] = H - (h ? d.harmony[h - 1] : 0);
     }
-    if ((err = particles(ctx, a, tile_red))) break;
+    if (jump) ctx->dz_mul = J;
+    err = particles(ctx, a, tile_red, jump ? ctx->Ub : ctx->U, jump && jc->o.net ? jc->corr : nullptr);
+    ctx->dz_mul = 1.0;
+    if (err) break;
+    const int kn = k + (jump ? J : 1);  // the next iteration
+    const felis_lattice *ln = jump ? &lat_tab[2 * kn] : l1;
     // the next solveField's deposit, with the lattice it will run under
-    if (l1->current_flag) {
+    if (ln->current_flag) {
       DepArgs dep;
-      if ((err = dep_args(ctx, l1, &dep))) break;
+      if ((err = dep_args(ctx, ln, &dep))) break;
       if ((err = deposit_slot(ctx, dep, dep_lds))) break;
     }
-    prev_shift = !l1->drift_flag && d.mode == 2;
-    if (prev_shift && (err = ctx->shift())) break;
-    if ((k & 31) == 31 || k == n_steps - 1) err = ctx->check_err();
+    if (jump) {
+      jc->jumps++;
+      ctx->ring_base = ((ctx->ring_base - J) % ctx->S + ctx->S) % ctx->S;
+      KTimer kt(ctx, FELIS_K_SHIFT);
+      hipLaunchKernelGGL(k_zero_slots, dim3(ncb, 2 * nh, J), dim3(256), 0, ctx->stream, (cpx<float> *)ctx->U,
+                         ctx->ring_base, ctx->S, (long long)ctx->N, J);
+      shifted = J;
+    } else {
+      shifted = (!l1->drift_flag && d.mode == 2) ? 1 : 0;
+      if (shifted && (err = ctx->shift())) break;
+    }
+    if ((k & 31) == 31 || kn >= n_steps || jump) err = ctx->check_err();
+    k = kn;
   }
   cleanup();
   return err;
 }
 
-}  // namespace
-
-extern "C" int felis_run(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps, int wake_step,
-                         const double *wakeF, double *energy, double *bfac, double *twiss) {
+// Shared by felis_run and felis_run_surrogate: the loop, then the per-step
+// outputs copied out of one device buffer (rows of periods inside a jump
+// are NaN).
+int run_common(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps, int wake_step,
+               const double *wakeF, double *energy, double *bfac, double *twiss, JumpCtl *jc) {
   if (!ctx || !lat_tab || n_steps < 0) return FELIS_ERR_ARG;
   const felis_desc &d = ctx->d;
   const int nh = d.n_harm, ns = d.n_species;
@@ -329,7 +534,7 @@ extern "C" int felis_run(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, 
       ctx->err = "no particles uploaded";
       err = FELIS_ERR_STATE;
     } else {
-      err = run_mixed(ctx, lat_tab, it0, n_steps, wake_step, wakeF, dev, per);
+      err = run_mixed(ctx, lat_tab, it0, n_steps, wake_step, wakeF, dev, per, jc);
     }
   } else {
     for (int k = 0; k < n_steps && !err; ++k) {
@@ -352,8 +557,10 @@ extern "C" int felis_run(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, 
       ctx->err = std::string("run outputs: ") + hipGetErrorString(he);
       err = FELIS_ERR_HIP;
     } else {
+      const double nan = std::nan("");
       for (int k = 0; k < n_steps; ++k) {
-        const double *row = h.data() + (size_t)k * per;
+        double *row = h.data() + (size_t)k * per;
+        if (jc && jc->kind && jc->kind[k] == 2) std::fill(row, row + per, nan);
         if (energy) memcpy(energy + (size_t)k * ne, row, ne * sizeof(double));
         if (bfac) memcpy(bfac + (size_t)k * nb, row + ne, nb * sizeof(double));
         if (twiss) memcpy(twiss + (size_t)k * nt, row + ne + nb, nt * sizeof(double));
@@ -363,5 +570,42 @@ extern "C" int felis_run(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, 
     (void)hipStreamSynchronize(ctx->stream);
   }
   (void)hipFree(dev);
+  return err;
+}
+
+}  // namespace
+
+extern "C" int felis_run(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps, int wake_step,
+                         const double *wakeF, double *energy, double *bfac, double *twiss) {
+  return run_common(ctx, lat_tab, it0, n_steps, wake_step, wakeF, energy, bfac, twiss, nullptr);
+}
+
+extern "C" int felis_run_surrogate(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps,
+                                   int wake_step, const double *wakeF, const felis_jump_opts *opts, double *energy,
+                                   double *bfac, double *twiss, int *kind) {
+  if (!ctx || !opts || !lat_tab || n_steps < 0) return FELIS_ERR_ARG;
+  const felis_desc &d = ctx->d;
+  if (!ctx->mixed() || d.dimension != 3) {
+    ctx->err = "jumps run in the batched FP32/FP64 path (3D)";
+    return FELIS_ERR_UNSUPPORTED;
+  }
+  if (opts->periods < 1 || opts->periods > 64 || opts->periods > d.slice_n) {
+    ctx->err = "jump periods must be in 1..min(64, slice_n)";
+    return FELIS_ERR_ARG;
+  }
+  if (opts->net) {
+    int in = 0, out = 0, cap = 0;
+    felis_mlp_shape(opts->net, &in, &out, &cap);
+    if (in != features_per_slice(d.n_harm) || out != outputs_per_slice(d.n_harm) || cap < d.slice_n) {
+      ctx->err = "correction network: inputs 5 + 5 n_harm, outputs 2 n_harm + 4, capacity >= slice_n";
+      return FELIS_ERR_ARG;
+    }
+  }
+  JumpCtl jc;
+  jc.o = *opts;
+  std::vector<int> kv(std::max(1, n_steps), 0);
+  jc.kind = kv.data();
+  const int err = run_common(ctx, lat_tab, it0, n_steps, wake_step, wakeF, energy, bfac, twiss, &jc);
+  if (kind) std::copy(kv.begin(), kv.begin() + n_steps, kind);
   return err;
 }

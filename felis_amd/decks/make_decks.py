@@ -35,7 +35,8 @@ BETA = SIG ** 2 * GAMMA0 / (EMIT * 1e-6)
 
 
 def deck(*, mode: str, slices: int, n: int, harmonics, length: float, beam: str, extra_undulator: str = "",
-         wake: str = "activate = OFF\n radius = 4.5e-3") -> str:
+         wake: str = "activate = OFF\n radius = 4.5e-3", seed_power: float = 1e4,
+         energy_spread_pct: float = 0.01) -> str:
     maxz_um = (slices + 0.5) * LAMBDA0 * 1e6
     harm = "\n".join(f" harmony{i} = {h}" for i, h in enumerate(harmonics))
     return f"""// generated by felis_amd/decks/make_decks.py
@@ -66,7 +67,7 @@ def deck(*, mode: str, slices: int, n: int, harmonics, length: float, beam: str,
  ABC_coef = 1
 
 [Seed]
- power = 1e4          // [W]
+ power = {seed_power:g}          // [W]
  spot_sigma_R = 20    // [um]
  rms_duration = 1e6   // [fs] (flat over the window)
  focus = 0            // [m]
@@ -89,7 +90,7 @@ def deck(*, mode: str, slices: int, n: int, harmonics, length: float, beam: str,
 
 [EBeam]
  beam_energy = {(GAMMA0 - 1) * MC2:.9f}   // [MeV]
- energy_spread = 0.01     // [%]
+ energy_spread = {energy_spread_pct:g}     // [%]
  norm_emittance_x = {EMIT}
  norm_emittance_y = {EMIT}
  beta_x = {BETA:.9f}

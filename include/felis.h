@@ -147,6 +147,62 @@ int felis_calculate_twiss(felis_ctx *ctx, double *sums /* [n_species][7] */);
 int felis_run(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps, int wake_step,
               const double *wakeF, double *energy, double *bfac, double *twiss);
 
+/* -------- multi-period jumps (no counterpart in the reference) ---------- */
+/* While the gate is open, one iteration advances every slice `periods`
+ * undulator periods: one field solve and one particle push of step
+ * periods * dz, with the slippage of the fine steps kept by two window
+ * passes (each field slot takes the source of the slices it slips past; each
+ * slice is pushed in the mean of the field slots that slip past it), then a
+ * slippage of `periods` slots. The gate needs `periods` steps left in one
+ * undulator section with no wake update inside, and the fundamental
+ * bunching |b1| of the state below bunching_max; once it fails on the
+ * bunching, the rest of the run takes fine steps (the plain batched step). */
+typedef struct felis_mlp felis_mlp;
+typedef struct {
+  int periods;          /* J, 1..min(64, slice_n) */
+  int max_jumps;        /* jumps allowed in this call (< 0: no limit) */
+  double bunching_max;  /* open while |b1| < bunching_max */
+  double spread_max;    /* ... and the rms relative energy spread < spread_max (<= 0: not checked) */
+  felis_mlp *net;       /* learned correction (NULL: the jump alone): maps the
+                           felis_slice_features of each slice before the jump to
+                           (log|g_h|, arg g_h) per harmonic, the complex gain applied to
+                           the jump's new field, then db/b_1 and dP/P_1 (re, im), the
+                           change of bunching and energy modulation applied to the
+                           slice's particles in the jump's push (2*n_harm + 4 outputs) */
+} felis_jump_opts;
+/* felis_run with jumps (FELIS_MIXED_FP32, Time_Dependent). kind[n_steps]
+ * (may be NULL): 0 a fine step, 1 the first period of a jump, 2 a period
+ * inside a jump, whose output rows are NaN. */
+int felis_run_surrogate(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps, int wake_step,
+                        const double *wakeF, const felis_jump_opts *opts, double *energy, double *bfac,
+                        double *twiss, int *kind);
+
+/* Device-side snapshots of the evolving state (field with its slippage
+ * position, particles, per-slice wake) in one of two slots, batched 3D path:
+ * the training pipeline of the jump correction runs the fine steps and a
+ * jump from one state. */
+int felis_state_save(felis_ctx *ctx, int slot);
+int felis_state_restore(felis_ctx *ctx, int slot);
+/* Per-slice moments of the current state, out[slice_n][5 + 8*n_harm]:
+ * sum w, sum w dg, sum w dg^2, sum w dg e^{i theta} (re, im), then per
+ * harmonic sum w e^{iH theta}, sum w Ux e^{iH theta}, sum w Uy e^{iH theta}
+ * (re, im each), sum w |Ux|^2, sum w |Uy|^2; dg = gamma - gamma_ref, U
+ * bilinear at the particle. */
+int felis_slice_moments(felis_ctx *ctx, double *out);
+/* The correction network's inputs per slice, out[slice_n][5 + 5*n_harm]
+ * (FP32): log10 slice current [kA]; per harmonic log10 mean |U|^2 at the
+ * particles, log10 |b_h|, the coherent fraction |sum w U e^{iH theta}| /
+ * sqrt(sum w sum w |U|^2) and the cos, sin of its phase; log10 rms relative
+ * energy spread, the energy modulation fraction and the cos, sin of its
+ * phase against b_1. */
+int felis_slice_features(felis_ctx *ctx, float *out);
+/* The current state (a) against snapshot `slot` (b), same slippage:
+ * field_out[2*n_harm*slice_n][4] = sum conj(a) b (re, im), sum |a|^2,
+ * sum |b|^2 per (pol x harmonic, slice); part_out[slice_n][12] = sums of w
+ * times 1, c, s, c^2, s^2, cs, dth, dth c, dth s, dgm, dgm c, dgm s with c, s
+ * of theta_a, dth = theta_b - theta_a wrapped, dgm = gamma_b - gamma_a. */
+int felis_jump_residual(felis_ctx *ctx, int slot, double *field_out, double *part_out);
+
 /* -------- device ------------------------------------------------------------ */
 /* info[8]: compute units, max LDS bytes per workgroup, LDS bytes per CU,
  * wavefront size, L2 bytes, clock kHz, memory clock kHz, memory bus width. */
@@ -159,7 +215,7 @@ enum felis_kernel_id {
   FELIS_K_TRANSVERSE = 0, FELIS_K_PUSH, FELIS_K_DRIFT, FELIS_K_PERIODIC, FELIS_K_DEPOSIT,
   FELIS_K_EZ_DEPOSIT, FELIS_K_EZ_SOLVE, FELIS_K_ADI_X, FELIS_K_ADI_Y, FELIS_K_SHIFT,
   FELIS_K_ENERGY, FELIS_K_BFACTOR, FELIS_K_TWISS, FELIS_K_WAKE, FELIS_K_ZERO,
-  FELIS_K_PARTICLE_STEP, FELIS_K_ADI_FUSED, FELIS_K_COUNT
+  FELIS_K_PARTICLE_STEP, FELIS_K_ADI_FUSED, FELIS_K_JUMP_WINDOW, FELIS_K_COUNT
 };
 /* enable = 1 clears the counters and starts timing, 0 stops it. */
 int felis_profile(felis_ctx *ctx, int enable);
@@ -199,12 +255,17 @@ int felis_deck_beam_sigma(const felis_deck *deck, double *sig);
  * the reference; BF16 operands, FP32 accumulation and FP32 master weights).
  * dims[0..layers] are the widths; hidden layers use GELU, the output is
  * linear. Host arrays, row-major, rows <= capacity. */
-typedef struct felis_mlp felis_mlp;
 int felis_mlp_create(int layers, const int *dims, int capacity, uint64_t seed, felis_mlp **out);
 int felis_mlp_destroy(felis_mlp *net);
+/* in/out widths and row capacity */
+int felis_mlp_shape(const felis_mlp *net, int *in, int *out, int *capacity);
 const char *felis_mlp_last_error(const felis_mlp *net);
 int felis_mlp_get_layer(felis_mlp *net, int layer, float *W /* [in][out] */, float *b /* [out] */);
 int felis_mlp_set_layer(felis_mlp *net, int layer, const float *W, const float *b);
+/* the network reads (x - mu) * inv_sd per input column, in FP32 before the
+ * BF16 rounding (default mu = 0, inv_sd = 1) */
+int felis_mlp_set_input_affine(felis_mlp *net, const float *mu, const float *inv_sd);
+int felis_mlp_get_input_affine(felis_mlp *net, float *mu, float *inv_sd);
 int felis_mlp_forward(felis_mlp *net, int rows, const float *X, float *Y);
 /* one Adam step on the (row-weighted, wrow may be NULL) mean squared error;
  * lr = 0 only computes the gradients (felis_mlp_get_grad) */

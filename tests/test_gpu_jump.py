@@ -1,0 +1,293 @@
+"""Multi-period jumps of the batched step and the learned correction's device
+parts, through the C-ABI on the 64-slice cut of the 2048-slice workload
+(129 x 129, harmonics 1, 3, 5, Gaussian current).
+
+Parity and tolerances:
+  - a jump of one period is the fine step (the window passes reduce to
+    copies): total field energy and bunching agree to 1e-6 relative, the
+    rounding left by the deposit's atomic summation order;
+  - snapshots restore the state bit for bit;
+  - the per-slice moments, network inputs and jump residual sums match a
+    numpy restatement from the downloaded state to FP32 rounding (1e-4
+    relative of each sum's scale);
+  - a network whose outputs are a constant (log|g|, arg g) multiplies the
+    jump's new field by exactly that g (1e-5);
+  - jumps of 5 periods follow the fine run's field energy within 10% over
+    the first 100 periods (the shot-noise-dominated start, the jump's
+    largest error without a correction).
+"""
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+fl = pytest.importorskip("felis_amd._lib")
+
+ROOT = Path(__file__).resolve().parent.parent
+DECK = ROOT / "felis_amd" / "decks" / "cfg2s.inp"
+SEED = 20261203
+
+
+def _ctx():
+    deck = fl.Deck(DECK)
+    d = deck.desc(fl.MIXED_FP32)
+    ctx = fl.Context(d)
+    deck.load(ctx, seed=SEED)
+    return deck, d, ctx
+
+
+def _energy(e, nh):
+    return e[:, :nh].sum(1) + e[:, nh:].sum(1)
+
+
+def test_one_period_jump_is_the_fine_step():
+    deck, d, c1 = _ctx()
+    nh = d.n_harm
+    N = 30
+    tab = deck.lattice(0, N)
+    e1, b1, t1 = c1.run(tab, 0, N)
+    _, _, c2 = _ctx()
+    e2, b2, t2, kind = c2.run_surrogate(tab, 0, N, periods=1, bunching_max=math.inf)
+    # step 0 runs under the lattice handed in before the loop: a fine step
+    assert np.all(kind[1:] == 1)
+    E1, E2 = _energy(e1, nh), _energy(e2, nh)
+    assert np.max(np.abs(E2 - E1) / E1) < 1e-6
+    assert np.max(np.abs(b2 - b1)) < 1e-6 * np.max(np.abs(b1))
+    U1, U2 = c1.download_field(fl.UX), c2.download_field(fl.UX)
+    assert np.max(np.abs(U2 - U1)) < 1e-5 * np.max(np.abs(U1))
+    c1.close()
+    c2.close()
+
+
+def test_snapshot_restores_the_state():
+    deck, d, ctx = _ctx()
+    tab = deck.lattice(0, 12)
+    ctx.run(tab[:9], 0, 4, outputs=False)
+    p0 = ctx.download_particles()
+    U0 = ctx.download_field(fl.UX)
+    ctx.save_state(0)
+    ctx.run(tab[8:], 4, 8, outputs=False)
+    assert not np.array_equal(ctx.download_field(fl.UX), U0)
+    ctx.restore_state(0)
+    p1 = ctx.download_particles()
+    for k in p0:
+        assert np.array_equal(p0[k], p1[k]), k
+    assert np.array_equal(ctx.download_field(fl.UX), U0)
+    ctx.close()
+
+
+def _bilinear(U, x, y, d):
+    """U [ny, nx] at particle positions, the kernels' cell and weights."""
+    fx = (x - d.min_x) / d.dx
+    fy = (y - d.min_y) / d.dy
+    i = fx.astype(np.int64)
+    j = fy.astype(np.int64)
+    inside = (fx >= 0) & (fy >= 0) & (i < d.nx - 1) & (j < d.ny - 1)
+    i = np.where(inside, i, 0)
+    j = np.where(inside, j, 0)
+    wx1 = fx - i
+    wy1 = fy - j
+    v = (U[j, i] * (1 - wx1) * (1 - wy1) + U[j + 1, i] * (1 - wx1) * wy1 + U[j, i + 1] * wx1 * (1 - wy1)
+         + U[j + 1, i + 1] * wx1 * wy1)
+    return np.where(inside, v, 0.0)
+
+
+def _moments_numpy(deck, d, ctx):
+    p = ctx.download_particles()
+    Ux, Uy = ctx.download_field(fl.UX), ctx.download_field(fl.UY)
+    so, sw = deck.beam_layout(d.n_species)
+    nh = d.n_harm
+    M = np.zeros((d.slice_n, 5 + 8 * nh))
+    dg = p["gamma"] - d.gamma_ref
+    for sl in range(d.slice_n):
+        for sp in range(d.n_species):
+            seg = sp * d.slice_n + sl
+            a, b = so[seg], so[seg + 1]
+            w = sw[seg]
+            th = p["theta"][a:b]
+            g = dg[a:b]
+            e1 = np.exp(1j * th)
+            M[sl, 0] += w * (b - a)
+            M[sl, 1] += w * g.sum()
+            M[sl, 2] += w * (g * g).sum()
+            M[sl, 3] += w * (g * e1.real).sum()
+            M[sl, 4] += w * (g * e1.imag).sum()
+            for h in range(nh):
+                H = d.harmony[h]
+                eh = np.exp(1j * H * th)
+                ux = _bilinear(Ux[h, sl + 1], p["x"][a:b], p["y"][a:b], d)
+                uy = _bilinear(Uy[h, sl + 1], p["x"][a:b], p["y"][a:b], d)
+                q = 5 + 8 * h
+                M[sl, q] += w * eh.real.sum()
+                M[sl, q + 1] += w * eh.imag.sum()
+                M[sl, q + 2] += w * (ux * eh).real.sum()
+                M[sl, q + 3] += w * (ux * eh).imag.sum()
+                M[sl, q + 4] += w * (uy * eh).real.sum()
+                M[sl, q + 5] += w * (uy * eh).imag.sum()
+                M[sl, q + 6] += w * (np.abs(ux) ** 2).sum()
+                M[sl, q + 7] += w * (np.abs(uy) ** 2).sum()
+    return M
+
+
+def test_slice_moments_and_features():
+    deck, d, ctx = _ctx()
+    tab = deck.lattice(0, 40)
+    ctx.run(tab, 0, 40, outputs=False)  # bunching and energy modulation of a few periods
+    M = ctx.slice_moments()
+    Mr = _moments_numpy(deck, d, ctx)
+    for q in range(M.shape[1]):
+        scale = np.max(np.abs(Mr[:, q])) + 1e-300
+        assert np.max(np.abs(M[:, q] - Mr[:, q])) < 1e-4 * scale, q
+    # network inputs from the moments (layout of felis_slice_features)
+    X = ctx.slice_features()
+    nh = d.n_harm
+    amp_kA = 1.60217662e-19 * 299792458.0 / (d.lambda0 * d.num_slice) * 1e-3
+    w = Mr[:, 0]
+    assert np.allclose(X[:, 0], np.log10(w * amp_kA + 1e-9), atol=1e-4)
+    for h in range(nh):
+        q = Mr[:, 5 + 8 * h:13 + 8 * h]
+        I = (q[:, 6] + q[:, 7]) / w
+        fa = np.hypot(q[:, 2] + q[:, 4], q[:, 3] + q[:, 5])
+        assert np.allclose(X[:, 1 + 5 * h], np.log10(I + 1e-30), atol=1e-3)
+        assert np.allclose(X[:, 2 + 5 * h], np.log10(np.hypot(q[:, 0], q[:, 1]) / w + 1e-12), atol=1e-3)
+        assert np.allclose(X[:, 3 + 5 * h], fa / np.sqrt(w * (q[:, 6] + q[:, 7]) + 1e-30), atol=1e-4)
+    ctx.close()
+
+
+def test_jump_residual_sums():
+    deck, d, ctx = _ctx()
+    nh = d.n_harm
+    tab = deck.lattice(0, 15)
+    ctx.run(tab[:21], 0, 10, outputs=False)
+    ctx.save_state(0)
+    tab2 = tab[20:31]  # steps 10..14 (entry 0 is the lattice of the step before)
+    ctx.run(tab2, 10, 5, outputs=False)
+    ctx.save_state(1)
+    pb = ctx.download_particles()
+    Ub = [ctx.download_field(fl.UX), ctx.download_field(fl.UY)]
+    ctx.restore_state(0)
+    _, _, _, kind = ctx.run_surrogate(tab2, 10, 5, periods=5, bunching_max=math.inf, max_jumps=1)
+    assert kind[0] == 1
+    pa = ctx.download_particles()
+    Ua = [ctx.download_field(fl.UX), ctx.download_field(fl.UY)]
+    f, p = ctx.jump_residual(1)
+    for pol in range(2):
+        for h in range(nh):
+            a = Ua[pol][h, 1:d.slice_n + 1].reshape(d.slice_n, -1)
+            b = Ub[pol][h, 1:d.slice_n + 1].reshape(d.slice_n, -1)
+            ref = np.stack([(np.conj(a) * b).sum(1).real, (np.conj(a) * b).sum(1).imag,
+                            (np.abs(a) ** 2).sum(1), (np.abs(b) ** 2).sum(1)], 1)
+            got = f[pol * nh + h]
+            for q in range(4):
+                scale = np.max(np.abs(ref[:, q])) + 1e-300
+                assert np.max(np.abs(got[:, q] - ref[:, q])) <= 1e-4 * scale, (pol, h, q)
+    so, sw = deck.beam_layout(d.n_species)
+    P = 2 * math.pi * d.num_slice
+    for sl in range(0, d.slice_n, 7):
+        a, b = so[sl], so[sl + 1]
+        w = sw[sl]
+        ta = pa["theta"][a:b]
+        dth = pb["theta"][a:b] - ta
+        dth -= P * np.floor((dth + P / 2) / P)
+        dgm = pb["gamma"][a:b] - pa["gamma"][a:b]
+        c, s = np.cos(ta), np.sin(ta)
+        ref = w * np.array([len(ta), c.sum(), s.sum(), (c * c).sum(), (s * s).sum(), (c * s).sum(), dth.sum(),
+                            (dth * c).sum(), (dth * s).sum(), dgm.sum(), (dgm * c).sum(), (dgm * s).sum()])
+        assert np.allclose(p[sl], ref, rtol=1e-4, atol=1e-4 * np.max(np.abs(ref))), sl
+    ctx.close()
+
+
+def test_constant_network_gain_multiplies_the_jump_field():
+    """A network with zero weights and output bias (log|g|, arg g) per
+    harmonic (no particle correction): the jump's field is the uncorrected
+    jump's times g."""
+    deck, d, ctx = _ctx()
+    nh = d.n_harm
+    F = 5 + 5 * nh
+    net = fl.MLP([F, 16, 2 * nh + 4], capacity=d.slice_n, seed=3)
+    net.set_layer(0, np.zeros((F, 16), np.float32), np.zeros(16, np.float32))
+    gl = [(0.05 * (h + 1), -0.2 + 0.1 * h) for h in range(nh)]
+    bias = np.array([v for pair in gl for v in pair] + [0, 0, 0, 0], np.float32)
+    net.set_layer(1, np.zeros((16, 2 * nh + 4), np.float32), bias)
+    tab = deck.lattice(0, 15)
+    ctx.run(tab[:21], 0, 10, outputs=False)
+    ctx.save_state(0)
+    tab2 = tab[20:31]
+    _, _, _, k1 = ctx.run_surrogate(tab2, 10, 5, periods=5, bunching_max=math.inf, max_jumps=1)
+    ctx.save_state(1)
+    ctx.restore_state(0)
+    _, _, _, k2 = ctx.run_surrogate(tab2, 10, 5, periods=5, bunching_max=math.inf, max_jumps=1, net=net)
+    assert k1[0] == 1 and k2[0] == 1
+    f, _ = ctx.jump_residual(1)   # a = corrected, b = plain jump: <a,b>/|a|^2 = 1/g
+    for h in range(nh):
+        g = math.exp(gl[h][0]) * complex(math.cos(gl[h][1]), math.sin(gl[h][1]))
+        fx = f[h]
+        live = fx[:, 2] > 1e-6 * fx[:, 2].max()
+        r = (fx[live, 0] + 1j * fx[live, 1]) / fx[live, 2]
+        assert np.max(np.abs(r * g - 1)) < 1e-5, h
+    net.close()
+    ctx.close()
+
+
+def test_constant_network_moves_bunching_and_energy_modulation():
+    """Output bias db/b_1 = rb, dP/P_1 = rp (no field gain): the slices'
+    bunching and energy modulation after the jump move by rb b_1 and rp P_1
+    of the state before it (first order in the small phase and energy
+    changes: 3% of the change)."""
+    deck, d, ctx = _ctx()
+    nh = d.n_harm
+    F = 5 + 5 * nh
+    rb, rp = complex(0.08, -0.05), complex(-0.06, 0.04)
+    net = fl.MLP([F, 16, 2 * nh + 4], capacity=d.slice_n, seed=3)
+    net.set_layer(0, np.zeros((F, 16), np.float32), np.zeros(16, np.float32))
+    bias = np.array([0.0] * (2 * nh) + [rb.real, rb.imag, rp.real, rp.imag], np.float32)
+    net.set_layer(1, np.zeros((16, 2 * nh + 4), np.float32), bias)
+    tab = deck.lattice(0, 45)
+    ctx.run(tab[:81], 0, 40, outputs=False)
+    m0 = ctx.slice_moments()
+    ctx.save_state(0)
+    sub = tab[80:91]
+    ctx.run_surrogate(sub, 40, 5, periods=5, bunching_max=math.inf, max_jumps=1)
+    ma = ctx.slice_moments()
+    ctx.restore_state(0)
+    ctx.run_surrogate(sub, 40, 5, periods=5, bunching_max=math.inf, max_jumps=1, net=net)
+    mb = ctx.slice_moments()
+    w = m0[:, 0]
+    b0 = (m0[:, 5] + 1j * m0[:, 6]) / w
+    P0 = (m0[:, 3] + 1j * m0[:, 4]) / w - (m0[:, 1] / w) * b0
+    db = (mb[:, 5] + 1j * mb[:, 6] - ma[:, 5] - 1j * ma[:, 6]) / w
+    dP = (mb[:, 3] + 1j * mb[:, 4] - ma[:, 3] - 1j * ma[:, 4]) / w
+    core = w > 0.5 * w.max()
+    assert np.max(np.abs(db[core] - rb * b0[core])) < 0.03 * np.max(np.abs(rb * b0[core]))
+    assert np.max(np.abs(dP[core] - rp * P0[core])) < 0.03 * np.max(np.abs(rp * P0[core]))
+    net.close()
+    ctx.close()
+
+
+def test_jumps_follow_the_fine_run():
+    deck, d, c1 = _ctx()
+    nh = d.n_harm
+    N = 100
+    tab = deck.lattice(0, N)
+    e1, _, _ = c1.run(tab, 0, N)
+    _, _, c2 = _ctx()
+    e2, _, _, kind = c2.run_surrogate(tab, 0, N, periods=5, bunching_max=0.05)
+    assert np.sum(kind == 1) >= 10
+    E1, E2 = _energy(e1, nh), _energy(e2, nh)
+    ok = ~np.isnan(E2)
+    assert np.all(kind[~ok] == 2)
+    assert np.max(np.abs(E2[ok] / E1[ok] - 1)) < 0.10
+    c1.close()
+    c2.close()
+
+
+def test_gate_closes_on_bunching():
+    """With a gate below the state's bunching, no period jumps."""
+    deck, d, ctx = _ctx()
+    tab = deck.lattice(0, 20)
+    _, _, _, kind = ctx.run_surrogate(tab, 0, 20, periods=5, bunching_max=0.0)
+    assert np.all(kind == 0)
+    ctx.close()
