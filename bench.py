@@ -147,6 +147,37 @@ def gpu_parity(args, g, cpu):
     return float(np.max(np.abs(tot_g - tot_o) / np.abs(tot_o)))
 
 
+def surrogate_runs(args, periods: int = 10, gate: float = 0.05):
+    """The whole run of the deck (every period of the undulator) three ways
+    from the same initial state: the batched step; jumps of `periods` periods
+    while |b1| < gate; the same with the committed correction network. Wall
+    times of the step loop and the relative error of the total field energy
+    (the integrated FEL power) against the batched run."""
+    from felis_amd import _lib as fl
+    from felis_amd import surrogate as su
+    deck = ROOT / "felis_amd" / "decks" / f"{args.config}.inp"
+    dk = fl.Deck(deck)
+    slices = dk.slice_n
+    dk.close()
+    model = su.load()
+    rf = su.run_curves(deck, seed=args.seed)
+    rj = su.run_curves(deck, seed=args.seed, periods=periods, gate=gate)
+    net = model.network(capacity=slices)
+    rn = su.run_curves(deck, seed=args.seed, periods=periods, gate=gate, net=net)
+    net.close()
+
+    def err(r):
+        ok = ~np.isnan(r["E"])
+        rel = r["E"][ok] / rf["E"][ok] - 1
+        return {"seconds": r["seconds"], "speedup_of_batched": rf["seconds"] / r["seconds"],
+                "jumps": int(np.sum(r["kind"] == 1)), "periods_in_jumps": int(np.sum(r["kind"] > 0)),
+                "final_power_rel_err": float(r["E"][-1] / rf["E"][-1] - 1), "max_power_rel_err": float(
+                    np.max(np.abs(rel)))}
+    return {"periods_per_jump": periods, "bunching_gate": gate, "periods": int(rf["steps"]),
+            "batched_seconds": rf["seconds"], "jump": err(rj), "jump_learned": err(rn),
+            "model": "felis_amd/models/jump_corr_J10.npz", "model_meta": model.meta}
+
+
 def calibrate_copy(mib: int):
     """A plain device-to-device copy of known size, for scaling the HBM byte
     counters (FETCH_SIZE, WRITE_SIZE) of a profiler run."""
@@ -173,6 +204,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=10,
                     help="periods of the plain FP64 path timed beside a mixed run (0: skip)")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the whole-run comparison of multi-period jumps (with and without the learned "
+                         "correction) against the batched step")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB three times (profiler counter scale)")
     args = ap.parse_args()
@@ -224,6 +258,8 @@ def main():
         out["plain_fp64"] = {"ms_per_step": plain_ms, "steps": args.plain_steps,
                              "value": d.slice_n * args.plain_steps / gp["seconds"],
                              "speedup_of_batched": plain_ms / out["ms_per_step"]}
+    if mixed and not args.no_surrogate and g["deck"].mode == 2:
+        out["surrogate"] = surrogate_runs(args)
     if not args.no_cpu:
         cpu = run_cpu(args, g)
         out["cpu_baseline"] = {"value": d.slice_n * args.cpu_steps / cpu["seconds"], "unit": "slice·periods/s",

@@ -34,7 +34,7 @@ EXPORTS = [
     "felis_mlp_forward", "felis_mlp_train_step", "felis_mlp_get_grad", "felis_gemm_bf16",
     "felis_run_surrogate", "felis_state_save", "felis_state_restore", "felis_slice_moments",
     "felis_jump_residual", "felis_slice_features", "felis_mlp_shape", "felis_mlp_set_input_affine",
-    "felis_mlp_get_input_affine",
+    "felis_mlp_get_input_affine", "felis_jump_emission_residual",
 ]
 
 
@@ -81,7 +81,8 @@ class Lattice(C.Structure):
 
 class JumpOpts(C.Structure):
     _fields_ = [("periods", C.c_int), ("max_jumps", C.c_int), ("bunching_max", C.c_double),
-                ("spread_max", C.c_double), ("net", C.c_void_p)]
+                ("spread_max", C.c_double), ("net", C.c_void_p), ("net_bunching_max", C.c_double),
+                ("pad", C.c_int * 2)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -122,6 +123,7 @@ def load() -> C.CDLL:
         L.felis_state_restore.argtypes = [C.c_void_p, C.c_int]
         L.felis_slice_moments.argtypes = [C.c_void_p, _dp]
         L.felis_jump_residual.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.felis_jump_emission_residual.argtypes = [C.c_void_p, C.c_int, _dp]
         L.felis_slice_features.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
         L.felis_mlp_shape.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.felis_mlp_set_input_affine.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]
@@ -302,10 +304,12 @@ class Context:
         return e, b, t
 
     def run_surrogate(self, lat_tab, it0: int, n_steps: int, periods: int, bunching_max: float = 0.05,
-                      max_jumps: int = -1, wake_step: int = 0, wakeF=None, net=None, spread_max: float = 0.0):
+                      max_jumps: int = -1, wake_step: int = 0, wakeF=None, net=None, spread_max: float = 0.0,
+                      net_bunching_max: float = 0.0):
         """felis_run with multi-period jumps while |b1| < bunching_max (and
         the rms relative energy spread < spread_max when that is > 0); net is
-        the learned field correction (an MLP) or None.
+        the learned emission correction (an MLP) or None, applied while
+        |b1| < net_bunching_max (<= 0: always).
         Returns energy, bfac, twiss (NaN rows inside jumps) and the step kinds."""
         d = self.desc
         arr = (Lattice * len(lat_tab))(*lat_tab)
@@ -314,7 +318,7 @@ class Context:
         t = np.zeros((n_steps, 7 * d.n_species))
         kind = np.zeros(max(1, n_steps), np.int32)
         o = JumpOpts(int(periods), int(max_jumps), float(bunching_max), float(spread_max),
-                     net.h if net is not None else None)
+                     net.h if net is not None else None, float(net_bunching_max), (C.c_int * 2)(0, 0))
         wf = None if wakeF is None else np.ascontiguousarray(wakeF, np.float64)
         self._chk(self.L.felis_run_surrogate(self.h, arr, it0, n_steps, wake_step, _p(wf), C.byref(o), _p(e), _p(b),
                                              _p(t), kind.ctypes.data_as(C.POINTER(C.c_int))))
@@ -328,7 +332,7 @@ class Context:
 
     def slice_moments(self):
         d = self.desc
-        out = np.zeros((d.slice_n, 5 + 8 * d.n_harm))
+        out = np.zeros((d.slice_n, 9 + 8 * d.n_harm))
         self._chk(self.L.felis_slice_moments(self.h, _p(out)))
         return out
 
@@ -347,6 +351,15 @@ class Context:
         p = np.zeros((d.slice_n, 12))
         self._chk(self.L.felis_jump_residual(self.h, slot, _p(f), _p(p)))
         return f, p
+
+    def jump_emission_residual(self, slot: int):
+        """After a jump (state a) against snapshot `slot` (b): per (pol x
+        harmonic, slice) sum conj(E)(b - a) (re, im) and sum |E|^2, E the
+        jump's emission into that slice's slot, [2*n_harm, slice_n, 3]."""
+        d = self.desc
+        out = np.zeros((2 * d.n_harm, d.slice_n, 3))
+        self._chk(self.L.felis_jump_emission_residual(self.h, slot, _p(out)))
+        return out
 
     def sync(self):
         self._chk(self.L.felis_sync(self.h))

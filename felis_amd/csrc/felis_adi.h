@@ -227,7 +227,10 @@ __global__ void __launch_bounds__(512)
   const int i0 = 1 + sub * E;
   // Sc is nonzero only inside this slot's deposit window (written by the
   // deposit kernel); cells outside it are neither read nor cleared.
+  // cf = 0: no source; 1: source read and cleared; 2: source read and kept
+  // (a multi-period jump's windowed source, read again by its correction)
   const int4 wn = cf ? win[slot] : make_int4(1, 0, 1, 0);
+  const bool clear_src = cf == 1;
   // ---- x-implicit half step, 4 rows per wave per pass ----
   // Loads are unconditional: out-of-line elements read a clamped in-slot
   // address and their multipliers in the tables are zero; Sc outside the
@@ -293,7 +296,7 @@ __global__ void __launch_bounds__(512)
       const cpx<float> v = y[e] * H + T[o.xaP + k] * (y[e] - up) + T[o.xaM + k] * (y[e] - um) + sc[e];
       if (rowok && i <= nx - 2) {
         dst[e] = yrp * v;
-        if (jwin && i >= wn.x && i <= wn.y) Sn[j * nx + i] = mk(0.f, 0.f);
+        if (clear_src && jwin && i >= wn.x && i <= wn.y) Sn[j * nx + i] = mk(0.f, 0.f);
       }
     }
   }

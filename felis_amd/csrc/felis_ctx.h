@@ -83,6 +83,7 @@ struct felis_ctx {
   double dz_mul = 1.0;
   void *Sw = nullptr, *Ub = nullptr;
   int4 *win2 = nullptr, *box = nullptr;
+  int last_jump = 0;  // periods of the last jump (its windowed source is kept in Sw)
   // device-side copies of the evolving state (felis_state_save/restore):
   // field with its ring position, particles, per-slice wake
   struct Snap {
@@ -158,8 +159,7 @@ void abc_tables(const felis_desc &d, double dz, std::vector<felis::cpx<double>> 
 int dep_args(felis_ctx *ctx, const felis_lattice *lat, felis::DepArgs *out);
 // learned jump correction (felis_sur.hip), enqueued on ctx->stream:
 // moments M and network inputs X of every slice; network outputs Y into the
-// per-system field gain and the per-slice particle correction; sum w,
-// sum w dg, sum w dg^2 over the slices
+// per-system emission factor; sum w, sum w dg, sum w dg^2 over the slices
 int felis_slice_features_dev(felis_ctx *ctx, double *M, float *X);
-int felis_decode_corr_dev(felis_ctx *ctx, const float *Y, const double *M, void *gain, float *corr);
+int felis_decode_corr_dev(felis_ctx *ctx, const float *Y, void *gain);
 int felis_moment_totals_dev(felis_ctx *ctx, const double *M, double *out3);

@@ -131,15 +131,11 @@ static __global__ void __launch_bounds__(256) k_clear_box(int nx, int ny, int S,
   for (int s = s0; s <= s1; ++s) p[(long long)s * N] = mk(0.f, 0.f);
 }
 
-// Ub[ph][phys(s)] = 1/J sum_{i<J, s-i>=1} U[ph][phys(s-i)] / g[ph][s-i] for
-// the chunk's slots, at one cell of the widened beam box; U and Ub share the
-// ring. g is the learned gain the field solve applied to slot s-i (null: 1):
-// the push sees the jump's own field, with which the particles follow the
-// fine steps (the gain corrects only the stored field).
+// Ub[ph][phys(s)] = 1/J sum_{i<J, s-i>=1} U[ph][phys(s-i)] for the chunk's
+// slots, at one cell of the widened beam box; U and Ub share the ring.
 static __global__ void __launch_bounds__(256) k_field_window(int nx, int ny, int S, int slice_n, int ring_base, int J,
                                                       const int4 *__restrict__ bbox,
                                                       const cpx<float> *__restrict__ U,
-                                                      const cpx<float> *__restrict__ gain,
                                                       cpx<float> *__restrict__ Ub) {
   const long long N = (long long)nx * ny;
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -150,12 +146,8 @@ static __global__ void __launch_bounds__(256) k_field_window(int nx, int ny, int
   if (s0 > slice_n) return;
   const int s1 = min(slice_n, s0 + kJumpChunk - 1);
   const cpx<float> *src = U + (long long)blockIdx.y * S * N + c;
-  const cpx<float> *gp = gain ? gain + (long long)blockIdx.y * slice_n - 1 : nullptr;  // gp[s]: slot s
   cpx<float> *dst = Ub + (long long)blockIdx.y * S * N + c;
-  auto at = [&](int s) {
-    const cpx<float> v = src[(long long)phys_slot(s, ring_base, S) * N];
-    return gp ? cdiv(v, gp[s]) : v;
-  };
+  auto at = [&](int s) { return src[(long long)phys_slot(s, ring_base, S) * N]; };
   const double inv = 1.0 / J;
   double sr = 0.0, si = 0.0;
   for (int s = max(1, s0 - J + 1); s < s0; ++s) {
@@ -174,6 +166,35 @@ static __global__ void __launch_bounds__(256) k_field_window(int nx, int ny, int
       sr -= w.re;
       si -= w.im;
     }
+  }
+}
+
+// The learned correction of a jump's emission: the field solve adds the
+// windowed source Sw in both half sweeps, so the jump's emitted field at slot
+// s is E = 2 Sw[s] / H (the source varies slowly on the scale diffraction
+// reaches in one jump). U[ph][phys(s)] += (e - 1) E with e = gain[ph][s]
+// the network's factor, on the beam box (Sw is zero outside it); the field
+// that came into the jump is left as it is.
+static __global__ void __launch_bounds__(256) k_emit_correct(GridArgs g, const int4 *__restrict__ bbox,
+                                                      const cpx<float> *__restrict__ gain,
+                                                      const cpx<float> *__restrict__ Sw,
+                                                      cpx<float> *__restrict__ U) {
+  const long long N = g.N;
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const int i = (int)(c % g.nx), j = (int)(c / g.nx);
+  if (!in_box(*bbox, i, j)) return;
+  const int s0 = 1 + blockIdx.z * kJumpChunk;
+  const int s1 = min(g.slice_n, s0 + kJumpChunk - 1);
+  const int ph = blockIdx.y, h = ph % g.nh;
+  const float k2 = 2.f / (float)g.harm[h];
+  const cpx<float> *ep = gain + (long long)ph * g.slice_n - 1;  // ep[s]: slot s
+  for (int s = s0; s <= s1; ++s) {
+    const cpx<float> e = ep[s];
+    const cpx<float> f = mk((e.re - 1.f) * k2, e.im * k2);
+    const cpx<float> src = Sw[((long long)ph * g.S + s) * N + c];
+    cpx<float> *u = U + ((long long)ph * g.S + phys_slot(s, g.ring_base, g.S)) * N + c;
+    *u = *u + f * src;
   }
 }
 

@@ -265,7 +265,7 @@ __global__ void __launch_bounds__(kTile)
             float *__restrict__ x, float *__restrict__ y, float *__restrict__ px, float *__restrict__ py,
             const double *__restrict__ seg_w, const cpx<float> *__restrict__ Ux, const cpx<float> *__restrict__ Uy,
             const cpx<double> *__restrict__ Ez, const double *__restrict__ bes, const double *__restrict__ wakeE,
-            double *__restrict__ tile_red, unsigned *err, const float *__restrict__ corr) {
+            double *__restrict__ tile_red, unsigned *err) {
   __shared__ double red[(2 * NH + 1 + 7) * kTile];
   const int t = blockIdx.x, k = threadIdx.x;
   const int seg = tl.seg[t];
@@ -312,14 +312,6 @@ __global__ void __launch_bounds__(kTile)
         th += tt * g.dz;
         th -= g.dz * wk;
       }
-    }
-    // learned correction of a multi-period jump (null on every plain step):
-    // per slice, theta and gamma move by c0 + c1 cos(theta) + c2 sin(theta)
-    if (corr) {
-      const float *cp = corr + (long long)sl * 6;
-      const cpx<float> e1 = expi_f32(th);
-      th += (double)(cp[0] + cp[1] * e1.re + cp[2] * e1.im);
-      gm += (double)(cp[3] + cp[4] * e1.re + cp[5] * e1.im);
     }
     // theta wrap (rearrangeParticles.cpp:84-108)
     const double dTh = g.dPhi;

@@ -115,13 +115,13 @@ bool adi_scan_fits(felis_ctx *ctx) {
 }
 
 template <int NH>
-int launch_particles(felis_ctx *ctx, const PStepArgs &a, double *tile_red, void *Ufield, const float *corr) {
+int launch_particles(felis_ctx *ctx, const PStepArgs &a, double *tile_red, void *Ufield) {
   GridArgs g = ctx->grid();
   KTimer kt(ctx, FELIS_K_PARTICLE_STEP);
   hipLaunchKernelGGL((k_pstep<NH>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, g, a, ctx->tiles(), ctx->theta,
                      (float *)ctx->gamma, (float *)ctx->x, (float *)ctx->y, (float *)ctx->px, (float *)ctx->py,
                      ctx->seg_w, (const cpx<float> *)Ufield, (const cpx<float> *)((char *)Ufield + ctx->pol_bytes()),
-                     (const cpx<double> *)ctx->Ez, ctx->bessel, ctx->wakeE, tile_red, ctx->errw, corr);
+                     (const cpx<double> *)ctx->Ez, ctx->bessel, ctx->wakeE, tile_red, ctx->errw);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ctx->err = std::string("particle step launch: ") + hipGetErrorString(e);
@@ -160,8 +160,8 @@ int launch_deposit_slot(felis_ctx *ctx, const DepArgs &dep, int lds) {
     default: return FELIS_ERR_UNSUPPORTED;       \
   }
 
-int particles(felis_ctx *ctx, const PStepArgs &a, double *tile_red, void *Ufield, const float *corr = nullptr) {
-  NH_DISPATCH(launch_particles, ctx->d.n_harm, ctx, a, tile_red, Ufield, corr)
+int particles(felis_ctx *ctx, const PStepArgs &a, double *tile_red, void *Ufield) {
+  NH_DISPATCH(launch_particles, ctx->d.n_harm, ctx, a, tile_red, Ufield)
 }
 int deposit_slot(felis_ctx *ctx, const DepArgs &dep, int lds) {
   NH_DISPATCH(launch_deposit_slot, ctx->d.n_harm, ctx, dep, lds)
@@ -193,8 +193,9 @@ struct JumpCtl {
   int jumps = 0;
   // per-slice moments, network inputs and outputs, per-system gain, totals
   double *M = nullptr, *tot = nullptr;
-  float *X = nullptr, *Y = nullptr, *corr = nullptr;
+  float *X = nullptr, *Y = nullptr;
   cpx<float> *gain = nullptr;
+  bool use_net = false;  // this jump is corrected by the network
 };
 
 // the lattice of steps k..k+J-1 is one undulator section with the field
@@ -318,10 +319,9 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
     }
     // one buffer: moments, totals, inputs, outputs, gain
     const size_t nm = (size_t)d.slice_n * moments_per_slice(nh), nx_ = (size_t)d.slice_n * features_per_slice(nh),
-                 ny_ = (size_t)d.slice_n * outputs_per_slice(nh), ng = (size_t)2 * nh * d.slice_n,
-                 nc = (size_t)d.slice_n * 6;
+                 ny_ = (size_t)d.slice_n * outputs_per_slice(nh), ng = (size_t)2 * nh * d.slice_n;
     if (!e && hipMalloc((void **)&jc->M, (nm + 4) * sizeof(double) + ng * sizeof(cpx<float>) +
-                                             (nx_ + ny_ + nc) * sizeof(float)) != hipSuccess) {
+                                             (nx_ + ny_) * sizeof(float)) != hipSuccess) {
       ctx->err = "jump correction buffers";
       e = FELIS_ERR_HIP;
     }
@@ -330,7 +330,6 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
       jc->gain = (cpx<float> *)(jc->tot + 4);
       jc->X = (float *)(jc->gain + ng);
       jc->Y = jc->X + nx_;
-      jc->corr = jc->Y + ny_;
     }
     if (e) {
       cleanup();
@@ -345,6 +344,7 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
   hipLaunchKernelGGL(k_win_empty, dim3((unsigned)((ctx->S + 255) / 256)), dim3(256), 0, ctx->stream, ctx->win,
                      ctx->S);
   int shifted = 0;  // slots the last iteration slipped the field by
+  bool energy_full = false;  // the field changed after the slice energies were summed
   bool gate = jc != nullptr;
   int err = FELIS_OK;
   for (int k = 0; k < n_steps && !err;) {
@@ -366,6 +366,8 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
       KTimer kt(ctx, FELIS_K_ENERGY);
       hipLaunchKernelGGL(k_reduce_diag, dim3((unsigned)(ne + nbf + 7 * ns)), dim3(256), 0, ctx->stream, g,
                          ctx->slice_e, shifted, tile_red, nred, species_tile, ns, row);
+      if (energy_full && (err = ctx->energy(row))) break;
+      energy_full = false;
     }
     // jump gate: J periods left in one undulator section with no wake update
     // inside, the jump budget not spent, and the fundamental bunching of the
@@ -398,6 +400,7 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
         }
         const double b1 = b[2] > 0 ? std::sqrt(b[0] * b[0] + b[1] * b[1]) / b[2] : 0.0;
         if (!(b1 < o.bunching_max)) gate = false;  // the linear regime has ended: no more jumps
+        jc->use_net = o.net && (o.net_bunching_max <= 0 || b1 < o.net_bunching_max);
         if (o.spread_max > 0 && b[3] > 0) {
           const double mu = b[4] / b[3], var = std::max(b[5] / b[3] - mu * mu, 0.0);
           if (!(std::sqrt(var) / d.gamma_ref < o.spread_max)) gate = false;
@@ -428,24 +431,22 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
         hipLaunchKernelGGL(k_clear_box, dim3(ncb, 2 * nh, nchunk), dim3(256), 0, ctx->stream, d.nx, d.ny, ctx->S,
                            d.slice_n, ctx->box, (cpx<float> *)ctx->Sc);
       }
-      // the learned correction: per-slice inputs -> network -> field gain
-      // and particle correction
-      const cpx<float> *gain = nullptr;
-      if (jc->o.net) {
+      // the learned correction: per-slice inputs -> network -> emission factor
+      if (jc->use_net) {
         if ((err = felis_mlp_forward_dev(jc->o.net, d.slice_n, jc->X, jc->Y, ctx->stream)) ||
-            (err = felis_decode_corr_dev(ctx, jc->Y, jc->M, jc->gain, jc->corr)))
+            (err = felis_decode_corr_dev(ctx, jc->Y, jc->gain)))
           break;
-        gain = jc->gain;
       }
-      if ((err = launch_adi_scan(ctx, jc->tab, 1, (cpx<float> *)ctx->Sw, ctx->win2, gain))) break;
-      // the push field: the J field slots that pass each slice (the jump's
-      // own field, without the learned gain)
+      // the field solve keeps the windowed source: the correction and the
+      // training residual read it after the push
+      if ((err = launch_adi_scan(ctx, jc->tab, 2, (cpx<float> *)ctx->Sw, ctx->win2))) break;
+      // the push field: the J field slots that pass each slice
       hipLaunchKernelGGL(k_win_bbox, dim3(1), dim3(256), 0, ctx->stream, ctx->win, d.slice_n, d.nx, d.ny,
                          kJumpMargin, ctx->box + 1);
       {
         KTimer kt(ctx, FELIS_K_JUMP_WINDOW);
         hipLaunchKernelGGL(k_field_window, dim3(ncb, 2 * nh, nchunk), dim3(256), 0, ctx->stream, d.nx, d.ny,
-                           ctx->S, d.slice_n, gw.ring_base, J, ctx->box + 1, (const cpx<float> *)ctx->U, gain,
+                           ctx->S, d.slice_n, gw.ring_base, J, ctx->box + 1, (const cpx<float> *)ctx->U,
                            (cpx<float> *)ctx->Ub);
       }
     } else if (fused) {
@@ -487,7 +488,7 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
       a.hstep[h] = H - (h ? d.harmony[h - 1] : 0);
     }
     if (jump) ctx->dz_mul = J;
-    err = particles(ctx, a, tile_red, jump ? ctx->Ub : ctx->U, jump && jc->o.net ? jc->corr : nullptr);
+    err = particles(ctx, a, tile_red, jump ? ctx->Ub : ctx->U);
     ctx->dz_mul = 1.0;
     if (err) break;
     const int kn = k + (jump ? J : 1);  // the next iteration
@@ -499,7 +500,16 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
       if ((err = deposit_slot(ctx, dep, dep_lds))) break;
     }
     if (jump) {
+      // the learned emission correction, before the field slips
+      if (jc->use_net) {
+        GridArgs gc = ctx->grid();
+        KTimer kt(ctx, FELIS_K_JUMP_WINDOW);
+        hipLaunchKernelGGL(k_emit_correct, dim3(ncb, 2 * nh, nchunk), dim3(256), 0, ctx->stream, gc, ctx->box,
+                           (const cpx<float> *)jc->gain, (const cpx<float> *)ctx->Sw, (cpx<float> *)ctx->U);
+        energy_full = true;
+      }
       jc->jumps++;
+      ctx->last_jump = J;
       ctx->ring_base = ((ctx->ring_base - J) % ctx->S + ctx->S) % ctx->S;
       KTimer kt(ctx, FELIS_K_SHIFT);
       hipLaunchKernelGGL(k_zero_slots, dim3(ncb, 2 * nh, J), dim3(256), 0, ctx->stream, (cpx<float> *)ctx->U,
@@ -597,7 +607,7 @@ extern "C" int felis_run_surrogate(felis_ctx *ctx, const felis_lattice *lat_tab,
     int in = 0, out = 0, cap = 0;
     felis_mlp_shape(opts->net, &in, &out, &cap);
     if (in != features_per_slice(d.n_harm) || out != outputs_per_slice(d.n_harm) || cap < d.slice_n) {
-      ctx->err = "correction network: inputs 5 + 5 n_harm, outputs 2 n_harm + 4, capacity >= slice_n";
+      ctx->err = "correction network: inputs 5 + 5 n_harm, outputs 2 n_harm, capacity >= slice_n";
       return FELIS_ERR_ARG;
     }
   }

@@ -13,8 +13,9 @@ namespace felis {
 //   0 sum w, 1 sum w dg, 2 sum w dg^2, 3-4 sum w dg e^{i theta},
 //   then per harmonic h (8 each): sum w e^{iH theta}, sum w Ux_h e^{iH theta},
 //   sum w Uy_h e^{iH theta} (re, im each), sum w |Ux_h|^2, sum w |Uy_h|^2
+//   and last sum w Ux_1, sum w Uy_1 of the first harmonic (re, im each)
 //   (dg = gamma - gamma_ref, U bilinear at the particle)
-__host__ __device__ constexpr int moments_per_slice(int nh) { return 5 + 8 * nh; }
+__host__ __device__ constexpr int moments_per_slice(int nh) { return 9 + 8 * nh; }
 // network inputs per slice (k_slice_features): log10 of the slice current in
 // kA, per harmonic log10 of the mean |U|^2 at the particles, log10 |b_h|,
 // the coherent fraction |sum w U e^{iH theta}| / sqrt(sum w sum w |U|^2) and
@@ -48,7 +49,7 @@ static __global__ void __launch_bounds__(256)
                     const double *__restrict__ seg_w, const double *__restrict__ theta,
                     const float *__restrict__ gamma, const float *__restrict__ x, const float *__restrict__ y,
                     const cpx<float> *__restrict__ Ux, const cpx<float> *__restrict__ Uy, double *__restrict__ out) {
-  constexpr int NQ = 5 + 8 * NH;
+  constexpr int NQ = 9 + 8 * NH;
   const int sl = blockIdx.x, slot = sl + 1;
   double acc[NQ];
 #pragma unroll
@@ -91,6 +92,12 @@ static __global__ void __launch_bounds__(256)
           acc[10 + 8 * h] += w * py.im;
           acc[11 + 8 * h] += w * (double)norm2(ux);
           acc[12 + 8 * h] += w * (double)norm2(uy);
+          if (h == 0) {
+            acc[5 + 8 * NH] += w * ux.re;
+            acc[6 + 8 * NH] += w * ux.im;
+            acc[7 + 8 * NH] += w * uy.re;
+            acc[8 + 8 * NH] += w * uy.im;
+          }
         }
       }
     }
@@ -135,25 +142,17 @@ static __global__ void k_slice_features(int slice_n, int nh, double amp_kA, doub
   x[o + 3] = ca > 0 ? (float)(ci / ca) : 0.f;
 }
 
-// network outputs per slice: (log|g_h|, arg g_h) for each harmonic, then
-// the bunching and energy-modulation corrections db / b_1 and dP / P_1
-// (re, im each; b_1 and P_1 of the state before the jump)
-__host__ __device__ constexpr int outputs_per_slice(int nh) { return 2 * nh + 4; }
+// network outputs per slice: (log|e_h|, arg e_h) for each harmonic, the
+// factor on the jump's emission
+__host__ __device__ constexpr int outputs_per_slice(int nh) { return 2 * nh; }
 
-// The network's outputs into the complex field gain of every (pol x
-// harmonic, slice) system and the per-slice particle correction of the
-// jump's push (theta, gamma += c0 + c1 cos theta + c2 sin theta with c0 =
-// c3 = 0): a phase change c1 cos + c2 sin moves the bunching b_1 by
-// (-c2 + i c1) / 2, an energy change c4 cos + c5 sin moves the energy
-// modulation P_1 by (c4 + i c5) / 2. log|g| is held to [-0.5, 0.5], arg g to
-// [-1, 1] rad and the relative particle corrections to modulus 0.5.
+// The network's outputs into the emission factor of every (pol x harmonic,
+// slice) system; log|e| is held to [-0.5, 0.5] and arg e to [-1, 1] rad.
 static __global__ void k_decode_corr(int slice_n, int nh, const float *__restrict__ Y,
-                                     const double *__restrict__ M, cpx<float> *__restrict__ gain,
-                                     float *__restrict__ corr) {
+                                     cpx<float> *__restrict__ gain) {
   const int sl = blockIdx.x * blockDim.x + threadIdx.x;
   if (sl >= slice_n) return;
-  const int O = outputs_per_slice(nh), NQ = moments_per_slice(nh);
-  const float *y = Y + (long long)sl * O;
+  const float *y = Y + (long long)sl * outputs_per_slice(nh);
   for (int h = 0; h < nh; ++h) {
     const float la = fminf(fmaxf(y[2 * h], -0.5f), 0.5f);
     const float ph = fminf(fmaxf(y[2 * h + 1], -1.f), 1.f);
@@ -162,25 +161,6 @@ static __global__ void k_decode_corr(int slice_n, int nh, const float *__restric
     gain[(long long)h * slice_n + sl] = g;
     gain[(long long)(nh + h) * slice_n + sl] = g;
   }
-  const double *m = M + (long long)sl * NQ;
-  const double w = m[0], iw = w > 0 ? 1.0 / w : 0.0;
-  const double mu = m[1] * iw;
-  const cpx<double> b1 = mk(m[5] * iw, m[6] * iw);
-  const cpx<double> P1 = mk(m[3] * iw - mu * b1.re, m[4] * iw - mu * b1.im);
-  auto clampc = [](float re, float im) {
-    const float r = sqrtf(re * re + im * im);
-    const float s = r > 0.5f ? 0.5f / r : 1.f;
-    return mk((double)(re * s), (double)(im * s));
-  };
-  const cpx<double> rb = clampc(y[2 * nh], y[2 * nh + 1]), rp = clampc(y[2 * nh + 2], y[2 * nh + 3]);
-  const cpx<double> db = rb * b1, dP = rp * P1;
-  float *c = corr + (long long)sl * 6;
-  c[0] = 0.f;
-  c[1] = (float)(2.0 * db.im);
-  c[2] = (float)(-2.0 * db.re);
-  c[3] = 0.f;
-  c[4] = (float)(2.0 * dP.re);
-  c[5] = (float)(2.0 * dP.im);
 }
 
 // Per (pol x harmonic, slice) system: sum conj(a) b, sum |a|^2, sum |b|^2
@@ -198,6 +178,32 @@ static __global__ void __launch_bounds__(256) k_field_cmp(GridArgs g, const cpx<
     acc[3] += (double)b.re * b.re + (double)b.im * b.im;
   }
   block_sums_store<4>(acc, out + (long long)sys * 4);
+}
+
+// Per (pol x harmonic, slice) system after a jump of J periods: the jump's
+// emission E = 2 Sw[s - J] / H of the field now at slot s = slice + 1 (Sw is
+// indexed by the slot the field was solved at) against the difference of
+// state b and state a there: sum conj(E) (b - a) (re, im) and sum |E|^2.
+static __global__ void __launch_bounds__(256) k_emit_cmp(GridArgs g, int J, const cpx<float> *__restrict__ A,
+                                                         const cpx<float> *__restrict__ B,
+                                                         const cpx<float> *__restrict__ Sw, double *__restrict__ out) {
+  const int sys = blockIdx.x, sl = sys % g.slice_n, ph = sys / g.slice_n, h = ph % g.nh;
+  const int s = sl + 1, s0 = s - J;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (s0 >= 1) {
+    const long long off = ((long long)ph * g.S + phys_slot(s, g.ring_base, g.S)) * g.N;
+    const cpx<float> *src = Sw + ((long long)ph * g.S + s0) * g.N;
+    const double k2 = 2.0 / g.harm[h];
+    for (long long k = threadIdx.x; k < g.N; k += blockDim.x) {
+      const cpx<float> a = A[off + k], b = B[off + k], e = src[k];
+      const double er = k2 * e.re, ei = k2 * e.im;
+      const double dr = (double)b.re - a.re, di = (double)b.im - a.im;
+      acc[0] += er * dr + ei * di;
+      acc[1] += er * di - ei * dr;
+      acc[2] += er * er + ei * ei;
+    }
+  }
+  block_sums_store<3>(acc, out + (long long)sys * 3);
 }
 
 // Per slice: the least-squares sums of the particle residual (state b minus

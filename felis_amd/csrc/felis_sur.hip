@@ -78,11 +78,10 @@ int felis_slice_features_dev(felis_ctx *ctx, double *M, float *X) {
 }
 
 // the network's outputs [slice_n][outputs_per_slice] into the per-system
-// field gain and the per-slice particle correction (M: the moments the
-// inputs were formed from)
-int felis_decode_corr_dev(felis_ctx *ctx, const float *Y, const double *M, void *gain, float *corr) {
+// emission factor
+int felis_decode_corr_dev(felis_ctx *ctx, const float *Y, void *gain) {
   hipLaunchKernelGGL(k_decode_corr, dim3((unsigned)((ctx->d.slice_n + 127) / 128)), dim3(128), 0, ctx->stream,
-                     ctx->d.slice_n, ctx->d.n_harm, Y, M, (cpx<float> *)gain, corr);
+                     ctx->d.slice_n, ctx->d.n_harm, Y, (cpx<float> *)gain);
   SCHK(hipGetLastError());
   return FELIS_OK;
 }
@@ -219,6 +218,29 @@ int felis_jump_residual(felis_ctx *ctx, int slot, double *field_out, double *par
                        hipSuccess) ||
       hipStreamSynchronize(ctx->stream) != hipSuccess)
     e = fail(ctx, FELIS_ERR_HIP, "jump residual");
+  (void)hipFree(dev);
+  return e;
+}
+
+int felis_jump_emission_residual(felis_ctx *ctx, int slot, double *out) {
+  if (!ctx || slot < 0 || slot > 1 || !out) return FELIS_ERR_ARG;
+  felis_ctx::Snap &sn = ctx->snap[slot];
+  if (!sn.valid || sn.np != ctx->np) return fail(ctx, FELIS_ERR_STATE, "no snapshot of this state in the slot");
+  if (sn.ring_base != ctx->ring_base) return fail(ctx, FELIS_ERR_STATE, "snapshot and state differ in slippage");
+  if (!ctx->last_jump || !ctx->Sw) return fail(ctx, FELIS_ERR_STATE, "no jump has run on this context");
+  const felis_desc &d = ctx->d;
+  const int nsys = 2 * d.n_harm * d.slice_n;
+  double *dev = nullptr;
+  SCHK(hipMalloc((void **)&dev, (size_t)nsys * 3 * sizeof(double)));
+  GridArgs g = ctx->grid();
+  hipLaunchKernelGGL(k_emit_cmp, dim3(nsys), dim3(256), 0, ctx->stream, g, ctx->last_jump,
+                     (const cpx<float> *)ctx->U, (const cpx<float> *)sn.U, (const cpx<float> *)ctx->Sw, dev);
+  int e = FELIS_OK;
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(out, dev, (size_t)nsys * 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess)
+    e = fail(ctx, FELIS_ERR_HIP, "jump emission residual");
   (void)hipFree(dev);
   return e;
 }

@@ -140,33 +140,34 @@ def gain_targets(f: np.ndarray, nh: int, periods: int):
     return T, bb[0], ok.all(0)
 
 
-def particle_targets(p: np.ndarray, m: np.ndarray):
-    """Per-slice particle correction targets from felis_jump_residual particle
-    sums p [slice, 12] and the moments m [slice, 5 + 8 nh] of the state
-    before the jump: the least-squares residuals dtheta, dgamma ~ c0 + c1 cos
-    + c2 sin of the jump's phases give the bunching change db = (-c2 + i c1)/2
-    and the energy-modulation change dP = (c4 + i c5)/2; the targets are
-    db / b_1 and dP / P_1 (re, im each) of the state before the jump, held to
-    modulus 0.5. Returns [slice, 4] and the rows that are defined."""
-    S0, Sc, Ss, Scc, Sss, Scs = (p[:, q] for q in range(6))
-    G = np.stack([np.stack([S0, Sc, Ss], -1), np.stack([Sc, Scc, Scs], -1), np.stack([Ss, Scs, Sss], -1)], -2)
-    ok = (S0 > 0) & (np.abs(np.linalg.det(G)) > 1e-12 * np.maximum(S0, 1e-300) ** 3)
-    Gs = np.where(ok[:, None, None], G, np.eye(3))
-    ct = np.linalg.solve(Gs, p[:, 6:9][..., None])[..., 0]
-    cg = np.linalg.solve(Gs, p[:, 9:12][..., None])[..., 0]
-    db = (-ct[:, 2] + 1j * ct[:, 1]) / 2
-    dP = (cg[:, 1] + 1j * cg[:, 2]) / 2
-    w = np.where(m[:, 0] > 0, m[:, 0], 1.0)
-    b1 = (m[:, 5] + 1j * m[:, 6]) / w
-    P1 = (m[:, 3] + 1j * m[:, 4]) / w - (m[:, 1] / w) * b1
-    ok &= (np.abs(b1) > 0) & (np.abs(P1) > 0)
-    rb = np.where(ok, db / np.where(ok, b1, 1.0), 0.0)
-    rp = np.where(ok, dP / np.where(ok, P1, 1.0), 0.0)
-    for r in (rb, rp):
-        a = np.abs(r)
-        r *= np.where(a > 0.5, 0.5 / np.maximum(a, 1e-300), 1.0)
-    T = np.stack([rb.real, rb.imag, rp.real, rp.imag], 1)
-    return T, ok & np.isfinite(T).all(1)
+def emission_targets(fe: np.ndarray, f: np.ndarray, nh: int, periods: int):
+    """Emission factor per (slice, harmonic) from felis_jump_emission_residual
+    sums fe (both polarisations): the fine steps' field is the jump's plus
+    (e - 1) E, E the jump's emission, with (e - 1) = sum conj(E) (b - a) /
+    sum |E|^2; as [slice, 2 nh] = (log|e|, arg e), plus the fine field energy
+    of the fundamental per slice (f: felis_jump_residual field sums) as the
+    row weight. The sums are taken after the slippage of `periods` slots:
+    row s is the field solved at slot s + 1 (rows whose field has left the
+    window, or that took no emission, are not ok)."""
+    fe = np.asarray(fe)
+    f = np.asarray(f)
+    sn = fe.shape[1]
+    fa = np.zeros_like(fe)
+    ff = np.zeros_like(f)
+    if periods < sn:
+        fa[:, :sn - periods] = fe[:, periods:]
+        ff[:, :sn - periods] = f[:, periods:]
+    num = (fa[:nh, :, 0] + fa[nh:, :, 0]) + 1j * (fa[:nh, :, 1] + fa[nh:, :, 1])
+    den = fa[:nh, :, 2] + fa[nh:, :, 2]
+    ok = den > 0
+    e = np.where(ok, 1.0 + num / np.where(ok, den, 1.0), 1.0)
+    ok &= np.abs(e) > 0
+    e = np.where(ok, e, 1.0)
+    T = np.empty((sn, 2 * nh))
+    T[:, 0::2] = np.log(np.abs(e)).T
+    T[:, 1::2] = np.angle(e).T
+    bb = ff[0, :, 3] + ff[nh, :, 3]
+    return T, bb, ok.all(0)
 
 
 def pairs(deck_path, periods: int, gate: float, seed: int, rows_per_jump: int | None = None, rng=None,
@@ -187,28 +188,34 @@ def pairs(deck_path, periods: int, gate: float, seed: int, rows_per_jump: int | 
     k = 0
     J = periods
     pnet = policy.network(capacity=d.slice_n) if policy is not None else None
+    # the first step runs under the lattice handed in before the loop: fine
+    ctx.run(tab[0:3], 0, 1, outputs=False)
+    k = 1
     while k + J <= n_steps:
         b, tot = ctx.bfactor()
         if not abs(b[0]) / tot < gate:
             break
         X = ctx.slice_features()
-        m = ctx.slice_moments()
         ctx.save_state(0)
         ctx.run(tab[2 * k:2 * (k + J) + 1], k, J, outputs=False)
         ctx.save_state(1)
         ctx.restore_state(0)
-        ctx.run_surrogate(tab[2 * k:2 * (k + J) + 1], k, J, periods=J, bunching_max=math.inf, max_jumps=1)
-        f, pr = ctx.jump_residual(1)
+        _, _, _, kind = ctx.run_surrogate(tab[2 * k:2 * (k + J) + 1], k, J, periods=J, bunching_max=math.inf,
+                                          max_jumps=1)
+        if kind[0] != 1:  # no jump here (the lattice changes within the J periods)
+            ctx.restore_state(1)
+            k += J
+            continue
+        f, _ = ctx.jump_residual(1)
+        fe = ctx.jump_emission_residual(1)
         if pnet is None:
             ctx.restore_state(1)
         else:
             ctx.restore_state(0)
             ctx.run_surrogate(tab[2 * k:2 * (k + J) + 1], k, J, periods=J, bunching_max=math.inf, max_jumps=1,
                               net=pnet)
-        Tf, e_fine, okf = gain_targets(f, nh, J)
-        Tp, okp = particle_targets(pr, m)
-        T = np.concatenate([Tf, Tp], 1)
-        w = np.where(okf & okp & np.isfinite(T).all(1), e_fine, 0.0)
+        T, e_fine, okf = emission_targets(fe, f, nh, J)
+        w = np.where(okf & np.isfinite(T).all(1), e_fine, 0.0)
         if w.sum() > 0:
             w = np.clip(w / w[w > 0].mean(), 0.0, 10.0)
             sel = np.nonzero(w > 0)[0]
@@ -224,7 +231,7 @@ def pairs(deck_path, periods: int, gate: float, seed: int, rows_per_jump: int | 
     deck.close()
     if not Xs:
         F = 5 + 5 * nh
-        return np.zeros((0, F), np.float32), np.zeros((0, 2 * nh + 4), np.float32), np.zeros(0, np.float32)
+        return np.zeros((0, F), np.float32), np.zeros((0, 2 * nh), np.float32), np.zeros(0, np.float32)
     return (np.concatenate(Xs).astype(np.float32), np.concatenate(Ts).astype(np.float32),
             np.concatenate(Ws).astype(np.float32))
 
@@ -345,7 +352,7 @@ def saturation(z, P):
 
 
 def run_curves(deck_path, seed: int, periods: int | None = None, gate: float = 0.05, net=None,
-               n_steps: int | None = None):
+               n_steps: int | None = None, net_gate: float = 0.0):
     """One run of the deck: plain batched (periods None) or with jumps.
     Returns steps, total field energy per step (NaN inside jumps), |b1|,
     kinds and wall seconds of the step loop."""
@@ -362,7 +369,8 @@ def run_curves(deck_path, seed: int, periods: int | None = None, gate: float = 0
         e, b, _ = ctx.run(tab, 0, N)
         kind = np.zeros(N, np.int32)
     else:
-        e, b, _, kind = ctx.run_surrogate(tab, 0, N, periods=periods, bunching_max=gate, net=net)
+        e, b, _, kind = ctx.run_surrogate(tab, 0, N, periods=periods, bunching_max=gate, net=net,
+                                          net_bunching_max=net_gate)
     ctx.sync()
     dt = time.perf_counter() - t0
     E = e[:, :nh].sum(1) + e[:, nh:].sum(1)
@@ -372,3 +380,73 @@ def run_curves(deck_path, seed: int, periods: int | None = None, gate: float = 0
     ctx.close()
     deck.close()
     return dict(E=E, b1=b1, kind=kind, seconds=dt, final=float(fin), steps=N)
+
+
+def single_jump_errors(model: "Model | None", deck_path, seed: int, at_steps, periods: int = 10):
+    """One jump from states of the fine run at each step in at_steps, with and
+    without the network, against the fine steps from the same state: per
+    state the weighted mean |U_jump - U_fine|^2 / |U_fine|^2 of the
+    fundamental (weights: fine field energy per slice)."""
+    deck = fl.Deck(deck_path)
+    d = deck.desc(fl.MIXED_FP32)
+    nh = d.n_harm
+    ctx = fl.Context(d)
+    deck.load(ctx, seed=seed)
+    N = max(at_steps) + periods
+    tab = deck.lattice(0, N)
+    net = model.network(capacity=d.slice_n) if model is not None else None
+    out = []
+    k = 0
+    J = periods
+    for ks in sorted(at_steps):
+        if ks > k:
+            ctx.run(tab[2 * k:2 * ks + 1], k, ks - k, outputs=False)
+            k = ks
+        ctx.save_state(0)
+        ctx.run(tab[2 * k:2 * (k + J) + 1], k, J, outputs=False)
+        ctx.save_state(1)
+        row = dict(step=ks)
+        for tag, nn in (("jump", None), ("jump_net", net)):
+            if tag == "jump_net" and nn is None:
+                continue
+            ctx.restore_state(0)
+            ctx.run_surrogate(tab[2 * k:2 * (k + J) + 1], k, J, periods=J, bunching_max=math.inf, max_jumps=1,
+                              net=nn)
+            f, _ = ctx.jump_residual(1)
+            a2 = f[0, :, 2] + f[nh, :, 2]
+            b2 = f[0, :, 3] + f[nh, :, 3]
+            ab = f[0, :, 0] + f[nh, :, 0]
+            diff = a2 + b2 - 2 * ab          # |a - b|^2 per slice
+            row[tag] = float(np.sum(diff) / np.sum(b2))
+        ctx.restore_state(1)
+        k += J
+        out.append(row)
+    if net is not None:
+        net.close()
+    ctx.close()
+    deck.close()
+    return out
+
+
+def _main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description="train the jump correction network on the GPU")
+    ap.add_argument("--decks", type=int, default=100, help="random decks per round")
+    ap.add_argument("--rounds", type=int, default=2, help="rounds (the first on the fine runs' states)")
+    ap.add_argument("--periods", type=int, default=10)
+    ap.add_argument("--gate", type=float, default=0.05)
+    ap.add_argument("--slices", type=int, default=768)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--out", default=str(DEFAULT_MODEL))
+    a = ap.parse_args(argv)
+    t0 = time.perf_counter()
+    model = fit(n_decks=a.decks, rounds=a.rounds, periods=a.periods, gate=a.gate, seed=a.seed, slices=a.slices,
+                steps=a.steps, log=lambda m: print(m, flush=True))
+    Path(a.out).parent.mkdir(parents=True, exist_ok=True)
+    save(model, a.out)
+    print(f"saved {a.out} after {time.perf_counter() - t0:.1f} s", flush=True)
+
+
+if __name__ == "__main__":
+    _main()

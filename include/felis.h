@@ -165,10 +165,11 @@ typedef struct {
   double spread_max;    /* ... and the rms relative energy spread < spread_max (<= 0: not checked) */
   felis_mlp *net;       /* learned correction (NULL: the jump alone): maps the
                            felis_slice_features of each slice before the jump to
-                           (log|g_h|, arg g_h) per harmonic, the complex gain applied to
-                           the jump's new field, then db/b_1 and dP/P_1 (re, im), the
-                           change of bunching and energy modulation applied to the
-                           slice's particles in the jump's push (2*n_harm + 4 outputs) */
+                           (log|e_h|, arg e_h) per harmonic (2*n_harm outputs), the
+                           complex factor on the jump's emission E = 2 Sw / H: after the
+                           push the field gains (e_h - 1) E */
+  double net_bunching_max; /* the network corrects jumps while |b1| < this (<= 0: always) */
+  int pad[2];
 } felis_jump_opts;
 /* felis_run with jumps (FELIS_MIXED_FP32, Time_Dependent). kind[n_steps]
  * (may be NULL): 0 a fine step, 1 the first period of a jump, 2 a period
@@ -183,11 +184,12 @@ int felis_run_surrogate(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, i
  * jump from one state. */
 int felis_state_save(felis_ctx *ctx, int slot);
 int felis_state_restore(felis_ctx *ctx, int slot);
-/* Per-slice moments of the current state, out[slice_n][5 + 8*n_harm]:
+/* Per-slice moments of the current state, out[slice_n][9 + 8*n_harm]:
  * sum w, sum w dg, sum w dg^2, sum w dg e^{i theta} (re, im), then per
  * harmonic sum w e^{iH theta}, sum w Ux e^{iH theta}, sum w Uy e^{iH theta}
- * (re, im each), sum w |Ux|^2, sum w |Uy|^2; dg = gamma - gamma_ref, U
- * bilinear at the particle. */
+ * (re, im each), sum w |Ux|^2, sum w |Uy|^2, then sum w Ux, sum w Uy of the
+ * first harmonic (re, im each); dg = gamma - gamma_ref, U bilinear at the
+ * particle. */
 int felis_slice_moments(felis_ctx *ctx, double *out);
 /* The correction network's inputs per slice, out[slice_n][5 + 5*n_harm]
  * (FP32): log10 slice current [kA]; per harmonic log10 mean |U|^2 at the
@@ -202,6 +204,11 @@ int felis_slice_features(felis_ctx *ctx, float *out);
  * times 1, c, s, c^2, s^2, cs, dth, dth c, dth s, dgm, dgm c, dgm s with c, s
  * of theta_a, dth = theta_b - theta_a wrapped, dgm = gamma_b - gamma_a. */
 int felis_jump_residual(felis_ctx *ctx, int slot, double *field_out, double *part_out);
+/* After a jump of J periods (the current state a) against snapshot `slot`
+ * (b), same slippage: out[2*n_harm*slice_n][3] = sum conj(E) (b - a) (re,
+ * im) and sum |E|^2 per (pol x harmonic, slice), E = 2 Sw / H the jump's
+ * emission into the field now at the slice's slot (solved J slots back). */
+int felis_jump_emission_residual(felis_ctx *ctx, int slot, double *out);
 
 /* -------- device ------------------------------------------------------------ */
 /* info[8]: compute units, max LDS bytes per workgroup, LDS bytes per CU,

@@ -94,26 +94,22 @@ def test_model_file_round_trip(tmp_path):
         assert np.array_equal(x, y)
 
 
-def test_particle_targets_recover_a_planted_correction():
-    """Residual sums of dtheta = c1 cos + c2 sin, dgamma = c4 cos + c5 sin on
-    quiet-start phases: the targets are db / b_1, dP / P_1 with db = (-c2 +
-    i c1)/2, dP = (c4 + i c5)/2."""
-    n = 64
-    th = 2 * np.pi * (np.arange(n) + 0.5) / n
-    c, s_ = np.cos(th), np.sin(th)
-    c1, c2, c4, c5 = 2e-4, -1e-4, 3e-3, 1e-3
-    dth = c1 * c + c2 * s_
-    dgm = c4 * c + c5 * s_
-    p = np.array([[n, c.sum(), s_.sum(), (c * c).sum(), (s_ * s_).sum(), (c * s_).sum(), dth.sum(),
-                   (dth * c).sum(), (dth * s_).sum(), dgm.sum(), (dgm * c).sum(), (dgm * s_).sum()]])
-    nh = 1
-    m = np.zeros((1, 5 + 8 * nh))
-    m[0, 0] = n
-    b1 = 0.01 * np.exp(0.3j)
-    P1 = 0.05 * np.exp(-1.1j)
-    m[0, 5], m[0, 6] = (n * b1).real, (n * b1).imag
-    m[0, 3], m[0, 4] = (n * P1).real, (n * P1).imag
-    T, ok = su.particle_targets(p, m)
-    assert ok.all()
-    assert np.allclose(T[0, 0] + 1j * T[0, 1], ((-c2 + 1j * c1) / 2) / b1, rtol=1e-9)
-    assert np.allclose(T[0, 2] + 1j * T[0, 3], ((c4 + 1j * c5) / 2) / P1, rtol=1e-9)
+def test_emission_targets_aligned():
+    """b = a + (e - 1) E for a planted e per (slice, harmonic): the targets
+    recover (log|e|, arg e), read J slots after the field was solved."""
+    rng = np.random.default_rng(4)
+    nh, sn, J = 3, 16, 5
+    a = rng.normal(size=(2 * nh, sn, 7)) + 1j * rng.normal(size=(2 * nh, sn, 7))
+    E = rng.normal(size=(2 * nh, sn, 7)) + 1j * rng.normal(size=(2 * nh, sn, 7))
+    e = np.exp(rng.uniform(-0.2, 0.2, (nh, sn)) + 1j * rng.uniform(-0.5, 0.5, (nh, sn)))
+    b = a + (np.concatenate([e, e]) - 1)[:, :, None] * E
+    fe = np.zeros((2 * nh, sn, 3))
+    fe[:, J:, 0] = (np.conj(E) * (b - a)).sum(-1).real[:, :sn - J]
+    fe[:, J:, 1] = (np.conj(E) * (b - a)).sum(-1).imag[:, :sn - J]
+    fe[:, J:, 2] = (np.abs(E) ** 2).sum(-1)[:, :sn - J]
+    f = np.ones((2 * nh, sn, 4))
+    T, w, ok = su.emission_targets(fe, f, nh, J)
+    assert ok[:sn - J].all() and not ok[sn - J:].any()
+    for h in range(nh):
+        assert np.allclose(T[:sn - J, 2 * h], np.log(np.abs(e[h, :sn - J])), atol=1e-12)
+        assert np.allclose(T[:sn - J, 2 * h + 1], np.angle(e[h, :sn - J]), atol=1e-12)
