@@ -37,14 +37,16 @@ WORKLOADS = {
 }
 
 
-def kernel_bytes(name: str, d, np_: int, mixed: bool) -> float:
+def kernel_bytes(name: str, d, np_: int, mixed: bool, npol: int = 2) -> float:
     """Algorithmic HBM bytes of one launch of a kernel: each operand it must
-    read or write, once (the bilinear field gather is counted as cached)."""
+    read or write, once (the bilinear field gather is counted as cached).
+    npol: polarisations the launch solves (a y field that is exactly zero is
+    skipped by the batched step and not counted)."""
     rs = 4 if mixed else 8          # particle coordinate bytes (theta is 8)
     es = 8 if mixed else 16         # complex field element bytes
     cells = d.nx * d.ny
     interior = (d.nx - 2) * (d.ny - 2)
-    nsys = 2 * d.n_harm * d.slice_n
+    nsys = npol * d.n_harm * d.slice_n
     table = {
         "transverse": np_ * 9 * rs,                       # x,y,px,py,gamma in; x,y,px,py out
         "push": np_ * (8 + 5 * rs + 8 + rs),              # theta + 5 coords in; theta, gamma out
@@ -108,7 +110,7 @@ def run_gpu(args, precision=None, steps=None, warmup=None, keep_init=None):
     prof = ctx.profile_read()
     ctx.profile(False)
     return dict(deck=deck, desc=desc, ctx=ctx, seconds=t1 - t0, prof=prof, energy=energy, init=init, tab=tab,
-                wakeF=wakeF, np=ctx.np)
+                wakeF=wakeF, np=ctx.np, npol=ctx.live_polarisations() if precision == "mixed" else 2)
 
 
 def run_cpu(args, g):
@@ -227,7 +229,7 @@ def main():
     dom = max(prof, key=lambda k: prof[k][0])
     ms, n = prof[dom]
     avg_s = ms / n * 1e-3
-    bpl = kernel_bytes(dom, d, g["np"], mixed)
+    bpl = kernel_bytes(dom, d, g["np"], mixed, g["npol"])
     achieved = bpl / avg_s / 1e9 if bpl else 0.0
     out = {
         "metric": METRIC,
@@ -244,7 +246,7 @@ def main():
         "data": "synthetic: quiet-start beam and Gaussian seed generated on the device from the deck",
         "config": {"workload": f"{args.config}: {WORKLOADS[args.config]}", "precision": args.precision,
                    "slices": d.slice_n, "particles": g["np"], "grid": f"{d.nx}x{d.ny}",
-                   "harmonics": [d.harmony[i] for i in range(d.n_harm)]},
+                   "harmonics": [d.harmony[i] for i in range(d.n_harm)], "polarisations_solved": g["npol"]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS,
                      "traffic": measured_traffic(args.config, args.precision, dom), "bytes_per_launch": bpl,

@@ -34,7 +34,7 @@ EXPORTS = [
     "felis_mlp_forward", "felis_mlp_train_step", "felis_mlp_get_grad", "felis_gemm_bf16",
     "felis_run_surrogate", "felis_state_save", "felis_state_restore", "felis_slice_moments",
     "felis_jump_residual", "felis_slice_features", "felis_mlp_shape", "felis_mlp_set_input_affine",
-    "felis_mlp_get_input_affine", "felis_jump_emission_residual",
+    "felis_mlp_get_input_affine", "felis_jump_emission_residual", "felis_live_polarisations",
 ]
 
 
@@ -124,6 +124,7 @@ def load() -> C.CDLL:
         L.felis_slice_moments.argtypes = [C.c_void_p, _dp]
         L.felis_jump_residual.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.felis_jump_emission_residual.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.felis_live_polarisations.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.felis_slice_features.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
         L.felis_mlp_shape.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.felis_mlp_set_input_affine.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]
@@ -360,6 +361,13 @@ class Context:
         out = np.zeros((2 * d.n_harm, d.slice_n, 3))
         self._chk(self.L.felis_jump_emission_residual(self.h, slot, _p(out)))
         return out
+
+    def live_polarisations(self) -> int:
+        """1 while the y field and its source are exactly zero (skipped by the
+        batched step), else 2."""
+        n = C.c_int()
+        self._chk(self.L.felis_live_polarisations(self.h, C.byref(n)))
+        return n.value
 
     def sync(self):
         self._chk(self.L.felis_sync(self.h))

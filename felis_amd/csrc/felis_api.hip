@@ -637,6 +637,11 @@ extern "C" int felis_upload_field(felis_ctx *ctx, int which, const double *data)
   if (which < 0 || which > 3) return FELIS_ERR_ARG;
   const bool isU = which < 2;
   const int pol = which & 1;
+  if (pol == 1) {
+    const size_t n = 2 * (size_t)d.n_harm * ctx->S * ctx->N;
+    for (size_t k = 0; k < n && !ctx->uy_live; ++k)
+      if (data[k] != 0.0) ctx->uy_live = true;
+  }
   const size_t N = ctx->N, S = ctx->S, es = ctx->esize();
   char *base = (char *)(isU ? ctx->U : ctx->Sc);
   std::vector<char> tmp(N * es);
@@ -667,6 +672,11 @@ extern "C" int felis_download_field(felis_ctx *ctx, int which, double *data) {
   if (which < 0 || which > 3) return FELIS_ERR_ARG;
   const bool isU = which < 2;
   const int pol = which & 1;
+  if (pol == 1) {
+    const size_t n = 2 * (size_t)d.n_harm * ctx->S * ctx->N;
+    for (size_t k = 0; k < n && !ctx->uy_live; ++k)
+      if (data[k] != 0.0) ctx->uy_live = true;
+  }
   const size_t N = ctx->N, S = ctx->S, es = ctx->esize();
   const char *base = (const char *)(isU ? ctx->U : ctx->Sc);
   std::vector<char> tmp(N * es);
@@ -1065,6 +1075,7 @@ int felis_ctx::update_wake(const double *wakeF_host, double *total_cnt) {
   if (!lat) return fail(ctx, FELIS_ERR_ARG, "null lattice")
 
 extern "C" int felis_solve_field(felis_ctx *ctx, const felis_lattice *lat) {
+  if (ctx && lat) ctx->lat_uy(lat);
   NEED_CTX();
   NEED_LAT();
   int e = ctx->solve_field(lat);

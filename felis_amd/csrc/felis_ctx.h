@@ -80,6 +80,17 @@ struct felis_ctx {
   // while a jump is enqueued; the window passes' buffers are allocated on
   // the first jump and kept (Sw: windowed source, Ub: windowed push field,
   // both shaped like Sc and U; win2: windows of Sw; box: beam box)
+  // Uy may be nonzero. Clear while the y field and its source are exactly
+  // zero (zero at creation, a seed or upload with no y part, every lattice so
+  // far with eta_Y = 0): the batched step then solves, deposits and reads the
+  // x polarisation only. Set for good by a y upload or seed, or by a lattice
+  // with eta_Y != 0 (lat_uy).
+  bool uy_live = false;
+  int npol() const { return uy_live ? 2 : 1; }
+  // eta_Y != 0 couples the y polarisation (particlePush.cpp:228-238)
+  void lat_uy(const felis_lattice *lat) {
+    if (lat->K0_alpha != 1 || lat->ue != 0.0) uy_live = true;
+  }
   double dz_mul = 1.0;
   void *Sw = nullptr, *Ub = nullptr;
   int4 *win2 = nullptr, *box = nullptr;

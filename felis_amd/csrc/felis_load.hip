@@ -331,6 +331,9 @@ extern "C" int felis_deck_load_seed(const felis_deck *D, felis_ctx *ctx) {
   hipError_t he = hipMemcpy(dss, ss.data(), sizeof(SeedSlice) * d.slice_n, hipMemcpyHostToDevice);
   // D.polarity is never parsed, so the factor is sqrt(1+0)/2 (loadSeed.cpp:53)
   const double factor = 0.5;
+  // Uy = amp (-1 + laser_alpha e^{i laser_psi}) is exactly zero only for
+  // laser_alpha = 1, laser_psi = 0
+  if (!(D->laser_alpha == 1.0 && D->laser_psi == 0.0)) ctx->uy_live = true;
   GridArgs g = ctx->grid();
   dim3 grid((unsigned)((ctx->N + 255) / 256), (unsigned)d.slice_n);
   if (he == hipSuccess) {

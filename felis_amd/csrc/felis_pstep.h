@@ -128,7 +128,7 @@ __device__ inline void transverse_f32(const LatArgs &l, float dz, float gam, flo
 // push_theta_gamma_3D of one particle (particlePush.cpp:260-379): the field
 // coupling in FP32, the slippage phase ku - ks/(2 gamma^2) K2, gamma and
 // theta in FP64. One sincos per RK stage; harmonics by complex powers.
-template <int NH>
+template <int NH, bool UY>
 __device__ inline void push_tg_f32(const GridArgs &g, const PStepArgs &a, const LatArgs &l, int slot, double wakeE,
                                    const cpx<float> *__restrict__ Ux, const cpx<float> *__restrict__ Uy,
                                    const cpx<double> *__restrict__ Ez, const double *__restrict__ bes, float x0,
@@ -188,8 +188,9 @@ __device__ inline void push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
     const cpx<float> *b0 = Uy + (long long)h * g.S * g.N + base;
     const cpx<float> ux =
         a0[0] * (wx0 * wy0) + a0[g.nx] * (wx0 * wy1) + a0[1] * (wx1 * wy0) + a0[g.nx + 1] * (wx1 * wy1);
-    const cpx<float> uy =
-        b0[0] * (wx0 * wy0) + b0[g.nx] * (wx0 * wy1) + b0[1] * (wx1 * wy0) + b0[g.nx + 1] * (wx1 * wy1);
+    const cpx<float> uy = UY ? b0[0] * (wx0 * wy0) + b0[g.nx] * (wx0 * wy1) + b0[1] * (wx1 * wy0) +
+                                   b0[g.nx + 1] * (wx1 * wy1)
+                             : mk(0.f, 0.f);
     const int H = g.harm[h];
     sumU2 += (norm2(ux) + norm2(uy)) * a.hu2[h];
     const float xb = xi * a.hdb[h];
@@ -214,7 +215,7 @@ __device__ inline void push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
       fx = (expP * (ka * J1 - J2) + expM * (ka * J2 - J3)) * s;
       fy = (expP * (ka * J1 + J2) + expM * (ka * J2 + J3)) * s;
     }
-    cu[h] = etaX * fx * ux - etaY * fy * uy;
+    cu[h] = UY ? etaX * fx * ux - etaY * fy * uy : etaX * fx * ux;
   }
   const float sq = (float)(K0 * sqrt(ue * ue + 1.0));
   const float ks_f = (float)ks;
@@ -258,8 +259,9 @@ __device__ inline void push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
   gam = gam0 + (dz6 * sg - dz * wakeE);
 }
 
-// One workgroup of kTile threads per tile, one particle per thread.
-template <int NH>
+// One workgroup of kTile threads per tile, one particle per thread. UY =
+// false: Uy is exactly zero (see k_deposit_slot) and is not read.
+template <int NH, bool UY>
 __global__ void __launch_bounds__(kTile)
     k_pstep(GridArgs g, PStepArgs a, Tiles tl, double *__restrict__ theta, float *__restrict__ gamma,
             float *__restrict__ x, float *__restrict__ y, float *__restrict__ px, float *__restrict__ py,
@@ -287,7 +289,7 @@ __global__ void __launch_bounds__(kTile)
     transverse_f32(a.l0, dzh, gf, xx, yy, ppx, ppy);
     transverse_f32(a.l1, dzh, gf, xx, yy, ppx, ppy);
     if (!a.drift) {
-      push_tg_f32<NH>(g, a, a.l1, slot, wk, Ux, Uy, Ez, bes, xx, yy, ppx, ppy, th, gm, err);
+      push_tg_f32<NH, UY>(g, a, a.l1, slot, wk, Ux, Uy, Ez, bes, xx, yy, ppx, ppy, th, gm, err);
     } else {
       const double fx_ = ((double)xx - g.minX) * a.inv_dx, fy_ = ((double)yy - g.minY) * a.inv_dy;
       const int idxI = (int)fx_, idxJ = (int)fy_;
@@ -303,8 +305,9 @@ __global__ void __launch_bounds__(kTile)
           const cpx<float> *b0 = Uy + (long long)h * g.S * g.N + base;
           const cpx<float> ux =
               a0[0] * (wx0 * wy0) + a0[g.nx] * (wx0 * wy1) + a0[1] * (wx1 * wy0) + a0[g.nx + 1] * (wx1 * wy1);
-          const cpx<float> uy =
-              b0[0] * (wx0 * wy0) + b0[g.nx] * (wx0 * wy1) + b0[1] * (wx1 * wy0) + b0[g.nx + 1] * (wx1 * wy1);
+          const cpx<float> uy = UY ? b0[0] * (wx0 * wy0) + b0[g.nx] * (wx0 * wy1) + b0[1] * (wx1 * wy0) +
+                                         b0[g.nx + 1] * (wx1 * wy1)
+                                   : mk(0.f, 0.f);
           sumU2 += (norm2(ux) + norm2(uy)) * a.hu2[h];
         }
         const double K2 = 1.0 + (double)(ppx * ppx + ppy * ppy);
@@ -367,7 +370,9 @@ __global__ void __launch_bounds__(kTile)
 // stored to the interior of Sc with plain stores: those cells are written by
 // this workgroup only and were cleared by the ADI that read them. Window cells
 // on the boundary ring and the reference's flat-index spill (A.5) use atomics.
-template <int NH>
+// NPOL = 1 deposits the x polarisation only: the y source is exactly zero
+// (eta_Y = 0: a planar undulator with K0_alpha = 1) and Uy is zero.
+template <int NH, int NPOL>
 __global__ void __launch_bounds__(1024)
     k_deposit_slot(GridArgs g, DepArgs d, int n_species, const long long *__restrict__ seg_off,
                    const double *__restrict__ seg_w, const double *__restrict__ theta,
@@ -410,7 +415,7 @@ __global__ void __launch_bounds__(1024)
   const bool empty = bb[1] < 0;
   const int ww = empty ? 0 : bb[1] + 2 - wi0, wh = empty ? 0 : bb[3] + 2 - wj0;
   const int W = ww * wh;
-  int hp = W > 0 ? lds_bytes / (W * 2 * (int)sizeof(cpx<float>)) : NH;
+  int hp = W > 0 ? lds_bytes / (W * NPOL * (int)sizeof(cpx<float>)) : NH;
   if (hp > NH) hp = NH;
   const int step = hp > 0 ? hp : 1;
   // the only interior Sc cells this deposit can make nonzero
@@ -458,7 +463,7 @@ __global__ void __launch_bounds__(1024)
   for (int h0 = 0; h0 < NH; h0 += step) {
     const int hc = min(step, NH - h0);
     if (hp > 0) {
-      for (int q = tid; q < hc * 2 * W; q += nt) win[q] = mk(0.f, 0.f);
+      for (int q = tid; q < hc * NPOL * W; q += nt) win[q] = mk(0.f, 0.f);
       __syncthreads();
     }
     for (int sp = 0; sp < n_species; ++sp) {
@@ -482,19 +487,23 @@ __global__ void __launch_bounds__(1024)
         const cpx<float> vx = acc[c][hh][0], vy = acc[c][hh][1];                                                \
         if (kin) {                                                                                              \
           const int q = (kJ + jj - wj0) * ww + (kI + ii - wi0);                                                 \
-          float *p0 = reinterpret_cast<float *>(lds_d) + 2 * ((hh * 2 + 0) * W + q);                            \
-          float *p1 = reinterpret_cast<float *>(lds_d) + 2 * ((hh * 2 + 1) * W + q);                            \
+          float *p0 = reinterpret_cast<float *>(lds_d) + 2 * ((hh * NPOL + 0) * W + q);                         \
           atomicAdd(p0, vx.re);                                                                                 \
           atomicAdd(p0 + 1, vx.im);                                                                             \
-          atomicAdd(p1, vy.re);                                                                                 \
-          atomicAdd(p1 + 1, vy.im);                                                                             \
+          if (NPOL == 2) {                                                                                      \
+            float *p1 = reinterpret_cast<float *>(lds_d) + 2 * ((hh * NPOL + 1) * W + q);                       \
+            atomicAdd(p1, vy.re);                                                                               \
+            atomicAdd(p1 + 1, vy.im);                                                                           \
+          }                                                                                                     \
         } else {                                                                                                \
           const long long o =                                                                                   \
               (long long)(h0 + hh) * g.S * g.N + (long long)slot * g.N + (long long)kJ * g.nx + kI + jj * g.nx + ii; \
           unsafeAtomicAdd(&Scx[o].re, vx.re);                                                                   \
           unsafeAtomicAdd(&Scx[o].im, vx.im);                                                                   \
-          unsafeAtomicAdd(&Scy[o].re, vy.re);                                                                   \
-          unsafeAtomicAdd(&Scy[o].im, vy.im);                                                                   \
+          if (NPOL == 2) {                                                                                      \
+            unsafeAtomicAdd(&Scy[o].re, vy.re);                                                                 \
+            unsafeAtomicAdd(&Scy[o].im, vy.im);                                                                 \
+          }                                                                                                     \
         }                                                                                                       \
       }                                                                                                         \
     }                                                                                                           \
@@ -559,7 +568,7 @@ __global__ void __launch_bounds__(1024)
           for (int c = 0; c < 4; ++c) {
             const float wgt = -(wx[c >> 1] * wy[c & 1]);
             acc[c][hh][0] = acc[c][hh][0] + cxv * wgt;
-            acc[c][hh][1] = acc[c][hh][1] + cyv * wgt;
+            if (NPOL == 2) acc[c][hh][1] = acc[c][hh][1] + cyv * wgt;
           }
         }
       }
@@ -569,9 +578,9 @@ __global__ void __launch_bounds__(1024)
     }
     if (hp > 0) {
       __syncthreads();
-      for (int q = tid; q < hc * 2 * W; q += nt) {
+      for (int q = tid; q < hc * NPOL * W; q += nt) {
         const cpx<float> v = win[q];
-        const int hh = q / (2 * W), pol = (q / W) & 1, r = q % W;
+        const int hh = q / (NPOL * W), pol = (q / W) % NPOL, r = q % W;
         const int jy = wj0 + r / ww, ix = wi0 + r % ww;
         cpx<float> *dst = (pol ? Scy : Scx) + (long long)(h0 + hh) * g.S * g.N + (long long)slot * g.N +
                           (long long)jy * g.nx + ix;

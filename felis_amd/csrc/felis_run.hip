@@ -67,7 +67,7 @@ int launch_adi_scan(felis_ctx *ctx, const cpx<float> *tab, int cf, cpx<float> *S
                     const cpx<float> *gain = nullptr) {
   GridArgs g = ctx->grid();
   const felis_desc &d = ctx->d;
-  const int nsys = 2 * d.n_harm * d.slice_n;
+  const int nsys = ctx->npol() * d.n_harm * d.slice_n;  // the x systems first: a dead Uy is skipped
   const size_t lds = adi_scan_lds(d.nx, d.ny);
   const int E = adi_scan_E(d.nx, d.ny);
   cpx<float> *U = (cpx<float> *)ctx->U;
@@ -118,10 +118,17 @@ template <int NH>
 int launch_particles(felis_ctx *ctx, const PStepArgs &a, double *tile_red, void *Ufield) {
   GridArgs g = ctx->grid();
   KTimer kt(ctx, FELIS_K_PARTICLE_STEP);
-  hipLaunchKernelGGL((k_pstep<NH>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, g, a, ctx->tiles(), ctx->theta,
-                     (float *)ctx->gamma, (float *)ctx->x, (float *)ctx->y, (float *)ctx->px, (float *)ctx->py,
-                     ctx->seg_w, (const cpx<float> *)Ufield, (const cpx<float> *)((char *)Ufield + ctx->pol_bytes()),
-                     (const cpx<double> *)ctx->Ez, ctx->bessel, ctx->wakeE, tile_red, ctx->errw);
+  const cpx<float> *Ux = (const cpx<float> *)Ufield, *Uy = (const cpx<float> *)((char *)Ufield + ctx->pol_bytes());
+  if (ctx->uy_live)
+    hipLaunchKernelGGL((k_pstep<NH, true>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, g, a, ctx->tiles(),
+                       ctx->theta, (float *)ctx->gamma, (float *)ctx->x, (float *)ctx->y, (float *)ctx->px,
+                       (float *)ctx->py, ctx->seg_w, Ux, Uy, (const cpx<double> *)ctx->Ez, ctx->bessel, ctx->wakeE,
+                       tile_red, ctx->errw);
+  else
+    hipLaunchKernelGGL((k_pstep<NH, false>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, g, a, ctx->tiles(),
+                       ctx->theta, (float *)ctx->gamma, (float *)ctx->x, (float *)ctx->y, (float *)ctx->px,
+                       (float *)ctx->py, ctx->seg_w, Ux, Uy, (const cpx<double> *)ctx->Ez, ctx->bessel, ctx->wakeE,
+                       tile_red, ctx->errw);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ctx->err = std::string("particle step launch: ") + hipGetErrorString(e);
@@ -134,11 +141,17 @@ template <int NH>
 int launch_deposit_slot(felis_ctx *ctx, const DepArgs &dep, int lds) {
   GridArgs g = ctx->grid();
   KTimer kt(ctx, FELIS_K_DEPOSIT);
-  hipLaunchKernelGGL((k_deposit_slot<NH>), dim3(ctx->d.slice_n), dim3(1024), lds, ctx->stream, g, dep,
-                     ctx->d.n_species, ctx->seg_off, ctx->seg_w, ctx->theta, (const float *)ctx->gamma,
-                     (const float *)ctx->x, (const float *)ctx->y, (const float *)ctx->px, (const float *)ctx->py,
-                     ctx->bessel, (cpx<float> *)ctx->Sc, (cpx<float> *)((char *)ctx->Sc + ctx->pol_bytes()),
-                     ctx->ScR, ctx->win, lds, ctx->errw);
+  cpx<float> *Scx = (cpx<float> *)ctx->Sc, *Scy = (cpx<float> *)((char *)ctx->Sc + ctx->pol_bytes());
+  if (ctx->uy_live)
+    hipLaunchKernelGGL((k_deposit_slot<NH, 2>), dim3(ctx->d.slice_n), dim3(1024), lds, ctx->stream, g, dep,
+                       ctx->d.n_species, ctx->seg_off, ctx->seg_w, ctx->theta, (const float *)ctx->gamma,
+                       (const float *)ctx->x, (const float *)ctx->y, (const float *)ctx->px, (const float *)ctx->py,
+                       ctx->bessel, Scx, Scy, ctx->ScR, ctx->win, lds, ctx->errw);
+  else
+    hipLaunchKernelGGL((k_deposit_slot<NH, 1>), dim3(ctx->d.slice_n), dim3(1024), lds, ctx->stream, g, dep,
+                       ctx->d.n_species, ctx->seg_off, ctx->seg_w, ctx->theta, (const float *)ctx->gamma,
+                       (const float *)ctx->x, (const float *)ctx->y, (const float *)ctx->px, (const float *)ctx->py,
+                       ctx->bessel, Scx, Scy, ctx->ScR, ctx->win, lds, ctx->errw);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ctx->err = std::string("deposit launch: ") + hipGetErrorString(e);
@@ -169,7 +182,10 @@ int deposit_slot(felis_ctx *ctx, const DepArgs &dep, int lds) {
 
 template <int NH>
 hipError_t set_deposit_lds(int lds) {
-  return hipFuncSetAttribute((const void *)k_deposit_slot<NH>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  hipError_t e = hipFuncSetAttribute((const void *)k_deposit_slot<NH, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     lds);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute((const void *)k_deposit_slot<NH, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
 }
 hipError_t deposit_lds_attr(int nh, int lds) {
   switch (nh) {
@@ -343,6 +359,10 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
                      (long long)(fsz / sizeof(unsigned)), 0u);
   hipLaunchKernelGGL(k_win_empty, dim3((unsigned)((ctx->S + 255) / 256)), dim3(256), 0, ctx->stream, ctx->win,
                      ctx->S);
+  // the fused ADI sums the slice energies of the systems it solves; those of
+  // a dead y polarisation stay zero
+  hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, ctx->stream, (unsigned *)ctx->slice_e,
+                     (long long)(2 * nh * (size_t)d.slice_n * sizeof(double) / sizeof(unsigned)), 0u);
   int shifted = 0;  // slots the last iteration slipped the field by
   bool energy_full = false;  // the field changed after the slice energies were summed
   bool gate = jc != nullptr;
@@ -358,6 +378,7 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
       // diagnostics of the state handed in and, when the first solve needs one, its deposit
       if ((err = ctx->energy(row)) || (err = ctx->bfactor(row + ne)) || (err = ctx->twiss(row + ne + nbf))) break;
       if (ls->current_flag) {
+        ctx->lat_uy(ls);
         DepArgs dep;
         if ((err = dep_args(ctx, ls, &dep)) || (err = deposit_slot(ctx, dep, dep_lds))) break;
       }
@@ -426,9 +447,9 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
                          d.slice_n, J, ctx->win2);
       {
         KTimer kt(ctx, FELIS_K_JUMP_WINDOW);
-        hipLaunchKernelGGL(k_src_window, dim3(ncb, 2 * nh, nchunk), dim3(256), 0, ctx->stream, d.nx, d.ny, ctx->S,
+        hipLaunchKernelGGL(k_src_window, dim3(ncb, ctx->npol() * nh, nchunk), dim3(256), 0, ctx->stream, d.nx, d.ny, ctx->S,
                            d.slice_n, J, ctx->box, (const cpx<float> *)ctx->Sc, (cpx<float> *)ctx->Sw);
-        hipLaunchKernelGGL(k_clear_box, dim3(ncb, 2 * nh, nchunk), dim3(256), 0, ctx->stream, d.nx, d.ny, ctx->S,
+        hipLaunchKernelGGL(k_clear_box, dim3(ncb, ctx->npol() * nh, nchunk), dim3(256), 0, ctx->stream, d.nx, d.ny, ctx->S,
                            d.slice_n, ctx->box, (cpx<float> *)ctx->Sc);
       }
       // the learned correction: per-slice inputs -> network -> emission factor
@@ -445,7 +466,7 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
                          kJumpMargin, ctx->box + 1);
       {
         KTimer kt(ctx, FELIS_K_JUMP_WINDOW);
-        hipLaunchKernelGGL(k_field_window, dim3(ncb, 2 * nh, nchunk), dim3(256), 0, ctx->stream, d.nx, d.ny,
+        hipLaunchKernelGGL(k_field_window, dim3(ncb, ctx->npol() * nh, nchunk), dim3(256), 0, ctx->stream, d.nx, d.ny,
                            ctx->S, d.slice_n, gw.ring_base, J, ctx->box + 1, (const cpx<float> *)ctx->U,
                            (cpx<float> *)ctx->Ub);
       }
@@ -495,6 +516,7 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
     const felis_lattice *ln = jump ? &lat_tab[2 * kn] : l1;
     // the next solveField's deposit, with the lattice it will run under
     if (ln->current_flag) {
+      ctx->lat_uy(ln);
       DepArgs dep;
       if ((err = dep_args(ctx, ln, &dep))) break;
       if ((err = deposit_slot(ctx, dep, dep_lds))) break;
@@ -504,7 +526,7 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
       if (jc->use_net) {
         GridArgs gc = ctx->grid();
         KTimer kt(ctx, FELIS_K_JUMP_WINDOW);
-        hipLaunchKernelGGL(k_emit_correct, dim3(ncb, 2 * nh, nchunk), dim3(256), 0, ctx->stream, gc, ctx->box,
+        hipLaunchKernelGGL(k_emit_correct, dim3(ncb, ctx->npol() * nh, nchunk), dim3(256), 0, ctx->stream, gc, ctx->box,
                            (const cpx<float> *)jc->gain, (const cpx<float> *)ctx->Sw, (cpx<float> *)ctx->U);
         energy_full = true;
       }
@@ -618,4 +640,10 @@ extern "C" int felis_run_surrogate(felis_ctx *ctx, const felis_lattice *lat_tab,
   const int err = run_common(ctx, lat_tab, it0, n_steps, wake_step, wakeF, energy, bfac, twiss, &jc);
   if (kind) std::copy(kv.begin(), kv.begin() + n_steps, kind);
   return err;
+}
+
+extern "C" int felis_live_polarisations(felis_ctx *ctx, int *npol) {
+  if (!ctx || !npol) return FELIS_ERR_ARG;
+  *npol = ctx->npol();
+  return FELIS_OK;
 }

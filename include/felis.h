@@ -210,6 +210,11 @@ int felis_jump_residual(felis_ctx *ctx, int slot, double *field_out, double *par
  * emission into the field now at the slice's slot (solved J slots back). */
 int felis_jump_emission_residual(felis_ctx *ctx, int slot, double *out);
 
+/* Polarisations the batched step solves (1: the y field and its source are
+ * exactly zero so far and are skipped; 2: both). No counterpart in the
+ * reference, which always solves both. */
+int felis_live_polarisations(felis_ctx *ctx, int *npol);
+
 /* -------- device ------------------------------------------------------------ */
 /* info[8]: compute units, max LDS bytes per workgroup, LDS bytes per CU,
  * wavefront size, L2 bytes, clock kHz, memory clock kHz, memory bus width. */
