@@ -64,8 +64,8 @@ def kernel_bytes(name: str, d, np_: int, mixed: bool, npol: int = 2) -> float:
 
 
 # kernel-timer name -> rocprofv3 kernel name, for the measured HBM traffic
-PROFILED = {("cfg2", "mixed"): {"adi_fused": "felis::k_adi_scan<8>", "particle_step": "felis::k_pstep<3>",
-                                "deposit": "felis::k_deposit_slot<3>"}}
+PROFILED = {("cfg2", "mixed"): {"adi_fused": "felis::k_adi_scan<8>", "particle_step": "felis::k_pstep<3, false>",
+                                "deposit": "felis::k_deposit_slot<3, 1>"}}
 
 
 def measured_traffic(config: str, precision: str, kernel: str):
@@ -77,9 +77,9 @@ def measured_traffic(config: str, precision: str, kernel: str):
     if kernel not in names or not path.exists():
         return None
     for line in path.read_text().splitlines():
-        parts = line.split()
-        if len(parts) == 3 and parts[0] == "traffic" and parts[1] == names[kernel]:
-            return float(parts[2])
+        parts = line.strip().split(" ")
+        if len(parts) >= 3 and parts[0] == "traffic" and " ".join(parts[1:-1]) == names[kernel]:
+            return float(parts[-1])
     return None
 
 

@@ -265,3 +265,29 @@ def test_gate_closes_on_bunching():
     _, _, _, kind = ctx.run_surrogate(tab, 0, 20, periods=5, bunching_max=0.0)
     assert np.all(kind == 0)
     ctx.close()
+
+
+def test_dead_y_polarisation_is_skipped_exactly():
+    """The 64-slice deck (planar undulator, K0_alpha = 1, seed with no y part)
+    solves the x polarisation only; with a y field of 1e-30 in one corner
+    cell both are solved, and the x results agree (the deposit's atomic
+    order: 1e-6)."""
+    deck, d, c1 = _ctx()
+    nh = d.n_harm
+    assert c1.live_polarisations() == 1
+    N = 20
+    tab = deck.lattice(0, N)
+    e1, b1, _ = c1.run(tab, 0, N)
+    assert c1.live_polarisations() == 1
+    _, _, c2 = _ctx()
+    Uy = c2.download_field(fl.UY)
+    assert not np.any(Uy)
+    Uy[0, 1, 0, 0] = 1e-30
+    c2.upload_field(fl.UY, Uy)
+    assert c2.live_polarisations() == 2
+    e2, b2, _ = c2.run(tab, 0, N)
+    assert np.all(np.abs(e2[:, :nh] - e1[:, :nh]) <= 1e-6 * np.abs(e1[:, :nh]))
+    assert np.max(np.abs(b2 - b1)) < 1e-6 * np.max(np.abs(b1))
+    assert np.all(e1[:, nh:] == 0)
+    c1.close()
+    c2.close()
