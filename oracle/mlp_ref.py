@@ -34,36 +34,44 @@ def gelu_d(z):
     return 0.5 * (1 + t) + 0.5 * z * (1 - t * t) * c * (1 + 3 * 0.044715 * z * z)
 
 
-def forward(Ws, bs, X):
-    """Returns the output and the per-layer (H_in BF16, Z) for backward."""
-    H = bf16(X).astype(np.float64)
+def _id(a):
+    return np.asarray(a, np.float64)
+
+
+def forward(Ws, bs, X, rounding=True):
+    """Returns the output and the per-layer (H_in BF16, Z) for backward.
+    rounding=False keeps every operand in FP64 (the exact function whose
+    derivative the gradients restate)."""
+    r = bf16 if rounding else _id
+    H = r(X).astype(np.float64)
     cache = []
     for l, (W, b) in enumerate(zip(Ws, bs)):
-        Z = H @ bf16(W).astype(np.float64) + b
+        Z = H @ r(W).astype(np.float64) + b
         cache.append((H, Z))
         if l < len(Ws) - 1:
-            H = bf16(gelu(Z)).astype(np.float64)
+            H = r(gelu(Z)).astype(np.float64)
         else:
             H = Z
     return H, cache
 
 
-def gradients(Ws, bs, X, T, wrow=None):
+def gradients(Ws, bs, X, T, wrow=None, rounding=True):
     """Mean squared error, dW and db as the kernels form them."""
-    Y, cache = forward(Ws, bs, X)
+    r = bf16 if rounding else _id
+    Y, cache = forward(Ws, bs, X, rounding)
     M, N = Y.shape
     w = np.ones(M) if wrow is None else np.asarray(wrow, np.float64)
     loss = float(np.sum(w[:, None] * (Y - T) ** 2) / (M * N))
     dZf = 2 * w[:, None] * (Y - T) / (M * N)
-    dZ = bf16(dZf).astype(np.float64)
+    dZ = r(dZf).astype(np.float64)
     gWs, gbs = [None] * len(Ws), [None] * len(Ws)
     for l in range(len(Ws) - 1, -1, -1):
         H, Z = cache[l]
         if l < len(Ws) - 1:
             dZf = dH * gelu_d(Z)
-            dZ = bf16(dZf).astype(np.float64)
+            dZ = r(dZf).astype(np.float64)
         gWs[l] = H.T @ dZ
         gbs[l] = dZf.sum(0)
         if l > 0:
-            dH = dZ @ bf16(Ws[l]).astype(np.float64).T
+            dH = dZ @ r(Ws[l]).astype(np.float64).T
     return loss, gWs, gbs
