@@ -91,25 +91,36 @@ def run_gpu(args, precision=None, steps=None, warmup=None, keep_init=None):
     desc = deck.desc(prec)
     ctx = fl.Context(desc)
     deck.load(ctx, seed=args.seed)
+    tuning = None
+    if prec == fl.MIXED_FP32:
+        from felis_amd import tune
+        tuning = tune.apply(ctx, args.tuning)
     init = None
     if (not args.no_cpu) if keep_init is None else keep_init:
         init = dict(particles=ctx.download_particles(), Ux=ctx.download_field(fl.UX), Uy=ctx.download_field(fl.UY))
     W = args.warmup if warmup is None else warmup
     K = args.steps if steps is None else steps
-    tab = deck.lattice(0, W + K)
+    P = args.prof_steps
+    tab = deck.lattice(0, W + K + P)
     wakeF = deck.wake_function()[0] if desc.wake_on else None
     ctx.sync()
     if W:
         ctx.run(tab[0:2 * W + 1], 0, W, deck.wake_step, wakeF)
     ctx.sync()
-    ctx.profile(True)
+    # the timed steps run as they would in use (the batched path with the
+    # autotuned launch shapes); per-kernel HIP-event times come from the next
+    # P steps, launched one kernel at a time
     t0 = time.perf_counter()
     energy, bfac, twiss = ctx.run(tab[2 * W:2 * (W + K) + 1], W, K, deck.wake_step, wakeF)
     ctx.sync()
     t1 = time.perf_counter()
+    ctx.profile(True)
+    ctx.run(tab[2 * (W + K):2 * (W + K + P) + 1], W + K, P, deck.wake_step, wakeF)
+    ctx.sync()
     prof = ctx.profile_read()
     ctx.profile(False)
     return dict(deck=deck, desc=desc, ctx=ctx, seconds=t1 - t0, prof=prof, energy=energy, init=init, tab=tab,
+                tuning=tuning,
                 wakeF=wakeF, np=ctx.np, npol=ctx.live_polarisations() if precision == "mixed" else 2)
 
 
@@ -201,6 +212,8 @@ def main():
     ap.add_argument("--config", default="cfg2", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="mixed", choices=["plain", "mixed"])
     ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--prof-steps", type=int, default=10,
+                    help="steps after the timed ones, each kernel timed with HIP events (roofline)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -209,6 +222,11 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the whole-run comparison of multi-period jumps (with and without the learned "
                          "correction) against the batched step")
+    ap.add_argument("--tuning", default=None, metavar="JSON",
+                    help="autotuner cache the batched step's launch shapes come from (default: "
+                         "felis_amd/tuning.json)")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the autotuner on this workload first and add its result to the --tuning cache")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB three times (profiler counter scale)")
     args = ap.parse_args()
@@ -219,6 +237,10 @@ def main():
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return
+    if args.tune and args.precision == "mixed":
+        from felis_amd import tune
+        e = tune.tune(ROOT / "felis_amd" / "decks" / f"{args.config}.inp", seed=args.seed, verbose=False)
+        tune.write_cache(e, args.tuning)
     g = run_gpu(args)
     d = g["desc"]
     K = args.steps
@@ -246,12 +268,13 @@ def main():
         "data": "synthetic: quiet-start beam and Gaussian seed generated on the device from the deck",
         "config": {"workload": f"{args.config}: {WORKLOADS[args.config]}", "precision": args.precision,
                    "slices": d.slice_n, "particles": g["np"], "grid": f"{d.nx}x{d.ny}",
-                   "harmonics": [d.harmony[i] for i in range(d.n_harm)], "polarisations_solved": g["npol"]},
+                   "harmonics": [d.harmony[i] for i in range(d.n_harm)], "polarisations_solved": g["npol"],
+                   "tuning": g["tuning"]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS,
                      "traffic": measured_traffic(args.config, args.precision, dom), "bytes_per_launch": bpl,
                      "avg_launch_us": avg_s * 1e6},
-        "kernels_ms_per_step": {k: v[0] / K for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])},
+        "kernels_ms_per_step": {k: v[0] / args.prof_steps for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])},
     }
     if mixed and args.plain_steps > 0:
         gp = run_gpu(args, precision="plain", steps=args.plain_steps, warmup=1, keep_init=False)

@@ -35,6 +35,7 @@ EXPORTS = [
     "felis_run_surrogate", "felis_state_save", "felis_state_restore", "felis_slice_moments",
     "felis_jump_residual", "felis_slice_features", "felis_mlp_shape", "felis_mlp_set_input_affine",
     "felis_mlp_get_input_affine", "felis_jump_emission_residual", "felis_live_polarisations",
+    "felis_set_tuning", "felis_get_tuning",
 ]
 
 
@@ -77,6 +78,11 @@ class Lattice(C.Structure):
         ("g", C.c_double), ("lambdaU", C.c_double),
         ("und_type", C.c_int), ("current_flag", C.c_int), ("drift_flag", C.c_int), ("pad", C.c_int),
     ]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("step_graph", C.c_int), ("deposit_lds", C.c_int), ("deposit_threads", C.c_int),
+                ("pad", C.c_int)]
 
 
 class JumpOpts(C.Structure):
@@ -125,6 +131,8 @@ def load() -> C.CDLL:
         L.felis_jump_residual.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.felis_jump_emission_residual.argtypes = [C.c_void_p, C.c_int, _dp]
         L.felis_live_polarisations.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.felis_set_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
+        L.felis_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
         L.felis_slice_features.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
         L.felis_mlp_shape.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.felis_mlp_set_input_affine.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]
@@ -361,6 +369,16 @@ class Context:
         out = np.zeros((2 * d.n_harm, d.slice_n, 3))
         self._chk(self.L.felis_jump_emission_residual(self.h, slot, _p(out)))
         return out
+
+    def set_tuning(self, step_graph: int = 0, deposit_lds: int = 0, deposit_threads: int = 0):
+        """Launch shapes of the batched step (felis_tuning)."""
+        t = Tuning(int(step_graph), int(deposit_lds), int(deposit_threads), 0)
+        self._chk(self.L.felis_set_tuning(self.h, C.byref(t)))
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        self._chk(self.L.felis_get_tuning(self.h, C.byref(t)))
+        return dict(step_graph=t.step_graph, deposit_lds=t.deposit_lds, deposit_threads=t.deposit_threads)
 
     def live_polarisations(self) -> int:
         """1 while the y field and its source are exactly zero (skipped by the

@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(512)
   // multi-period jump); null on every plain step
   const bool has_gain = gain != nullptr;
   const cpx<float> gs = has_gain ? gain[sys] : mk(1.f, 0.f);
-  cpx<float> *Un = U + ((long long)ph * g.S + phys_slot(slot, g.ring_base, g.S)) * g.N;
+  cpx<float> *Un = U + ((long long)ph * g.S + phys_slot(slot, ring_of(g), g.S)) * g.N;
   cpx<float> *Sn = Sc + ((long long)ph * g.S + slot) * g.N;
   const cpx<float> *Th = tab + (long long)h * o.block;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;

@@ -87,6 +87,7 @@ GridArgs felis_ctx::grid() const {
   g.dB = dB;
   g.dPhi = 2 * M_PI * d.num_slice;
   g.gref = mixed() ? d.gamma_ref : 0.0;
+  g.ring_dev = ring_dev_active;
   return g;
 }
 

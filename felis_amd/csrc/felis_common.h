@@ -88,7 +88,12 @@ struct GridArgs {
   int harm[FELIS_MAX_HARM];
   long long N;  // nx*ny
   double dx, dy, dr, minX, minY, dz, ks, dB, dPhi, gref;
+  // when set, the ring position is read from device memory instead of
+  // ring_base: a step captured once as a graph then serves every period
+  const int *ring_dev;
 };
+
+__device__ inline int ring_of(const GridArgs &g) { return g.ring_dev ? *g.ring_dev : g.ring_base; }
 
 struct LatArgs {
   double K0, ue, K0_alpha, ku, g, Phi;

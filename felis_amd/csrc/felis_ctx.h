@@ -105,6 +105,12 @@ struct felis_ctx {
     bool valid = false;
   } snap[2];
 
+  felis_tuning tune{};  // launch shapes (felis_set_tuning); zero: the defaults
+
+  // while a step is captured as a graph: the device word holding the ring
+  // position, which grid() hands to the kernels (null otherwise)
+  const int *ring_dev_active = nullptr;
+
   // batched step (felis_run) resources
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;

@@ -288,12 +288,14 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
 // false: Uy is exactly zero (see k_deposit_slot) and is not read.
 template <int NH, bool UY>
 __global__ void __launch_bounds__(kTile)
-    k_pstep(GridArgs g, PStepArgs a, Tiles tl, double *__restrict__ theta, float *__restrict__ gamma,
+    k_pstep(GridArgs g0, PStepArgs a, Tiles tl, double *__restrict__ theta, float *__restrict__ gamma,
             float *__restrict__ x, float *__restrict__ y, float *__restrict__ px, float *__restrict__ py,
             const double *__restrict__ seg_w, const cpx<float> *__restrict__ Ux, const cpx<float> *__restrict__ Uy,
             const cpx<double> *__restrict__ Ez, const double *__restrict__ bes, const double *__restrict__ wakeE,
             double *__restrict__ tile_red, unsigned *err) {
   __shared__ double red[(2 * NH + 1 + 7) * kTile];
+  GridArgs g = g0;
+  g.ring_base = ring_of(g0);
   const int t = blockIdx.x, k = threadIdx.x;
   const int seg = tl.seg[t];
   const int sl = seg % g.slice_n, slot = sl + 1;
@@ -717,8 +719,10 @@ __global__ void __launch_bounds__(1024)
 // thread), then a fixed tree.
 static __global__ void __launch_bounds__(256)
     k_reduce_diag(GridArgs g, const double *__restrict__ slice_e, int skip, const double *__restrict__ tile_red,
-                  int nred, const int *__restrict__ species_tile, int n_species, double *__restrict__ out) {
+                  int nred, const int *__restrict__ species_tile, int n_species, double *__restrict__ out,
+                  const int *__restrict__ step_dev, int per) {
   __shared__ double sh[256];
+  if (step_dev) out += (long long)(*step_dev) * per;  // a graph-captured step: the row of the current step
   const int o = blockIdx.x, tid = threadIdx.x, nh = g.nh;
   const long long ntiles = species_tile[n_species];
   const double *row;

@@ -142,13 +142,14 @@ int launch_deposit_slot(felis_ctx *ctx, const DepArgs &dep, int lds) {
   GridArgs g = ctx->grid();
   KTimer kt(ctx, FELIS_K_DEPOSIT);
   cpx<float> *Scx = (cpx<float> *)ctx->Sc, *Scy = (cpx<float> *)((char *)ctx->Sc + ctx->pol_bytes());
+  const int nt = ctx->tune.deposit_threads ? ctx->tune.deposit_threads : 1024;
   if (ctx->uy_live)
-    hipLaunchKernelGGL((k_deposit_slot<NH, 2>), dim3(ctx->d.slice_n), dim3(1024), lds, ctx->stream, g, dep,
+    hipLaunchKernelGGL((k_deposit_slot<NH, 2>), dim3(ctx->d.slice_n), dim3(nt), lds, ctx->stream, g, dep,
                        ctx->d.n_species, ctx->seg_off, ctx->seg_w, ctx->theta, (const float *)ctx->gamma,
                        (const float *)ctx->x, (const float *)ctx->y, (const float *)ctx->px, (const float *)ctx->py,
                        ctx->bessel, Scx, Scy, ctx->ScR, ctx->win, lds, ctx->errw);
   else
-    hipLaunchKernelGGL((k_deposit_slot<NH, 1>), dim3(ctx->d.slice_n), dim3(1024), lds, ctx->stream, g, dep,
+    hipLaunchKernelGGL((k_deposit_slot<NH, 1>), dim3(ctx->d.slice_n), dim3(nt), lds, ctx->stream, g, dep,
                        ctx->d.n_species, ctx->seg_off, ctx->seg_w, ctx->theta, (const float *)ctx->gamma,
                        (const float *)ctx->x, (const float *)ctx->y, (const float *)ctx->px, (const float *)ctx->py,
                        ctx->bessel, Scx, Scy, ctx->ScR, ctx->win, lds, ctx->errw);
@@ -259,6 +260,88 @@ int jump_buffers(felis_ctx *ctx) {
   return FELIS_OK;
 }
 
+// Device words of a graph-captured step: the ring position and the step
+// counter that picks the diagnostics row.
+__global__ void k_set_ring_step(int *ring_dev, int *step_dev, int ring, int step) {
+  if (threadIdx.x == 0) {
+    *ring_dev = ring;
+    *step_dev = step;
+  }
+}
+// slippage of one slot with the ring position in device memory: clear the
+// slot that becomes logical 0, then rotate (a second launch, so that no block
+// of the first reads a rotated position)
+__global__ void k_zero_next(cpx<float> *U, const int *ring_dev, int S, long long N, int nph) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int ph = blockIdx.y;
+  const int phys = (*ring_dev - 1 + S) % S;
+  if (k < N && ph < nph) U[((long long)ph * S + phys) * N + k] = mk(0.f, 0.f);
+}
+__global__ void k_ring_rotate(int *ring_dev, int *step_dev, int S) {
+  if (threadIdx.x == 0) {
+    *ring_dev = (*ring_dev - 1 + S) % S;
+    *step_dev += 1;
+  }
+}
+
+// the particle step's arguments for the lattice (it,0) l0 and (it,0.5) l1
+int pstep_args(felis_ctx *ctx, const felis_lattice *l0, const felis_lattice *l1, int nred, PStepArgs *out) {
+  const felis_desc &d = ctx->d;
+  PStepArgs a{};
+  if (l0->und_type != 1 && l0->und_type != 2) {
+    ctx->err = "undulator_type must be BiPolar or QuadPolar (particlePush.cpp:53-68)";
+    return FELIS_ERR_ARG;
+  }
+  if (!l1->drift_flag) {
+    int e = lat_args(ctx, l1, &a.l1);
+    if (e) return e;
+  } else {
+    a.l1.ku = l1->ku;
+  }
+  a.l0.K0 = l0->K0; a.l0.ue = l0->ue; a.l0.K0_alpha = l0->K0_alpha; a.l0.ku = l0->ku; a.l0.g = l0->g;
+  a.l0.und_type = l0->und_type;
+  a.l1.K0 = l1->K0; a.l1.ue = l1->ue; a.l1.K0_alpha = l1->K0_alpha; a.l1.ku = l1->ku; a.l1.g = l1->g;
+  a.l1.und_type = l1->und_type;
+  a.drift = l1->drift_flag;
+  a.nred = nred;
+  a.gamR = d.gamR;
+  a.inv_dx = 1.0 / d.dx;
+  a.inv_dy = 1.0 / d.dy;
+  a.inv_dr = 1.0 / d.dr;
+  for (int h = 0; h < d.n_harm; ++h) {
+    const int H = d.harmony[h];
+    a.hinv[h] = (float)(1.0 / H);
+    a.hdb[h] = (float)(H / ctx->dB);
+    a.hu2[h] = (float)(1.0 / (2.0 * H * H));
+    a.hstep[h] = H - (h ? d.harmony[h - 1] : 0);
+  }
+  *out = a;
+  return FELIS_OK;
+}
+
+bool same_lattice(const felis_lattice &a, const felis_lattice &b) {
+  return a.K0 == b.K0 && a.ue == b.ue && a.K0_alpha == b.K0_alpha && a.ku == b.ku && a.g == b.g &&
+         a.und_type == b.und_type && a.current_flag == b.current_flag && a.drift_flag == b.drift_flag;
+}
+
+// One step of a uniform undulator section with the ring position and the
+// diagnostics row in device memory, captured once and replayed: the
+// diagnostics of the state (slippage of one slot before it), the field solve,
+// the push, the next deposit and the slippage.
+struct StepGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  felis_lattice key{};
+  int npol = 0;
+  bool synced = false;  // the device ring position and step counter are current
+  void release() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+};
+
 // The batched FP32/FP64 loop; with jc, iterations whose gate is open advance
 // J periods at once (felis_jump.h).
 int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps, int wake_step,
@@ -276,9 +359,9 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
     return FELIS_ERR_UNSUPPORTED;
   }
   // the batched deposit adapts to any LDS budget (harmonics in chunks, else
-  // global atomics); FELIS_DEPOSIT_LDS overrides the default for tuning
+  // global atomics); the tuning may give it less than the device allows
   int dep_lds = ctx->lds_max;
-  if (const char *v = std::getenv("FELIS_DEPOSIT_LDS")) dep_lds = std::max(4096, std::min(ctx->lds_max, std::atoi(v)));
+  if (ctx->tune.deposit_lds > 0) dep_lds = std::max(4096, std::min(ctx->lds_max, ctx->tune.deposit_lds));
   if (deposit_lds_attr(d.n_harm, dep_lds) != hipSuccess) {
     ctx->err = "hipFuncSetAttribute(k_deposit_slot)";
     return FELIS_ERR_HIP;
@@ -304,6 +387,7 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
     }
     ctx->dz_mul = 1.0;
   };
+
   if (hipMalloc((void **)&tile_red, (size_t)std::max(1, ctx->ntiles) * nred * sizeof(double)) != hipSuccess ||
       hipMalloc((void **)&species_tile, (ns + 1) * sizeof(int)) != hipSuccess ||
       hipMalloc((void **)&adi_tab, h_tab.size() * sizeof(cpx<float>)) != hipSuccess ||
@@ -319,6 +403,17 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
     ctx->err = "species_tile upload";
     return FELIS_ERR_HIP;
   }
+  // replay one captured step over the uniform stretches of the run when the
+  // tuning asks for it, unless kernels are being timed one by one
+  const bool use_graph = fused && !ctx->profiling && ctx->tune.step_graph;
+  StepGraph sg;
+  int *ring_dev = nullptr;
+  if (use_graph && hipMalloc((void **)&ring_dev, 2 * sizeof(int)) != hipSuccess) {
+    cleanup();
+    ctx->err = "graph step words";
+    return FELIS_ERR_HIP;
+  }
+  int *step_dev = ring_dev ? ring_dev + 1 : nullptr;
   const int J = jc ? jc->o.periods : 1;
   if (jc) {
     // the field solve of a jump: ABC and Thomas tables for a step of J dz
@@ -349,6 +444,7 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
     }
     if (e) {
       cleanup();
+      (void)hipFree(ring_dev);
       return e;
     }
   }
@@ -367,10 +463,77 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
   bool energy_full = false;  // the field changed after the slice energies were summed
   bool gate = jc != nullptr;
   int err = FELIS_OK;
+  const unsigned ncb_all = (unsigned)((ctx->N + 255) / 256);
   for (int k = 0; k < n_steps && !err;) {
     const int it = it0 + k;
     const felis_lattice *ls = &lat_tab[2 * k], *l0 = &lat_tab[2 * k + 1], *l1 = &lat_tab[2 * k + 2];
-    if (d.mode == 2 && wakeF && wake_step > 0 && it % wake_step == 0) {
+    const bool wake_now = d.mode == 2 && wakeF && wake_step > 0 && it % wake_step == 0;
+    // a step of a uniform undulator section after a one-slot slippage, with
+    // no wake update and the jump gate shut: the captured step
+    if (use_graph && k > 0 && shifted == 1 && !energy_full && !wake_now && !gate && d.mode == 2 &&
+        same_lattice(*ls, *l0) && same_lattice(*l0, *l1) && ls->current_flag && !l1->drift_flag) {
+      ctx->lat_uy(l1);
+      if (!sg.exec || !same_lattice(sg.key, *l1) || sg.npol != ctx->npol()) {
+        sg.release();
+        PStepArgs a;
+        DepArgs dep;
+        if ((err = pstep_args(ctx, l0, l1, nred, &a)) || (err = dep_args(ctx, l1, &dep))) break;
+        if (!sg.synced) {
+          hipLaunchKernelGGL(k_set_ring_step, dim3(1), dim3(64), 0, ctx->stream, ring_dev, step_dev,
+                             ctx->ring_base, k);
+          sg.synced = true;
+        }
+        if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+          ctx->err = "hipStreamBeginCapture";
+          err = FELIS_ERR_HIP;
+          break;
+        }
+        ctx->ring_dev_active = ring_dev;
+        GridArgs g = ctx->grid();
+        hipLaunchKernelGGL(k_reduce_diag, dim3((unsigned)(ne + nbf + 7 * ns)), dim3(256), 0, ctx->stream, g,
+                           ctx->slice_e, 1, tile_red, nred, species_tile, ns, dev_out, step_dev, (int)per);
+        int ce = ctx->ez_solve(ls);
+        if (!ce) ce = launch_adi_scan(ctx, adi_tab, 1, (cpx<float> *)ctx->Sc, ctx->win);
+        if (!ce) ce = particles(ctx, a, tile_red, ctx->U);
+        if (!ce) ce = deposit_slot(ctx, dep, dep_lds);
+        if (!ce) {
+          hipLaunchKernelGGL(k_zero_next, dim3(ncb_all, 2 * nh), dim3(256), 0, ctx->stream, (cpx<float> *)ctx->U,
+                             (const int *)ring_dev, ctx->S, (long long)ctx->N, 2 * nh);
+          hipLaunchKernelGGL(k_ring_rotate, dim3(1), dim3(64), 0, ctx->stream, ring_dev, step_dev, ctx->S);
+        }
+        ctx->ring_dev_active = nullptr;
+        hipError_t he = hipStreamEndCapture(ctx->stream, &sg.graph);
+        if (ce) {
+          err = ce;
+          break;
+        }
+        if (he != hipSuccess || hipGraphInstantiate(&sg.exec, sg.graph, nullptr, nullptr, 0) != hipSuccess) {
+          ctx->err = "step graph capture";
+          err = FELIS_ERR_HIP;
+          break;
+        }
+        sg.key = *l1;
+        sg.npol = ctx->npol();
+      }
+      if (!sg.synced) {
+        hipLaunchKernelGGL(k_set_ring_step, dim3(1), dim3(64), 0, ctx->stream, ring_dev, step_dev, ctx->ring_base,
+                           k);
+        sg.synced = true;
+      }
+      if (hipGraphLaunch(sg.exec, ctx->stream) != hipSuccess) {
+        ctx->err = "hipGraphLaunch";
+        err = FELIS_ERR_HIP;
+        break;
+      }
+      ctx->ring_base = (ctx->ring_base - 1 + ctx->S) % ctx->S;
+      if (jc && jc->kind) jc->kind[k] = 0;
+      shifted = 1;
+      if ((k & 31) == 31 || k + 1 >= n_steps) err = ctx->check_err();
+      ++k;
+      continue;
+    }
+    sg.synced = false;  // an eager step moves the ring position on the host
+    if (wake_now) {
       if ((err = ctx->update_wake(wakeF, nullptr))) break;
     }
     double *row = dev_out + (size_t)k * per;
@@ -386,7 +549,7 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
       GridArgs g = ctx->grid();
       KTimer kt(ctx, FELIS_K_ENERGY);
       hipLaunchKernelGGL(k_reduce_diag, dim3((unsigned)(ne + nbf + 7 * ns)), dim3(256), 0, ctx->stream, g,
-                         ctx->slice_e, shifted, tile_red, nred, species_tile, ns, row);
+                         ctx->slice_e, shifted, tile_red, nred, species_tile, ns, row, (const int *)nullptr, 0);
       if (energy_full && (err = ctx->energy(row))) break;
       energy_full = false;
     }
@@ -480,34 +643,8 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
       }
     }
     // particles: transverse x2, theta/gamma or drift, wrap, diagnostic moments
-    PStepArgs a{};
-    if (l0->und_type != 1 && l0->und_type != 2) {
-      ctx->err = "undulator_type must be BiPolar or QuadPolar (particlePush.cpp:53-68)";
-      err = FELIS_ERR_ARG;
-      break;
-    }
-    if (!l1->drift_flag) {
-      if ((err = lat_args(ctx, l1, &a.l1))) break;
-    } else {
-      a.l1.ku = l1->ku;
-    }
-    a.l0.K0 = l0->K0; a.l0.ue = l0->ue; a.l0.K0_alpha = l0->K0_alpha; a.l0.ku = l0->ku; a.l0.g = l0->g;
-    a.l0.und_type = l0->und_type;
-    a.l1.K0 = l1->K0; a.l1.ue = l1->ue; a.l1.K0_alpha = l1->K0_alpha; a.l1.ku = l1->ku; a.l1.g = l1->g;
-    a.l1.und_type = l1->und_type;
-    a.drift = l1->drift_flag;
-    a.nred = nred;
-    a.gamR = d.gamR;
-    a.inv_dx = 1.0 / d.dx;
-    a.inv_dy = 1.0 / d.dy;
-    a.inv_dr = 1.0 / d.dr;
-    for (int h = 0; h < nh; ++h) {
-      const int H = d.harmony[h];
-      a.hinv[h] = (float)(1.0 / H);
-      a.hdb[h] = (float)(H / ctx->dB);
-      a.hu2[h] = (float)(1.0 / (2.0 * H * H));
-      a.hstep[h] = H - (h ? d.harmony[h - 1] : 0);
-    }
+    PStepArgs a;
+    if ((err = pstep_args(ctx, l0, l1, nred, &a))) break;
     if (jump) ctx->dz_mul = J;
     err = particles(ctx, a, tile_red, jump ? ctx->Ub : ctx->U);
     ctx->dz_mul = 1.0;
@@ -545,6 +682,8 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
     k = kn;
   }
   cleanup();
+  sg.release();
+  (void)hipFree(ring_dev);
   return err;
 }
 
@@ -645,5 +784,22 @@ extern "C" int felis_run_surrogate(felis_ctx *ctx, const felis_lattice *lat_tab,
 extern "C" int felis_live_polarisations(felis_ctx *ctx, int *npol) {
   if (!ctx || !npol) return FELIS_ERR_ARG;
   *npol = ctx->npol();
+  return FELIS_OK;
+}
+
+extern "C" int felis_set_tuning(felis_ctx *ctx, const felis_tuning *t) {
+  if (!ctx || !t || t->deposit_lds < 0) return FELIS_ERR_ARG;
+  if (t->deposit_threads != 0 && t->deposit_threads != 256 && t->deposit_threads != 512 &&
+      t->deposit_threads != 1024) {
+    ctx->err = "felis_set_tuning: deposit_threads must be 256, 512 or 1024";
+    return FELIS_ERR_ARG;
+  }
+  ctx->tune = *t;
+  return FELIS_OK;
+}
+
+extern "C" int felis_get_tuning(felis_ctx *ctx, felis_tuning *t) {
+  if (!ctx || !t) return FELIS_ERR_ARG;
+  *t = ctx->tune;
   return FELIS_OK;
 }

@@ -210,6 +210,17 @@ int felis_jump_residual(felis_ctx *ctx, int slot, double *field_out, double *par
  * emission into the field now at the slice's slot (solved J slots back). */
 int felis_jump_emission_residual(felis_ctx *ctx, int slot, double *out);
 
+/* Launch shapes of the batched step, picked per workload by the project's
+ * autotuner (felis_amd/tune.py) from measured step times. */
+typedef struct {
+  int step_graph;   /* 1: replay one captured step over uniform undulator stretches; 0: launch each kernel */
+  int deposit_lds;  /* LDS bytes per deposit workgroup (0: all a workgroup may use) */
+  int deposit_threads; /* threads per deposit workgroup: 256, 512 or 1024 (0: 1024) */
+  int pad;
+} felis_tuning;
+int felis_set_tuning(felis_ctx *ctx, const felis_tuning *t);
+int felis_get_tuning(felis_ctx *ctx, felis_tuning *t);
+
 /* Polarisations the batched step solves (1: the y field and its source are
  * exactly zero so far and are skipped; 2: both). No counterpart in the
  * reference, which always solves both. */
