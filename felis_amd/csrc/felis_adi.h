@@ -195,7 +195,7 @@ __device__ const cpx<float> kZeroCell = {0.f, 0.f};
 template <int E>
 __global__ void __launch_bounds__(512)
     k_adi_scan(GridArgs g, const cpx<float> *__restrict__ tab, int cf, cpx<float> *__restrict__ U,
-               cpx<float> *__restrict__ Sc, const int4 *__restrict__ win, double *__restrict__ slice_e,
+               const cpx<float> *__restrict__ Sc, const int4 *__restrict__ win, double *__restrict__ slice_e,
                const cpx<float> *__restrict__ gain) {
   extern __shared__ double lds_d[];
   __shared__ double red[8];
@@ -212,7 +212,7 @@ __global__ void __launch_bounds__(512)
   const bool has_gain = gain != nullptr;
   const cpx<float> gs = has_gain ? gain[sys] : mk(1.f, 0.f);
   cpx<float> *Un = U + ((long long)ph * g.S + phys_slot(slot, ring_of(g), g.S)) * g.N;
-  cpx<float> *Sn = Sc + ((long long)ph * g.S + slot) * g.N;
+  const cpx<float> *Sn = Sc + ((long long)ph * g.S + slot) * g.N;
   const cpx<float> *Th = tab + (long long)h * o.block;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
   for (int k = tid; k < o.staged; k += blockDim.x) T[k] = Th[k];
@@ -225,12 +225,10 @@ __global__ void __launch_bounds__(512)
   }
   __syncthreads();
   const int i0 = 1 + sub * E;
-  // Sc is nonzero only inside this slot's deposit window (written by the
-  // deposit kernel); cells outside it are neither read nor cleared.
-  // cf = 0: no source; 1: source read and cleared; 2: source read and kept
-  // (a multi-period jump's windowed source, read again by its correction)
+  // The source is read only inside this slot's window: the deposit kernel
+  // rewrote every interior cell of it, and cells outside it hold older
+  // deposits. cf = 0: no source.
   const int4 wn = cf ? win[slot] : make_int4(1, 0, 1, 0);
-  const bool clear_src = cf == 1;
   // ---- x-implicit half step, 4 rows per wave per pass ----
   // Loads are unconditional: out-of-line elements read a clamped in-slot
   // address and their multipliers in the tables are zero; Sc outside the
@@ -285,7 +283,6 @@ __global__ void __launch_bounds__(512)
     // times the y sweep's first multiplier for this row
     const cpx<float> left = mk(dpp_shr<1>(y[E - 1].re), dpp_shr<1>(y[E - 1].im));
     const cpx<float> right = mk(dpp_shl<1>(y[0].re), dpp_shl<1>(y[0].im));
-    const bool jwin = j >= wn.z && j <= wn.w;
     // row j-1 of the layout, columns i0-1 .. i0-1+E-1 are consecutive
     cpx<float> *dst = DY + L.at(j - 1, i0 - 1);
 #pragma unroll
@@ -294,10 +291,7 @@ __global__ void __launch_bounds__(512)
       const cpx<float> um = e == 0 ? left : y[e - 1];
       const cpx<float> up = e == E - 1 ? right : y[e + 1];
       const cpx<float> v = y[e] * H + T[o.xaP + k] * (y[e] - up) + T[o.xaM + k] * (y[e] - um) + sc[e];
-      if (rowok && i <= nx - 2) {
-        dst[e] = yrp * v;
-        if (clear_src && jwin && i >= wn.x && i <= wn.y) Sn[j * nx + i] = mk(0.f, 0.f);
-      }
+      if (rowok && i <= nx - 2) dst[e] = yrp * v;
     }
   }
 #undef FELIS_ADI_LOADX

@@ -15,8 +15,8 @@
 //
 // Both run over slots in chunks with a halo, one thread per (cell, pol x
 // harmonic, chunk), with the running sum in FP64; only the bounding box of
-// the beam's deposit windows is visited (Sc is zero outside the windows,
-// and the push reads the field only around its particles).
+// the beam's deposit windows is visited (the source of a slot is its deposit
+// window, and the push reads the field only around its particles).
 #pragma once
 
 #include "felis_common.h"
@@ -80,10 +80,13 @@ static __global__ void k_win_union(const int4 *__restrict__ win, int slice_n, in
 __device__ inline bool in_box(int4 b, int i, int j) { return i >= b.x && i <= b.y && j >= b.z && j <= b.w; }
 
 // Sw[ph][s] = sum_{i<J, s+i<=slice_n} Sc[ph][s+i] for the chunk's slots, at
-// one cell of the beam box. Sc and Sw are indexed by logical slot.
+// one interior cell of the beam box, where Sc[s] counts only inside the
+// deposit window win[s] (cells outside it hold older deposits); ring cells
+// of Sw are zero. Sc and Sw are indexed by logical slot.
 // grid: x = cells / 256, y = pol x harmonic, z = slot chunks.
 static __global__ void __launch_bounds__(256) k_src_window(int nx, int ny, int S, int slice_n, int J,
                                                     const int4 *__restrict__ bbox,
+                                                    const int4 *__restrict__ win,
                                                     const cpx<float> *__restrict__ Sc,
                                                     cpx<float> *__restrict__ Sw) {
   const long long N = (long long)nx * ny;
@@ -96,39 +99,26 @@ static __global__ void __launch_bounds__(256) k_src_window(int nx, int ny, int S
   const int s1 = min(slice_n, s0 + kJumpChunk - 1);
   const cpx<float> *src = Sc + (long long)blockIdx.y * S * N + c;
   cpx<float> *dst = Sw + (long long)blockIdx.y * S * N + c;
+  const bool interior = i > 0 && i < nx - 1 && j > 0 && j < ny - 1;
+  auto at = [&](int s) { return interior && in_box(win[s], i, j) ? src[(long long)s * N] : mk(0.f, 0.f); };
   double sr = 0.0, si = 0.0;
   for (int s = s0; s < s0 + J - 1 && s <= slice_n; ++s) {
-    const cpx<float> v = src[(long long)s * N];
+    const cpx<float> v = at(s);
     sr += v.re;
     si += v.im;
   }
   for (int s = s0; s <= s1; ++s) {
     const int sa = s + J - 1;
     if (sa <= slice_n) {
-      const cpx<float> v = src[(long long)sa * N];
+      const cpx<float> v = at(sa);
       sr += v.re;
       si += v.im;
     }
     dst[(long long)s * N] = mk((float)sr, (float)si);
-    const cpx<float> v = src[(long long)s * N];
+    const cpx<float> v = at(s);
     sr -= v.re;
     si -= v.im;
   }
-}
-
-// Sc[ph][s] = 0 on the beam box for s = 1..slice_n (after the source window
-// has read it), so that Sc is zero outside the next deposit's windows.
-static __global__ void __launch_bounds__(256) k_clear_box(int nx, int ny, int S, int slice_n, const int4 *__restrict__ bbox,
-                                                   cpx<float> *__restrict__ Sc) {
-  const long long N = (long long)nx * ny;
-  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= N) return;
-  const int i = (int)(c % nx), j = (int)(c / nx);
-  if (!in_box(*bbox, i, j)) return;
-  const int s0 = 1 + blockIdx.z * kJumpChunk;
-  const int s1 = min(slice_n, s0 + kJumpChunk - 1);
-  cpx<float> *p = Sc + (long long)blockIdx.y * S * N + c;
-  for (int s = s0; s <= s1; ++s) p[(long long)s * N] = mk(0.f, 0.f);
 }
 
 // Ub[ph][phys(s)] = 1/J sum_{i<J, s-i>=1} U[ph][phys(s-i)] for the chunk's

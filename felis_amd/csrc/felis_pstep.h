@@ -398,8 +398,12 @@ __global__ void __launch_bounds__(kTile)
 // One workgroup per slice: every species' particles of the slot deposit into
 // an LDS window over their bounding box (harmonics hp at a time), which is then
 // stored to the interior of Sc with plain stores: those cells are written by
-// this workgroup only and were cleared by the ADI that read them. Window cells
-// on the boundary ring and the reference's flat-index spill (A.5) use atomics.
+// this workgroup only, and every interior cell of the window is rewritten, so
+// Sc needs no clearing between steps (the field solve reads the window only).
+// Window cells on the boundary ring and the reference's flat-index spill
+// (A.5) use atomics; ring cells are never read by the batched field solve.
+// A window larger than the LDS budget (hp = 0) is first cleared in Sc and
+// then deposited with global atomics.
 // NPOL = 1 deposits the x polarisation only: the y source is exactly zero
 // (eta_Y = 0: a planar undulator with K0_alpha = 1) and Uy is zero.
 template <int NH, int NPOL>
@@ -490,6 +494,17 @@ __global__ void __launch_bounds__(1024)
   }
   const cpx<float> etaX = mk((float)d.etaXr, (float)d.etaXi), etaY = mk((float)d.etaYr, (float)d.etaYi);
   cpx<float> *win = reinterpret_cast<cpx<float> *>(lds_d);
+  if (hp == 0 && W > 0) {
+    // the window's interior cells start from zero for the atomics below
+    for (int q = tid; q < NH * NPOL * W; q += nt) {
+      const int hh = q / (NPOL * W), pol = (q / W) % NPOL, r = q % W;
+      const int jy = wj0 + r / ww, ix = wi0 + r % ww;
+      if (jy > 0 && jy < g.ny - 1 && ix > 0 && ix < g.nx - 1)
+        ((pol ? Scy : Scx) + (long long)hh * g.S * g.N + (long long)slot * g.N)[(long long)jy * g.nx + ix] =
+            mk(0.f, 0.f);
+    }
+    __syncthreads();
+  }
   for (int h0 = 0; h0 < NH; h0 += step) {
     const int hc = min(step, NH - h0);
     if (hp > 0) {

@@ -63,7 +63,7 @@ int enqueue_step(felis_ctx *ctx, const felis_lattice *lat, double *e_out, double
 
 // The batched ADI over every (slice, harmonic, pol) system with the scan
 // tables of adi_scan_tables (device copy in tab).
-int launch_adi_scan(felis_ctx *ctx, const cpx<float> *tab, int cf, cpx<float> *Sc, const int4 *win,
+int launch_adi_scan(felis_ctx *ctx, const cpx<float> *tab, int cf, const cpx<float> *Sc, const int4 *win,
                     const cpx<float> *gain = nullptr) {
   GridArgs g = ctx->grid();
   const felis_desc &d = ctx->d;
@@ -449,10 +449,12 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
     }
   }
   const size_t ne = 2 * (size_t)nh, nbf = 2 * (size_t)nh + 1;
-  // Sc starts clear; the fused ADI clears every interior cell it reads
+  // the two-pass ADI reads all of Sc: it starts clear and is cleared after
+  // every solve; the fused ADI reads each slot's deposit window only
   const size_t fsz = (size_t)2 * nh * ctx->S * ctx->N * ctx->esize();
-  hipLaunchKernelGGL(k_fill_u32, dim3(4096), dim3(256), 0, ctx->stream, (unsigned *)ctx->Sc,
-                     (long long)(fsz / sizeof(unsigned)), 0u);
+  if (!fused)
+    hipLaunchKernelGGL(k_fill_u32, dim3(4096), dim3(256), 0, ctx->stream, (unsigned *)ctx->Sc,
+                       (long long)(fsz / sizeof(unsigned)), 0u);
   hipLaunchKernelGGL(k_win_empty, dim3((unsigned)((ctx->S + 255) / 256)), dim3(256), 0, ctx->stream, ctx->win,
                      ctx->S);
   // the fused ADI sums the slice energies of the systems it solves; those of
@@ -611,9 +613,7 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
       {
         KTimer kt(ctx, FELIS_K_JUMP_WINDOW);
         hipLaunchKernelGGL(k_src_window, dim3(ncb, ctx->npol() * nh, nchunk), dim3(256), 0, ctx->stream, d.nx, d.ny, ctx->S,
-                           d.slice_n, J, ctx->box, (const cpx<float> *)ctx->Sc, (cpx<float> *)ctx->Sw);
-        hipLaunchKernelGGL(k_clear_box, dim3(ncb, ctx->npol() * nh, nchunk), dim3(256), 0, ctx->stream, d.nx, d.ny, ctx->S,
-                           d.slice_n, ctx->box, (cpx<float> *)ctx->Sc);
+                           d.slice_n, J, ctx->box, ctx->win, (const cpx<float> *)ctx->Sc, (cpx<float> *)ctx->Sw);
       }
       // the learned correction: per-slice inputs -> network -> emission factor
       if (jc->use_net) {
