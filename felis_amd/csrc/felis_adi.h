@@ -189,7 +189,18 @@ __device__ inline cpx<float> scan_bwd(cpx<float> B, const cpx<float> (&C)[4]) {
   return mk(dpp_shl<1>(B.re), dpp_shl<1>(B.im));
 }
 
-__device__ const cpx<float> kZeroCell = {0.f, 0.f};
+// word 3 of a raw buffer resource on gfx9-family GPUs (untyped, range checked:
+// loads past num_records return zero)
+constexpr int kBufWord3 = 0x00020000;
+
+__device__ inline cpx<float> buf_load_c(__amdgpu_buffer_rsrc_t r, int byte_off) {
+  return __builtin_bit_cast(cpx<float>, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));
+}
+template <int E>
+__device__ inline void buf_load_row(__amdgpu_buffer_rsrc_t r, int byte_off, cpx<float> (&q)[E]) {
+#pragma unroll
+  for (int e = 0; e < E; ++e) q[e] = buf_load_c(r, byte_off + e * (int)sizeof(cpx<float>));
+}
 
 // One workgroup (512 threads) per system sys = (pol*nh + h)*slice_n + sl.
 template <int E>
@@ -211,7 +222,10 @@ __global__ void __launch_bounds__(512)
   // multi-period jump); null on every plain step
   const bool has_gain = gain != nullptr;
   const cpx<float> gs = has_gain ? gain[sys] : mk(1.f, 0.f);
-  cpx<float> *Un = U + ((long long)ph * g.S + phys_slot(slot, ring_of(g), g.S)) * g.N;
+  // the ring position is the same for every lane (read from memory when a
+  // captured step replays): keep it scalar
+  const int rb = __builtin_amdgcn_readfirstlane(ring_of(g));
+  cpx<float> *Un = U + ((long long)ph * g.S + phys_slot(slot, rb, g.S)) * g.N;
   const cpx<float> *Sn = Sc + ((long long)ph * g.S + slot) * g.N;
   const cpx<float> *Th = tab + (long long)h * o.block;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
@@ -229,44 +243,60 @@ __global__ void __launch_bounds__(512)
   // rewrote every interior cell of it, and cells outside it hold older
   // deposits. cf = 0: no source.
   const int4 wn = cf ? win[slot] : make_int4(1, 0, 1, 0);
-  // ---- x-implicit half step, 4 rows per wave per pass ----
-  // Loads are unconditional: out-of-line elements read a clamped in-slot
-  // address and their multipliers in the tables are zero; Sc outside the
-  // window reads a zero cell.
+  // ---- x-implicit half step ----
+  // Each 16-lane group walks a band of consecutive rows, so a row's y
+  // stencil needs one new row of U (the other two are the previous row's,
+  // kept in registers) and the next row's loads are in flight while this
+  // row is solved. Loads go through buffer resources over this system's
+  // slot: elements past the row's last unknown read the next cells (their
+  // multipliers in the tables are zero) and rows past the slot read zero.
+  // The source counts only inside the deposit window.
+  const int ngr = nw * 4, gi = wv * 4 + grp;
+  const int R = (nyi + ngr - 1) / ngr;  // rows per band (at most kR)
+  constexpr int kR = (E + 1) / 2;       // 16E >= ny - 2 rows over 32 groups
+  const int jb0 = 1 + gi * R;
+  const int slot_bytes = (int)(g.N * (long long)sizeof(cpx<float>));
+  const __amdgpu_buffer_rsrc_t rU = __builtin_amdgcn_make_buffer_rsrc((void *)Un, 0, slot_bytes, kBufWord3);
+  const __amdgpu_buffer_rsrc_t rS = __builtin_amdgcn_make_buffer_rsrc((void *)Sn, 0, slot_bytes, kBufWord3);
+  const int elo = wn.x - i0, ehi = wn.y - i0;  // this lane's unknowns in the window's columns
+  auto row_off = [&](int j) { return (j * nx + i0) * (int)sizeof(cpx<float>); };
   cpx<float> qm[E], q0[E], qp[E], qs[E];
-#define FELIS_ADI_LOADX(JB)                                                        \
-  {                                                                                \
-    const int jl = 1 + (JB) + wv * 4 + grp;                                        \
-    const int jc = jl <= ny - 2 ? jl : ny - 2;                                     \
-    const bool jw = jl >= wn.z && jl <= wn.w;                                      \
-    _Pragma("unroll") for (int e = 0; e < E; ++e) {                                \
-      const int i = i0 + e;                                                        \
-      const int idx = jc * nx + (i <= nx - 2 ? i : nx - 2);                        \
-      qm[e] = Un[idx - nx];                                                        \
-      q0[e] = Un[idx];                                                             \
-      qp[e] = Un[idx + nx];                                                        \
-      qs[e] = *((jw && i >= wn.x && i <= wn.y) ? Sn + idx : &kZeroCell);           \
-    }                                                                              \
-  }
-  FELIS_ADI_LOADX(0);
-  for (int jb = 0; jb < nyi; jb += nw * 4) {
-    const int j = 1 + jb + wv * 4 + grp;
-    const bool rowok = j <= ny - 2;
+  // outside the window the offset points past the slot: the load reads zero
+  auto load_src = [&](int j, cpx<float>(&q)[E]) {
+    const bool jw = j >= wn.z && j <= wn.w;
+    const int o = row_off(j);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const bool in = jw && e >= elo && e <= ehi;
+      q[e] = buf_load_c(rS, in ? o + e * (int)sizeof(cpx<float>) : slot_bytes);
+    }
+  };
+  buf_load_row<E>(rU, row_off(jb0 - 1), qm);
+  buf_load_row<E>(rU, row_off(jb0), q0);
+  buf_load_row<E>(rU, row_off(jb0 + 1), qp);
+  load_src(jb0, qs);
+#pragma unroll
+  for (int t = 0; t < kR; ++t) {
+    const int j = jb0 + t;
+    const bool rowok = t < R && j <= ny - 2;
+    cpx<float> qn[E], qsn[E];
+    if (t + 1 < kR) {
+      buf_load_row<E>(rU, row_off(j + 2), qn);
+      load_src(j + 1, qsn);
+    }
     const int jj = rowok ? j : 1;
     const cpx<float> bP = T[o.rbP + jj], bM = T[o.rbM + jj], yrp = T[o.ryrp + jj];
     const cpx<float> diag = mk(H + bP.re + bM.re, bP.im + bM.im);
-    cpx<float> y[E], sc[E];
+    cpx<float> y[E];
     cpx<float> B = mk(0.f, 0.f);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int k = e * 16 + sub;
-      sc[e] = qs[e];
       // x-implicit right-hand side: y-explicit stencil of U plus the source
-      cpx<float> dd = diag * q0[e] - bP * qp[e] - bM * qm[e] + sc[e];
+      cpx<float> dd = diag * q0[e] - bP * qp[e] - bM * qm[e] + qs[e];
       B = cfma(T[o.xap + k], B, T[o.xrp + k] * dd);
       y[e] = B;
     }
-    if (jb + nw * 4 < nyi) FELIS_ADI_LOADX(jb + nw * 4);
     const cpx<float> din = scan_fwd(B, Cf);
     B = mk(0.f, 0.f);
 #pragma unroll
@@ -284,17 +314,25 @@ __global__ void __launch_bounds__(512)
     const cpx<float> left = mk(dpp_shr<1>(y[E - 1].re), dpp_shr<1>(y[E - 1].im));
     const cpx<float> right = mk(dpp_shl<1>(y[0].re), dpp_shl<1>(y[0].im));
     // row j-1 of the layout, columns i0-1 .. i0-1+E-1 are consecutive
-    cpx<float> *dst = DY + L.at(j - 1, i0 - 1);
+    cpx<float> *dst = DY + L.at(jj - 1, i0 - 1);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = i0 + e, k = e * 16 + sub;
       const cpx<float> um = e == 0 ? left : y[e - 1];
       const cpx<float> up = e == E - 1 ? right : y[e + 1];
-      const cpx<float> v = y[e] * H + T[o.xaP + k] * (y[e] - up) + T[o.xaM + k] * (y[e] - um) + sc[e];
+      const cpx<float> v = y[e] * H + T[o.xaP + k] * (y[e] - up) + T[o.xaM + k] * (y[e] - um) + qs[e];
       if (rowok && i <= nx - 2) dst[e] = yrp * v;
     }
+    if (t + 1 < kR) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        qm[e] = q0[e];
+        q0[e] = qp[e];
+        qp[e] = qn[e];
+        qs[e] = qsn[e];
+      }
+    }
   }
-#undef FELIS_ADI_LOADX
   __syncthreads();
   // ---- y-implicit half step down the columns, stored straight to U with
   // the slice energy accumulated in FP64 ----
