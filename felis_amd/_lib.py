@@ -82,7 +82,7 @@ class Lattice(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("step_graph", C.c_int), ("deposit_lds", C.c_int), ("deposit_threads", C.c_int),
-                ("pad", C.c_int)]
+                ("adi_threads", C.c_int)]
 
 
 class JumpOpts(C.Structure):
@@ -370,15 +370,16 @@ class Context:
         self._chk(self.L.felis_jump_emission_residual(self.h, slot, _p(out)))
         return out
 
-    def set_tuning(self, step_graph: int = 0, deposit_lds: int = 0, deposit_threads: int = 0):
+    def set_tuning(self, step_graph: int = 0, deposit_lds: int = 0, deposit_threads: int = 0, adi_threads: int = 0):
         """Launch shapes of the batched step (felis_tuning)."""
-        t = Tuning(int(step_graph), int(deposit_lds), int(deposit_threads), 0)
+        t = Tuning(int(step_graph), int(deposit_lds), int(deposit_threads), int(adi_threads))
         self._chk(self.L.felis_set_tuning(self.h, C.byref(t)))
 
     def tuning(self) -> dict:
         t = Tuning()
         self._chk(self.L.felis_get_tuning(self.h, C.byref(t)))
-        return dict(step_graph=t.step_graph, deposit_lds=t.deposit_lds, deposit_threads=t.deposit_threads)
+        return dict(step_graph=t.step_graph, deposit_lds=t.deposit_lds, deposit_threads=t.deposit_threads,
+                    adi_threads=t.adi_threads)
 
     def live_polarisations(self) -> int:
         """1 while the y field and its source are exactly zero (skipped by the

@@ -62,7 +62,7 @@ struct AdiTabOff {
     rbP = 11 * s; rbM = rbP + ny; ryrp = rbM + ny;
     scan = ryrp + ny;
     block = scan + 4 * 4 * 16;
-    staged = scan;  // everything but the scan constants is staged in LDS
+    staged = block;  // the whole block is staged in LDS
   }
 };
 
@@ -173,19 +173,20 @@ __device__ inline cpx<float> cfma(cpx<float> c, cpx<float> x, cpx<float> b) {  /
 }
 
 // forward scan of lane-segment ends: after it, B of lane l is the recurrence
-// value at the end of lane l's segment
-__device__ inline cpx<float> scan_fwd(cpx<float> B, const cpx<float> (&C)[4]) {
+// value at the end of lane l's segment. C points at this lane's constant of
+// level 0; level lv is 16 entries further.
+__device__ inline cpx<float> scan_fwd(cpx<float> B, const cpx<float> *C) {
   B = cfma(C[0], mk(dpp_shr<1>(B.re), dpp_shr<1>(B.im)), B);
-  B = cfma(C[1], mk(dpp_shr<2>(B.re), dpp_shr<2>(B.im)), B);
-  B = cfma(C[2], mk(dpp_shr<4>(B.re), dpp_shr<4>(B.im)), B);
-  B = cfma(C[3], mk(dpp_shr<8>(B.re), dpp_shr<8>(B.im)), B);
+  B = cfma(C[16], mk(dpp_shr<2>(B.re), dpp_shr<2>(B.im)), B);
+  B = cfma(C[32], mk(dpp_shr<4>(B.re), dpp_shr<4>(B.im)), B);
+  B = cfma(C[48], mk(dpp_shr<8>(B.re), dpp_shr<8>(B.im)), B);
   return mk(dpp_shr<1>(B.re), dpp_shr<1>(B.im));  // value entering the lane
 }
-__device__ inline cpx<float> scan_bwd(cpx<float> B, const cpx<float> (&C)[4]) {
+__device__ inline cpx<float> scan_bwd(cpx<float> B, const cpx<float> *C) {
   B = cfma(C[0], mk(dpp_shl<1>(B.re), dpp_shl<1>(B.im)), B);
-  B = cfma(C[1], mk(dpp_shl<2>(B.re), dpp_shl<2>(B.im)), B);
-  B = cfma(C[2], mk(dpp_shl<4>(B.re), dpp_shl<4>(B.im)), B);
-  B = cfma(C[3], mk(dpp_shl<8>(B.re), dpp_shl<8>(B.im)), B);
+  B = cfma(C[16], mk(dpp_shl<2>(B.re), dpp_shl<2>(B.im)), B);
+  B = cfma(C[32], mk(dpp_shl<4>(B.re), dpp_shl<4>(B.im)), B);
+  B = cfma(C[48], mk(dpp_shl<8>(B.re), dpp_shl<8>(B.im)), B);
   return mk(dpp_shl<1>(B.re), dpp_shl<1>(B.im));
 }
 
@@ -202,9 +203,9 @@ __device__ inline void buf_load_row(__amdgpu_buffer_rsrc_t r, int byte_off, cpx<
   for (int e = 0; e < E; ++e) q[e] = buf_load_c(r, byte_off + e * (int)sizeof(cpx<float>));
 }
 
-// One workgroup (512 threads) per system sys = (pol*nh + h)*slice_n + sl.
-template <int E>
-__global__ void __launch_bounds__(512)
+// One workgroup (NT threads: 512 or 1024) per system sys = (pol*nh + h)*slice_n + sl.
+template <int E, int NT>
+__global__ void __launch_bounds__(NT)
     k_adi_scan(GridArgs g, const cpx<float> *__restrict__ tab, int cf, cpx<float> *__restrict__ U,
                const cpx<float> *__restrict__ Sc, const int4 *__restrict__ win, double *__restrict__ slice_e,
                const cpx<float> *__restrict__ gain) {
@@ -231,12 +232,6 @@ __global__ void __launch_bounds__(512)
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
   for (int k = tid; k < o.staged; k += blockDim.x) T[k] = Th[k];
   const int grp = lane >> 4, sub = lane & 15;
-  cpx<float> Cf[4], Cb[4];
-#pragma unroll
-  for (int lv = 0; lv < 4; ++lv) {
-    Cf[lv] = Th[o.scan + lv * 16 + sub];
-    Cb[lv] = Th[o.scan + 64 + lv * 16 + sub];
-  }
   __syncthreads();
   const int i0 = 1 + sub * E;
   // The source is read only inside this slot's window: the deposit kernel
@@ -246,14 +241,14 @@ __global__ void __launch_bounds__(512)
   // ---- x-implicit half step ----
   // Each 16-lane group walks a band of consecutive rows, so a row's y
   // stencil needs one new row of U (the other two are the previous row's,
-  // kept in registers) and the next row's loads are in flight while this
-  // row is solved. Loads go through buffer resources over this system's
+  // kept in registers) and the next row's loads are in flight while the
+  // rest of this row is solved. Loads go through buffer resources over this system's
   // slot: elements past the row's last unknown read the next cells (their
   // multipliers in the tables are zero) and rows past the slot read zero.
   // The source counts only inside the deposit window.
   const int ngr = nw * 4, gi = wv * 4 + grp;
   const int R = (nyi + ngr - 1) / ngr;  // rows per band (at most kR)
-  constexpr int kR = (E + 1) / 2;       // 16E >= ny - 2 rows over 32 groups
+  constexpr int kR = (16 * E + NT / 16 - 1) / (NT / 16);  // 16E >= ny - 2 rows over NT/16 groups
   const int jb0 = 1 + gi * R;
   const int slot_bytes = (int)(g.N * (long long)sizeof(cpx<float>));
   const __amdgpu_buffer_rsrc_t rU = __builtin_amdgcn_make_buffer_rsrc((void *)Un, 0, slot_bytes, kBufWord3);
@@ -279,11 +274,6 @@ __global__ void __launch_bounds__(512)
   for (int t = 0; t < kR; ++t) {
     const int j = jb0 + t;
     const bool rowok = t < R && j <= ny - 2;
-    cpx<float> qn[E], qsn[E];
-    if (t + 1 < kR) {
-      buf_load_row<E>(rU, row_off(j + 2), qn);
-      load_src(j + 1, qsn);
-    }
     const int jj = rowok ? j : 1;
     const cpx<float> bP = T[o.rbP + jj], bM = T[o.rbM + jj], yrp = T[o.ryrp + jj];
     const cpx<float> diag = mk(H + bP.re + bM.re, bP.im + bM.im);
@@ -297,7 +287,15 @@ __global__ void __launch_bounds__(512)
       B = cfma(T[o.xap + k], B, T[o.xrp + k] * dd);
       y[e] = B;
     }
-    const cpx<float> din = scan_fwd(B, Cf);
+    // the next row's loads, into the registers of the row above (dead now);
+    // with 1024 threads the source follows at the end of the row, when this
+    // row's source is dead (registers: 128 per lane at 16 waves per CU)
+    cpx<float> qn[E], qsn[E];
+    if (t + 1 < kR) {
+      buf_load_row<E>(rU, row_off(j + 2), qn);
+      if (NT < 1024) load_src(j + 1, qsn);
+    }
+    const cpx<float> din = scan_fwd(B, T + o.scan + sub);
     B = mk(0.f, 0.f);
 #pragma unroll
     for (int e = E - 1; e >= 0; --e) {
@@ -306,7 +304,7 @@ __global__ void __launch_bounds__(512)
       B = cfma(T[o.xpp + k], B, dp);
       y[e] = B;
     }
-    const cpx<float> uin = scan_bwd(B, Cb);
+    const cpx<float> uin = scan_bwd(B, T + o.scan + 64 + sub);
 #pragma unroll
     for (int e = 0; e < E; ++e) y[e] = cfma(T[o.xGb + e * 16 + sub], uin, y[e]);  // Uc
     // y-implicit right-hand side: x-explicit stencil of Uc (zero on the ring),
@@ -329,18 +327,14 @@ __global__ void __launch_bounds__(512)
         qm[e] = q0[e];
         q0[e] = qp[e];
         qp[e] = qn[e];
-        qs[e] = qsn[e];
+        if (NT < 1024) qs[e] = qsn[e];
       }
+      if (NT >= 1024) load_src(j + 1, qs);
     }
   }
   __syncthreads();
   // ---- y-implicit half step down the columns, stored straight to U with
   // the slice energy accumulated in FP64 ----
-#pragma unroll
-  for (int lv = 0; lv < 4; ++lv) {
-    Cf[lv] = Th[o.scan + 128 + lv * 16 + sub];
-    Cb[lv] = Th[o.scan + 192 + lv * 16 + sub];
-  }
   double es = 0.0;
   for (int cb = 0; cb < nxi; cb += nw * 4) {
     const int col = cb + wv * 4 + grp;  // 0-based interior column
@@ -357,7 +351,7 @@ __global__ void __launch_bounds__(512)
       B = cfma(T[o.yap + e * 16 + sub], B, b);
       y[e] = B;
     }
-    const cpx<float> din = scan_fwd(B, Cf);
+    const cpx<float> din = scan_fwd(B, T + o.scan + 128 + sub);
     B = mk(0.f, 0.f);
 #pragma unroll
     for (int e = E - 1; e >= 0; --e) {
@@ -365,7 +359,7 @@ __global__ void __launch_bounds__(512)
       B = cfma(T[o.ypp + k], B, cfma(T[o.yAf + k], din, y[e]));
       y[e] = B;
     }
-    const cpx<float> uin = scan_bwd(B, Cb);
+    const cpx<float> uin = scan_bwd(B, T + o.scan + 192 + sub);
     cpx<float> *out = Un + (1 + sub * E) * nx + cc + 1;
 #pragma unroll
     for (int e = 0; e < E; ++e) {

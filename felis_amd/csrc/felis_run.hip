@@ -72,10 +72,15 @@ int launch_adi_scan(felis_ctx *ctx, const cpx<float> *tab, int cf, const cpx<flo
   const int E = adi_scan_E(d.nx, d.ny);
   cpx<float> *U = (cpx<float> *)ctx->U;
   KTimer kt(ctx, FELIS_K_ADI_FUSED);
-#define ADI_CASE(e)                                                                                       \
-  case e:                                                                                                 \
-    hipLaunchKernelGGL((k_adi_scan<e>), dim3(nsys), dim3(512), lds, ctx->stream, g, tab, cf, U, Sc, win,      \
-                       ctx->slice_e, gain);                                                                \
+  const bool wide = ctx->tune.adi_threads == 1024;
+#define ADI_CASE(e)                                                                                         \
+  case e:                                                                                                   \
+    if (wide)                                                                                               \
+      hipLaunchKernelGGL((k_adi_scan<e, 1024>), dim3(nsys), dim3(1024), lds, ctx->stream, g, tab, cf, U, Sc, \
+                         win, ctx->slice_e, gain);                                                          \
+    else                                                                                                    \
+      hipLaunchKernelGGL((k_adi_scan<e, 512>), dim3(nsys), dim3(512), lds, ctx->stream, g, tab, cf, U, Sc,   \
+                         win, ctx->slice_e, gain);                                                          \
     break;
   switch (E) {
     ADI_CASE(1) ADI_CASE(2) ADI_CASE(3) ADI_CASE(4) ADI_CASE(5) ADI_CASE(6) ADI_CASE(7) ADI_CASE(8) ADI_CASE(9)
@@ -99,19 +104,19 @@ bool adi_scan_fits(felis_ctx *ctx) {
   const size_t lds = adi_scan_lds(d.nx, d.ny);
   const int E = adi_scan_E(d.nx, d.ny);
   if (E > 9 || lds + 256 > (size_t)ctx->lds_max + 1024) return false;
-  const void *fn = nullptr;
+  const void *fn[2] = {nullptr, nullptr};
   switch (E) {
-    case 1: fn = (const void *)k_adi_scan<1>; break;
-    case 2: fn = (const void *)k_adi_scan<2>; break;
-    case 3: fn = (const void *)k_adi_scan<3>; break;
-    case 4: fn = (const void *)k_adi_scan<4>; break;
-    case 5: fn = (const void *)k_adi_scan<5>; break;
-    case 6: fn = (const void *)k_adi_scan<6>; break;
-    case 7: fn = (const void *)k_adi_scan<7>; break;
-    case 8: fn = (const void *)k_adi_scan<8>; break;
-    default: fn = (const void *)k_adi_scan<9>; break;
+#define ADI_FN(e)                                   \
+  case e:                                           \
+    fn[0] = (const void *)k_adi_scan<e, 512>;       \
+    fn[1] = (const void *)k_adi_scan<e, 1024>;      \
+    break;
+    ADI_FN(1) ADI_FN(2) ADI_FN(3) ADI_FN(4) ADI_FN(5) ADI_FN(6) ADI_FN(7) ADI_FN(8)
+    default: fn[0] = (const void *)k_adi_scan<9, 512>; fn[1] = (const void *)k_adi_scan<9, 1024>; break;
+#undef ADI_FN
   }
-  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess;
+  return hipFuncSetAttribute(fn[0], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess &&
+         hipFuncSetAttribute(fn[1], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess;
 }
 
 template <int NH>
@@ -792,6 +797,10 @@ extern "C" int felis_set_tuning(felis_ctx *ctx, const felis_tuning *t) {
   if (t->deposit_threads != 0 && t->deposit_threads != 256 && t->deposit_threads != 512 &&
       t->deposit_threads != 1024) {
     ctx->err = "felis_set_tuning: deposit_threads must be 256, 512 or 1024";
+    return FELIS_ERR_ARG;
+  }
+  if (t->adi_threads != 0 && t->adi_threads != 512 && t->adi_threads != 1024) {
+    ctx->err = "felis_set_tuning: adi_threads must be 512 or 1024";
     return FELIS_ERR_ARG;
   }
   ctx->tune = *t;

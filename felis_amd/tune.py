@@ -15,7 +15,11 @@ The tunables:
   deposit_threads  threads per deposit workgroup (one workgroup per slice);
   deposit_lds      LDS bytes a deposit workgroup uses for its windows (less
                    LDS: more workgroups per CU, but harmonics deposited in
-                   chunks).
+                   chunks);
+  adi_threads      threads per field-solve workgroup (one system per
+                   workgroup, its LDS allows one per CU: 1024 threads give
+                   each SIMD four waves to hide latency with, at 128
+                   registers per lane).
 
     python -m felis_amd.tune felis_amd/decks/cfg2.inp [--out felis_amd/tuning.json]
 """
@@ -33,8 +37,9 @@ from . import _lib as fl
 
 CACHE = Path(__file__).resolve().parent / "tuning.json"
 ARCH = "gfx950"
-DEFAULT = dict(step_graph=0, deposit_threads=1024, deposit_lds=0)
-SPACE = dict(step_graph=(0, 1), deposit_threads=(256, 512, 1024), deposit_lds=(0, 65536, 32768))
+DEFAULT = dict(step_graph=0, deposit_threads=1024, deposit_lds=0, adi_threads=512)
+SPACE = dict(step_graph=(0, 1), deposit_threads=(256, 512, 1024), deposit_lds=(0, 65536, 32768),
+             adi_threads=(512, 1024))
 
 
 def key(desc, n_particles: int, npol: int) -> str:

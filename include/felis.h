@@ -216,7 +216,7 @@ typedef struct {
   int step_graph;   /* 1: replay one captured step over uniform undulator stretches; 0: launch each kernel */
   int deposit_lds;  /* LDS bytes per deposit workgroup (0: all a workgroup may use) */
   int deposit_threads; /* threads per deposit workgroup: 256, 512 or 1024 (0: 1024) */
-  int pad;
+  int adi_threads;  /* threads per field-solve workgroup: 512 or 1024 (0: 512) */
 } felis_tuning;
 int felis_set_tuning(felis_ctx *ctx, const felis_tuning *t);
 int felis_get_tuning(felis_ctx *ctx, felis_tuning *t);

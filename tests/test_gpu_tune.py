@@ -44,8 +44,10 @@ def ref():
 @pytest.mark.parametrize("t", [dict(step_graph=1, deposit_threads=1024, deposit_lds=0),
                                dict(step_graph=0, deposit_threads=256, deposit_lds=0),
                                dict(step_graph=0, deposit_threads=512, deposit_lds=65536),
-                               dict(step_graph=1, deposit_threads=256, deposit_lds=32768)],
-                         ids=["graph", "dep256", "dep512_64k", "graph_dep256_32k"])
+                               dict(step_graph=1, deposit_threads=256, deposit_lds=32768),
+                               dict(adi_threads=1024),
+                               dict(step_graph=1, deposit_threads=512, deposit_lds=65536, adi_threads=1024)],
+                         ids=["graph", "dep256", "dep512_64k", "graph_dep256_32k", "adi1024", "graph_dep512_adi1024"])
 def test_tunings_agree_with_default(ref, t):
     r = _run(t)
     assert tune._rel_energy_err(r["e"], ref["e"]) < 1e-5
@@ -60,8 +62,10 @@ def test_set_tuning_rejects_bad_shapes():
         ctx.set_tuning(deposit_threads=384)
     with pytest.raises(fl.FelisError):
         ctx.set_tuning(deposit_lds=-1)
-    ctx.set_tuning(step_graph=1, deposit_threads=512, deposit_lds=40000)
-    assert ctx.tuning() == dict(step_graph=1, deposit_threads=512, deposit_lds=40000)
+    with pytest.raises(fl.FelisError):
+        ctx.set_tuning(adi_threads=256)
+    ctx.set_tuning(step_graph=1, deposit_threads=512, deposit_lds=40000, adi_threads=1024)
+    assert ctx.tuning() == dict(step_graph=1, deposit_threads=512, deposit_lds=40000, adi_threads=1024)
     ctx.close()
 
 
