@@ -64,15 +64,18 @@ def kernel_bytes(name: str, d, np_: int, mixed: bool, npol: int = 2) -> float:
 
 
 # kernel-timer name -> rocprofv3 kernel name, for the measured HBM traffic
-PROFILED = {("cfg2", "mixed"): {"adi_fused": "felis::k_adi_scan<8>", "particle_step": "felis::k_pstep<3, false>",
+# (the field solve's name carries its workgroup size, a tuned launch shape)
+PROFILED = {("cfg2", "mixed"): {"adi_fused": "felis::k_adi_scan<8, {adi_threads}>",
+                                "particle_step": "felis::k_pstep<3, false>",
                                 "deposit": "felis::k_deposit_slot<3, 1>"}}
 
 
-def measured_traffic(config: str, precision: str, kernel: str):
+def measured_traffic(config: str, precision: str, kernel: str, tuning=None):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE
     / WRITE_SIZE summary of this workload (scaled by a copy of known size), or
     None when that workload was not profiled."""
-    names = PROFILED.get((config, precision), {})
+    adi_threads = (tuning or {}).get("adi_threads") or 512
+    names = {k: v.format(adi_threads=adi_threads) for k, v in PROFILED.get((config, precision), {}).items()}
     path = ROOT / "profiles" / f"hbm_traffic_{config}_batched.md"
     if kernel not in names or not path.exists():
         return None
@@ -272,7 +275,7 @@ def main():
                    "tuning": g["tuning"]},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": measured_traffic(args.config, args.precision, dom), "bytes_per_launch": bpl,
+                     "traffic": measured_traffic(args.config, args.precision, dom, g["tuning"]), "bytes_per_launch": bpl,
                      "avg_launch_us": avg_s * 1e6},
         "kernels_ms_per_step": {k: v[0] / args.prof_steps for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])},
     }
