@@ -12,6 +12,7 @@ import numpy as np
 
 PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "libfelis_hip.so"
+HDF_PATH = PKG / "libfelis_hdf.so"
 MAX_HARM = 8
 
 OK = 0
@@ -35,8 +36,11 @@ EXPORTS = [
     "felis_run_surrogate", "felis_state_save", "felis_state_restore", "felis_slice_moments",
     "felis_jump_residual", "felis_slice_features", "felis_mlp_shape", "felis_mlp_set_input_affine",
     "felis_mlp_get_input_affine", "felis_jump_emission_residual", "felis_live_polarisations",
-    "felis_set_tuning", "felis_get_tuning",
+    "felis_set_tuning", "felis_get_tuning", "felis_get_desc", "felis_particle_layout", "felis_deck_save_settings",
 ]
+# every entry point include/felis_hdf.h declares (libfelis_hdf.so)
+HDF_EXPORTS = ["felis_save_field_hdf", "felis_restore_field_hdf", "felis_save_particles_hdf",
+               "felis_restore_particles_hdf", "felis_hdf_last_error"]
 
 
 def device_info() -> dict:
@@ -94,6 +98,7 @@ class JumpOpts(C.Structure):
 _dp = C.POINTER(C.c_double)
 _i64p = C.POINTER(C.c_int64)
 _lib = None
+_hdf = None
 
 
 def load() -> C.CDLL:
@@ -132,6 +137,9 @@ def load() -> C.CDLL:
         L.felis_jump_emission_residual.argtypes = [C.c_void_p, C.c_int, _dp]
         L.felis_live_polarisations.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.felis_set_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
+        L.felis_get_desc.argtypes = [C.c_void_p, C.POINTER(Desc)]
+        L.felis_particle_layout.argtypes = [C.c_void_p, _i64p, _dp, _i64p]
+        L.felis_deck_save_settings.argtypes = [C.c_void_p, _dp, _dp]
         L.felis_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
         L.felis_slice_features.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
         L.felis_mlp_shape.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -169,6 +177,23 @@ def load() -> C.CDLL:
         L.felis_device_info.argtypes = [C.POINTER(C.c_int)]
         _lib = L
     return _lib
+
+
+def load_hdf() -> C.CDLL:
+    """libfelis_hdf.so: the HDF5 dump/restore of a context (reference layout)."""
+    global _hdf
+    if _hdf is None:
+        load()
+        if not HDF_PATH.exists():
+            raise FileNotFoundError(f"{HDF_PATH} is not built: run python -m felis_amd.build (needs HDF5)")
+        H = C.CDLL(str(HDF_PATH))
+        H.felis_save_field_hdf.argtypes = [C.c_void_p, C.c_char_p, C.c_double]
+        H.felis_restore_field_hdf.argtypes = [C.c_void_p, C.c_char_p]
+        H.felis_save_particles_hdf.argtypes = [C.c_void_p, C.c_char_p, _dp]
+        H.felis_restore_particles_hdf.argtypes = [C.c_void_p, C.c_char_p]
+        H.felis_hdf_last_error.restype = C.c_char_p
+        _hdf = H
+    return _hdf
 
 
 def _p(a):
@@ -381,6 +406,36 @@ class Context:
         return dict(step_graph=t.step_graph, deposit_lds=t.deposit_lds, deposit_threads=t.deposit_threads,
                     adi_threads=t.adi_threads)
 
+    def particle_layout(self):
+        """(seg_off, seg_weight) of the last particle upload."""
+        d = self.desc
+        so = np.zeros(d.n_species * d.slice_n + 1, np.int64)
+        sw = np.zeros(d.n_species * d.slice_n)
+        n = np.zeros(1, np.int64)
+        self._chk(self.L.felis_particle_layout(self.h, _p(so), _p(sw), _p(n)))
+        return so, sw
+
+    def _hdf(self, rc: int):
+        if rc != OK:
+            raise FelisError(rc, load_hdf().felis_hdf_last_error().decode())
+
+    def save_field_hdf(self, path, spot_sig_r: float = 0.0):
+        """saveFieldHDF: field<it>.h5 in the reference's layout."""
+        self._hdf(load_hdf().felis_save_field_hdf(self.h, str(path).encode(), float(spot_sig_r)))
+
+    def restore_field_hdf(self, path):
+        self._hdf(load_hdf().felis_restore_field_hdf(self.h, str(path).encode()))
+
+    def save_particles_hdf(self, path, gamma0):
+        """saveParticleHDF: particle<it>.h5; gamma0 per species (energy/mc2 + 1)."""
+        g = np.ascontiguousarray(np.broadcast_to(np.asarray(gamma0, np.float64), (self.desc.n_species,)))
+        self._hdf(load_hdf().felis_save_particles_hdf(self.h, str(path).encode(), _p(g)))
+
+    def restore_particles_hdf(self, path):
+        self._hdf(load_hdf().felis_restore_particles_hdf(self.h, str(path).encode()))
+        so, _ = self.particle_layout()
+        self.np = int(so[-1])
+
     def live_polarisations(self) -> int:
         """1 while the y field and its source are exactly zero (skipped by the
         batched step), else 2."""
@@ -460,6 +515,15 @@ class Deck:
         sig = np.zeros(2 * n_species)
         self.L.felis_deck_beam_sigma(self.h, _p(sig))
         return sig.reshape(n_species, 2)
+
+    def save_settings(self, n_species: int):
+        """(gamma0 per species, spot_sig_r) that the savers record."""
+        g = np.zeros(n_species)
+        r = np.zeros(1)
+        rc = self.L.felis_deck_save_settings(self.h, _p(g), _p(r))
+        if rc != OK:
+            raise FelisError(rc, "felis_deck_save_settings")
+        return g, float(r[0])
 
     def beam_layout(self, n_species: int):
         so = np.zeros(n_species * self.slice_n + 1, np.int64)

@@ -72,5 +72,36 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
+# HDF5 dump/restore library: host code on the image's serial HDF5
+HDF5_DIR = Path(os.environ.get("FELIS_HDF5_DIR", "/opt/conda"))
+HDF_LIB = PKG / "libfelis_hdf.so"
+
+
+def build_hdf(verbose: bool = False):
+    """Builds libfelis_hdf.so (g++, linked to libfelis_hip.so and HDF5) when
+    the HDF5 headers are present; returns its path or None. libstdc++ and
+    libgcc are linked statically so that the HDF5 directory on the runpath
+    supplies HDF5 only."""
+    if not (HDF5_DIR / "include" / "hdf5.h").exists():
+        if verbose:
+            print(f"no HDF5 under {HDF5_DIR}: libfelis_hdf.so not built", flush=True)
+        return None
+    tmp = HDF_LIB.with_suffix(".so.tmp")
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", str(CSRC / "felis_hdf.cpp"),
+           f"-I{ROOT / 'include'}", "-isystem", str(HDF5_DIR / "include"),
+           f"-L{PKG}", "-lfelis_hip", f"-L{HDF5_DIR / 'lib'}", "-lhdf5",
+           "-static-libstdc++", "-static-libgcc",
+           f"-Wl,-rpath,$ORIGIN:{HDF5_DIR / 'lib'}", "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"g++ failed on felis_hdf.cpp ({res.returncode})")
+    os.replace(tmp, HDF_LIB)
+    return HDF_LIB
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_hdf(verbose=True))

@@ -436,6 +436,27 @@ extern "C" int felis_destroy(felis_ctx *ctx) {
 
 extern "C" const char *felis_last_error(const felis_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+extern "C" int felis_get_desc(const felis_ctx *ctx, felis_desc *out) {
+  if (!ctx || !out) return FELIS_ERR_ARG;
+  *out = ctx->d;
+  return FELIS_OK;
+}
+
+extern "C" int felis_particle_layout(felis_ctx *ctx, int64_t *seg_off, double *seg_weight, int64_t *n_particles) {
+  if (!ctx) return FELIS_ERR_ARG;
+  if (n_particles) *n_particles = ctx->np;
+  if (ctx->h_seg_off.empty()) {
+    if (!seg_off && !seg_weight) return FELIS_OK;
+    ctx->err = "felis_particle_layout: no particles uploaded";
+    return FELIS_ERR_STATE;
+  }
+  if (seg_off)
+    for (size_t k = 0; k < ctx->h_seg_off.size(); ++k) seg_off[k] = ctx->h_seg_off[k];
+  if (seg_weight)
+    for (size_t k = 0; k < ctx->h_seg_w.size(); ++k) seg_weight[k] = ctx->h_seg_w[k];
+  return FELIS_OK;
+}
+
 extern "C" int felis_device_info(int *info) {
   if (!info) return FELIS_ERR_ARG;
   int dev = 0;

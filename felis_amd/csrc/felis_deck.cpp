@@ -567,6 +567,14 @@ extern "C" int felis_deck_desc(const felis_deck *D, int precision, felis_desc *d
   return FELIS_OK;
 }
 
+extern "C" int felis_deck_save_settings(const felis_deck *D, double *gamma0, double *spot_sig_r) {
+  if (!D) return FELIS_ERR_ARG;
+  if (gamma0)
+    for (size_t s = 0; s < D->beams.size(); ++s) gamma0[s] = D->beams[s].gamma0;
+  if (spot_sig_r) *spot_sig_r = D->spot_sig_r;
+  return FELIS_OK;
+}
+
 extern "C" int felis_deck_steps(const felis_deck *D, int *info) {
   if (!D || !info) return FELIS_ERR_ARG;
   info[0] = D->max_step;

@@ -90,6 +90,8 @@ int felis_create(const felis_desc *desc, felis_ctx **out);
 int felis_destroy(felis_ctx *ctx);
 const char *felis_last_error(const felis_ctx *ctx);
 int felis_sync(felis_ctx *ctx);
+/* The description the context was created with. */
+int felis_get_desc(const felis_ctx *ctx, felis_desc *out);
 /* ABC coefficients alpha, alphaP (nx), beta, betaP (ny) as complex and the
  * Bessel table [bessel_n][maxH/2+2] (boundary.cpp:73-123; one order more than
  * the reference for an even maxH, whose last order it reads past its rows). */
@@ -102,6 +104,10 @@ int felis_upload_particles(felis_ctx *ctx, const int64_t *seg_off, const double 
                            const double *theta, const double *gamma);
 int felis_download_particles(felis_ctx *ctx, double *x, double *y, double *px, double *py,
                              double *theta, double *gamma);
+/* The segment layout of the last upload: seg_off [n_species*slice_n + 1]
+ * (segment = species*slice_n + slice), seg_weight [n_species*slice_n], and
+ * the particle count; any pointer may be NULL. */
+int felis_particle_layout(felis_ctx *ctx, int64_t *seg_off, double *seg_weight, int64_t *n_particles);
 /* which = felis_field. Ux/Uy/ScUx/ScUy: [n_harm][(slice_n+2)*nx*ny] complex;
  * Ez: [sc_l][(slice_n+2)*sc_f*nr] complex. */
 int felis_upload_field(felis_ctx *ctx, int which, const double *data);
@@ -273,6 +279,10 @@ int felis_deck_load_seed(const felis_deck *deck, felis_ctx *ctx);
 /* solveField.cpp:463-468: rms beam sizes sig[2s], sig[2s+1] of each species
  * from the deck's emittance and Twiss parameters. */
 int felis_deck_beam_sigma(const felis_deck *deck, double *sig);
+/* Savers' settings from the deck: gamma0[n_species] = loadList[s].energy /
+ * mc2 + 1 (saveParticleHDF.cpp:145-148) and spot_sig_r = Domain::spotSigR
+ * (parameterSetting.cpp:210) for felis_hdf.h. */
+int felis_deck_save_settings(const felis_deck *deck, double *gamma0, double *spot_sig_r);
 
 /* -------- learned surrogate: a matrix-core perceptron (no counterpart in
  * the reference; BF16 operands, FP32 accumulation and FP32 master weights).

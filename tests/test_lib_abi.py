@@ -7,8 +7,8 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent.parent
 
 
-def declared_symbols():
-    text = (ROOT / "include" / "felis.h").read_text()
+def declared_symbols(header: str = "felis.h"):
+    text = (ROOT / "include" / header).read_text()
     return sorted(set(re.findall(r"^\w[\w\s\*]*?\b(felis_\w+)\s*\(", text, flags=re.M)))
 
 
@@ -24,6 +24,19 @@ def test_library_exports_every_symbol():
         assert hasattr(lib, name), name
     nm = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIB_PATH)], capture_output=True, text=True).stdout
     for name in declared_symbols():
+        assert re.search(rf"\bT {name}\b", nm), name
+
+
+def test_hdf_library_exports_every_symbol():
+    """libfelis_hdf.so (HDF5 dump/restore) loads next to the core library and
+    exports every entry point include/felis_hdf.h declares."""
+    from felis_amd import _lib
+    assert sorted(_lib.HDF_EXPORTS) == declared_symbols("felis_hdf.h")
+    lib = _lib.load_hdf()
+    for name in declared_symbols("felis_hdf.h"):
+        assert hasattr(lib, name), name
+    nm = subprocess.run(["nm", "-D", "--defined-only", str(_lib.HDF_PATH)], capture_output=True, text=True).stdout
+    for name in declared_symbols("felis_hdf.h"):
         assert re.search(rf"\bT {name}\b", nm), name
 
 
