@@ -37,6 +37,7 @@ EXPORTS = [
     "felis_jump_residual", "felis_slice_features", "felis_mlp_shape", "felis_mlp_set_input_affine",
     "felis_mlp_get_input_affine", "felis_jump_emission_residual", "felis_live_polarisations",
     "felis_set_tuning", "felis_get_tuning", "felis_get_desc", "felis_particle_layout", "felis_deck_save_settings",
+    "felis_field_norms", "felis_field_diag",
 ]
 # every entry point include/felis_hdf.h declares (libfelis_hdf.so)
 HDF_EXPORTS = ["felis_save_field_hdf", "felis_restore_field_hdf", "felis_save_particles_hdf",
@@ -140,6 +141,8 @@ def load() -> C.CDLL:
         L.felis_get_desc.argtypes = [C.c_void_p, C.POINTER(Desc)]
         L.felis_particle_layout.argtypes = [C.c_void_p, _i64p, _dp, _i64p]
         L.felis_deck_save_settings.argtypes = [C.c_void_p, _dp, _dp]
+        L.felis_field_norms.argtypes = [C.c_void_p, C.c_double, _dp, _dp]
+        L.felis_field_diag.argtypes = [C.c_void_p, C.c_double, _dp, _dp]
         L.felis_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
         L.felis_slice_features.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
         L.felis_mlp_shape.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -405,6 +408,21 @@ class Context:
         self._chk(self.L.felis_get_tuning(self.h, C.byref(t)))
         return dict(step_graph=t.step_graph, deposit_lds=t.deposit_lds, deposit_threads=t.deposit_threads,
                     adi_threads=t.adi_threads)
+
+    def field_norms(self, spot_sig_r: float = 0.0):
+        """(powerNorm, intensityNorm) of the field dump."""
+        pn, inn = np.zeros(1), np.zeros(1)
+        self._chk(self.L.felis_field_norms(self.h, float(spot_sig_r), _p(pn), _p(inn)))
+        return float(pn[0]), float(inn[0])
+
+    def field_diag(self, spot_sig_r: float = 0.0):
+        """felField on the live field: power [2][n_harm][slice_n] (W) and
+        fluence [2][n_harm][nx*ny] (J/m^2)."""
+        d = self.desc
+        p = np.zeros((2, d.n_harm, d.slice_n))
+        f = np.zeros((2, d.n_harm, d.nx * d.ny))
+        self._chk(self.L.felis_field_diag(self.h, float(spot_sig_r), _p(p), _p(f)))
+        return p, f
 
     def particle_layout(self):
         """(seg_off, seg_weight) of the last particle upload."""

@@ -20,12 +20,7 @@ int fail(int code, const std::string &msg) {
   return code;
 }
 
-// constants.h of the reference
-constexpr double kEMass = 9.10938291e-31;  // kg
-constexpr double kVelocityC = 299792458.0; // m/s
-constexpr double kECharge = 1.60217662e-19;  // C
-constexpr double kZ0 = 376.730313668;      // Ohm
-constexpr int kDataCnt = 8;                // columns of a particle row
+constexpr int kDataCnt = 8;  // columns of a particle row
 
 // HDF5 handles closed on scope exit
 struct Hid {
@@ -145,11 +140,9 @@ extern "C" int felis_save_field_hdf(felis_ctx *ctx, const char *file, double spo
     }
   }
   // metadata, saveFieldHDF.cpp:66-88
-  const double area = d.dimension == 1 ? 2.0 * M_PI * spot_sig_r * spot_sig_r : d.dx * d.dy;
-  const double coef = kEMass * kVelocityC * kVelocityC * d.ks / kECharge;
-  const double power_norm = coef * coef / (2.0 * kZ0) * area;
+  double power_norm = 0.0, intensity_norm = 0.0;
+  if ((e = core(ctx, felis_field_norms(ctx, spot_sig_r, &power_norm, &intensity_norm), "felis_field_norms"))) return e;
   const double bucket_z = d.num_slice * d.lambda0;
-  const double intensity_norm = coef * coef / (2.0 * kZ0) * bucket_z / kVelocityC;
   const int nh = d.n_harm;
   const bool ok = write_meta(f.id, "sliceN", &d.slice_n, 1) && write_meta(f.id, "harmony", d.harmony, (hsize_t)nh) &&
                   write_meta(f.id, "numHarmony", &nh, 1) && write_meta(f.id, "nx", &d.nx, 1) &&

@@ -115,6 +115,16 @@ int felis_download_field(felis_ctx *ctx, int which, double *data);
 /* Per-slice slice power sum_{cells,h,pol} |U|^2 of slots 1..slice_n. */
 int felis_slice_energy(felis_ctx *ctx, double *out /* [2][n_harm][slice_n] */);
 
+/* -------- field diagnostics (diag/felField.cpp on the live field) --------- */
+/* saveFieldHDF.cpp:66-73: powerNorm and intensityNorm of the field dump
+ * (spot_sig_r = Domain::spotSigR, used when dimension = 1). */
+int felis_field_norms(const felis_ctx *ctx, double spot_sig_r, double *power_norm, double *intensity_norm);
+/* felField.cpp:100-160 without the dump: power[pol][h][slice] = powerNorm *
+ * sum over cells of |U|^2 (W) and fluence[pol][h][j*nx + i] = intensityNorm *
+ * sum over slices of |U|^2 (J/m^2), for slices 1..slice_n; either may be NULL. */
+int felis_field_diag(felis_ctx *ctx, double spot_sig_r, double *power /* [2][n_harm][slice_n] */,
+                     double *fluence /* [2][n_harm][nx*ny] */);
+
 /* -------- the per-step operator surface (mesh.h:153-176) --------------- */
 /* solveField.cpp:49-78: deposition (if current_flag) + radial Ez + ADI. */
 int felis_solve_field(felis_ctx *ctx, const felis_lattice *lat);
