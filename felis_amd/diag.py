@@ -1,4 +1,6 @@
-"""Field diagnostics of the reference's diag/felField.cpp, from the live field.
+"""Text outputs of the reference, from the live state: the diag tools'
+files (diag/felField.cpp, diag/felParticle.cpp) and the Static mode's
+savers (saveFile.cpp saveFieldsToTxt, saveParticleToTxt.cpp).
 
 felField reads a field dump and writes two text files per step:
   power<it>   the slice power of every harmonic and polarisation [W]
@@ -71,3 +73,63 @@ def write_cross(path, step: int, desc, fluence) -> None:
                     f.write(_g(fluence[1, h, i * ny + j]) + " ")
                 f.write("\n")
             f.write("\n")
+
+
+def write_fields_txt(path, desc, fluence, intensity_norm) -> None:
+    """saveFieldsToTxt (saveFile.cpp:8-60), the Static mode's Power<it>: per
+    cell (x, y) the sum over slices of |U|^2 of every harmonic, unnormalised
+    (fluence / intensity_norm), at the field's own flat index j*nx + i."""
+    nh, nx, ny = desc.n_harm, desc.nx, desc.ny
+    raw = np.asarray(fluence).reshape(2, nh, nx * ny) / intensity_norm
+    with open(path, "w") as f:
+        f.write("# x            y")
+        for h in range(nh):
+            H = desc.harmony[h]
+            f.write("            Px" + str(H) + "            Py" + str(H))
+        f.write("\n")
+        for i in range(nx):
+            x = i * desc.dx + desc.min_x
+            for j in range(ny):
+                y = j * desc.dy + desc.min_y
+                f.write(_g(x) + " " + _g(y) + " ")
+                for h in range(nh):
+                    f.write(_g(raw[0, h, nx * j + i]) + " " + _g(raw[1, h, nx * j + i]) + " ")
+                f.write("\n")
+            f.write("\n")
+
+
+def write_particles_txt(path, desc, particles, seg_off, seg_weight, species: int) -> None:
+    """saveParticlesToTxt (saveParticleToTxt.cpp:6-45), the Static mode's
+    <s>Particle<it>: every particle of one species, slice by slice, as
+    x y px py theta gamma weight."""
+    sn = desc.slice_n
+    with open(path, "w") as f:
+        f.write("# x            y            px           py")
+        f.write("           theta        gamma        weight\n")
+        for sl in range(sn):
+            seg = species * sn + sl
+            w = _g(seg_weight[seg])
+            for n in range(int(seg_off[seg]), int(seg_off[seg + 1])):
+                f.write(" ".join(_g(particles[k][n]) for k in ("x", "y", "px", "py", "theta", "gamma")) + " " + w + "\n")
+
+
+def write_felparticle(path, step: int, desc, particles, seg_off, seg_weight, species: int, skip: int = 1) -> None:
+    """felParticle (diag/felParticle.cpp:56-110) on the live particles: every
+    skip-th particle of one species (counted over the species' rows in slice
+    order) as s = (slice dPhi + theta) bucketZ / dPhi, x, y, gamma, px, py,
+    weight."""
+    sn = desc.slice_n
+    d_phi = 2 * np.pi * desc.num_slice
+    bucket = desc.lambda0 * desc.num_slice
+    with open(path, "w") as f:
+        f.write("# z = %g, \n" % (desc.min_z + step * desc.dz))
+        idx = 0
+        for sl in range(sn):
+            seg = species * sn + sl
+            for n in range(int(seg_off[seg]), int(seg_off[seg + 1])):
+                if idx % skip == 0:
+                    th = sl * d_phi + particles["theta"][n]
+                    f.write("%.6g %g %g %g %g %g %g\n" % (th * bucket / d_phi, particles["x"][n], particles["y"][n],
+                                                           particles["gamma"][n], particles["px"][n],
+                                                           particles["py"][n], seg_weight[seg]))
+                idx += 1

@@ -38,3 +38,35 @@ def test_cross_file(tmp_path):
     # i = 1, j = 0: x = -0.5, y = -2, value at flat index i*ny + j = 2
     assert lines[5] == "        -0.5           -2            2            6 "
     assert lines[4] == ""  # a blank line after each x block
+
+
+def _desc2():
+    return types.SimpleNamespace(n_harm=1, harmony=[1], nx=2, ny=2, dx=0.5, dy=0.25, min_x=-1.0, min_y=-2.0,
+                                 dz=0.5, slice_n=2, num_slice=1, lambda0=1e-6, min_z=0.0)
+
+
+def test_static_fields_file(tmp_path):
+    f = np.arange(8, dtype=float).reshape(2, 1, 4) * 2.0
+    diag.write_fields_txt(tmp_path / "Power3", _desc2(), f, 2.0)
+    lines = (tmp_path / "Power3").read_text().split("\n")
+    assert lines[0] == "# x            y            Px1            Py1"
+    # i = 1, j = 0 is the field's flat cell nx*j + i = 1 (no transposition)
+    assert lines[4] == "        -0.5           -2            1            5 "
+
+
+def test_particle_files(tmp_path):
+    d = _desc2()
+    p = {k: np.arange(3, dtype=float) + o for o, k in enumerate(("x", "y", "px", "py", "theta", "gamma"))}
+    so = np.array([0, 1, 3], np.int64)
+    w = np.array([0.5, 0.25])
+    diag.write_particles_txt(tmp_path / "0Particle0", d, p, so, w, 0)
+    lines = (tmp_path / "0Particle0").read_text().splitlines()
+    assert lines[0] == "# x            y            px           py           theta        gamma        weight"
+    assert lines[2] == "           1            2            3            4            5            6         0.25"
+    diag.write_felparticle(tmp_path / "f", 4, d, p, so, w, 0, skip=2)
+    lines = (tmp_path / "f").read_text().splitlines()
+    assert lines[0] == "# z = 2, "
+    # rows 0 and 2 of the species; row 2 is in slice 1: s = (2 pi + theta) lambda0 / (2 pi)
+    assert len(lines) == 3
+    s2 = (2 * np.pi + 6.0) * 1e-6 / (2 * np.pi)
+    assert lines[2] == "%.6g 2 3 7 4 5 0.25" % s2

@@ -98,3 +98,26 @@ def test_text_files(state, tmp_path):
     vals = np.array([[float(t) for t in l.split()] for l in body])
     want = np.stack([state["fluence"][pol, h] for h in range(d.n_harm) for pol in (0, 1)], axis=1)
     assert np.allclose(vals[:, 2:], want, rtol=6e-6, atol=0)
+
+
+def test_particle_and_static_text_files(state, tmp_path):
+    """felParticle's and the Static savers' text files carry the live state's
+    numbers at their 6 digits."""
+    d, ctx = state["d"], state["ctx"]
+    p = ctx.download_particles()
+    so, sw = ctx.particle_layout()
+    diag.write_felparticle(tmp_path / "0Particle3", 3, d, p, so, sw, 0, skip=97)
+    rows = np.array([[float(t) for t in l.split()] for l in (tmp_path / "0Particle3").read_text().splitlines()[1:]])
+    n = np.arange(0, so[d.slice_n], 97)
+    sl = np.searchsorted(so, n, side="right") - 1
+    s = (sl * 2 * np.pi * d.num_slice + p["theta"][n]) * d.lambda0 / (2 * np.pi)
+    want = np.stack([s, p["x"][n], p["y"][n], p["gamma"][n], p["px"][n], p["py"][n], sw[sl]], axis=1)
+    assert rows.shape == want.shape
+    assert np.allclose(rows, want, rtol=6e-6, atol=1e-300)
+    diag.write_fields_txt(tmp_path / "Power3", d, state["fluence"], ctx.field_norms(state["spot"])[1])
+    body = [l for l in (tmp_path / "Power3").read_text().splitlines()[1:] if l.strip()]
+    vals = np.array([[float(t) for t in l.split()] for l in body])
+    U = ctx.download_field(fl.UX).reshape(d.n_harm, d.slice_n + 2, d.ny, d.nx)[:, 1:-1]
+    raw = (np.abs(U) ** 2).sum(axis=1)            # [h][j][i]
+    want = raw.transpose(0, 2, 1).reshape(d.n_harm, -1)  # rows i-major, j-minor
+    assert np.allclose(vals[:, 2], want[0], rtol=6e-6, atol=0)
