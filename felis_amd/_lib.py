@@ -37,7 +37,7 @@ EXPORTS = [
     "felis_jump_residual", "felis_slice_features", "felis_mlp_shape", "felis_mlp_set_input_affine",
     "felis_mlp_get_input_affine", "felis_jump_emission_residual", "felis_live_polarisations",
     "felis_set_tuning", "felis_get_tuning", "felis_get_desc", "felis_particle_layout", "felis_deck_save_settings",
-    "felis_field_norms", "felis_field_diag",
+    "felis_field_norms", "felis_field_diag", "felis_wake_function", "felis_deck_wake_function_device",
 ]
 # every entry point include/felis_hdf.h declares (libfelis_hdf.so)
 HDF_EXPORTS = ["felis_save_field_hdf", "felis_restore_field_hdf", "felis_save_particles_hdf",
@@ -142,6 +142,9 @@ def load() -> C.CDLL:
         L.felis_particle_layout.argtypes = [C.c_void_p, _i64p, _dp, _i64p]
         L.felis_deck_save_settings.argtypes = [C.c_void_p, _dp, _dp]
         L.felis_field_norms.argtypes = [C.c_void_p, C.c_double, _dp, _dp]
+        L.felis_wake_function.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double,
+                                          _dp, _dp]
+        L.felis_deck_wake_function_device.argtypes = [C.c_void_p, _dp, _dp]
         L.felis_field_diag.argtypes = [C.c_void_p, C.c_double, _dp, _dp]
         L.felis_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
         L.felis_slice_features.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
@@ -180,6 +183,16 @@ def load() -> C.CDLL:
         L.felis_device_info.argtypes = [C.POINTER(C.c_int)]
         _lib = L
     return _lib
+
+
+def wake_function(shape: int, acdc: int, radius: float, cond: float, ctau: float, slice_n: int, dZ: float):
+    """wakeField.cpp:53-159 on the device for explicit settings: (wakeF, impZ)."""
+    wf, imp = np.zeros(slice_n), np.zeros(1001)
+    rc = load().felis_wake_function(int(shape), int(acdc), float(radius), float(cond), float(ctau), int(slice_n),
+                                    float(dZ), _p(wf), _p(imp))
+    if rc != OK:
+        raise FelisError(rc, "felis_wake_function")
+    return wf, imp
 
 
 def load_hdf() -> C.CDLL:
@@ -533,6 +546,14 @@ class Deck:
         sig = np.zeros(2 * n_species)
         self.L.felis_deck_beam_sigma(self.h, _p(sig))
         return sig.reshape(n_species, 2)
+
+    def wake_function_device(self):
+        """(wakeF, impZ) of wake_function, computed on the device."""
+        wf = np.zeros(self.slice_n); imp = np.zeros(1001)
+        rc = self.L.felis_deck_wake_function_device(self.h, _p(wf), _p(imp))
+        if rc != OK:
+            raise FelisError(rc, "felis_deck_wake_function_device")
+        return wf, imp
 
     def save_settings(self, n_species: int):
         """(gamma0 per species, spot_sig_r) that the savers record."""

@@ -280,6 +280,15 @@ int felis_deck_steps(const felis_deck *deck, int *info);
 int felis_deck_lattice(const felis_deck *deck, int it0, int n_steps, felis_lattice *tab);
 /* wakeField.cpp:53-159: wake function wakeF[sliceN]; impZ[1001] may be NULL. */
 int felis_deck_wake_function(const felis_deck *deck, double *wakeF, double *impZ);
+/* the same table computed on the device (the flat-pipe impedance integral
+ * is 5e7 complex evaluations). */
+int felis_deck_wake_function_device(const felis_deck *deck, double *wakeF, double *impZ);
+/* wakeField.cpp:53-159 on the device for explicit settings: shape 1 flat,
+ * 2 circular pipe; acdc 1 AC, else DC conductivity; radius [m], cond [S/m],
+ * ctau [m]; dZ = numSlice * lambda0. wakeF[slice_n], impZ[1001] (may be
+ * NULL). FELIS_ERR_NAN where the reference prints the NaN and exits. */
+int felis_wake_function(int shape, int acdc, double radius, double cond, double ctau, int slice_n, double dZ,
+                        double *wakeF, double *impZ);
 /* loadBeam.cpp:94-157: particle counts and weights per (species, slice). */
 int felis_deck_beam_layout(const felis_deck *deck, int64_t *seg_off, double *seg_weight);
 /* loadBeam.cpp:28-269 generated on the device; seed keys the shot noise. */
