@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import fel_cases as fc
+from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
 
@@ -176,6 +177,44 @@ def test_run_mixed_matches_oracle(harm, first_cf):
     assert np.max(np.abs(tot_g - tot_o) / tot_o) < 1e-4
     bg = np.abs(b_g[:, 0:2 * nh:2] + 1j * b_g[:, 1:2 * nh:2]) / b_g[:, 2 * nh:2 * nh + 1]
     assert np.max(np.abs(bg - b_o[:, :nh])) < 1e-3
+    assert fc.rel_err(t_g, t_o) < 1e-4
+    p = ctx.download_particles()
+    assert fc.rel_err(p["gamma"] - case.state["gamma"], o.gamma - case.state["gamma"]) < 1e-3
+
+
+@pytest.mark.parametrize("n", [33, 151], ids=["fused_adi", "two_pass_adi"])
+def test_run_mixed_lattice_and_wake(n):
+    """The batched loop through the features of the multi-undulator workload
+    (4096 slices, 151 x 151): a linear K taper, alternating quad gradients, a
+    two-period drift between undulator sections (no current, no slippage) and
+    wake updates every four periods from the circular AC resistive-wall wake
+    (radius 3 mm, 5.8e7 S/m, ctau 2.4 um), against the FP64 oracle loop. A
+    151 x 151 grid does not fit the LDS-resident ADI and takes the two-pass
+    solve. Tolerances as in test_run_mixed_matches_oracle."""
+    case = fc.make_case(slice_n=6, harmonics=(1,), ppslice=256, nx=n, ny=n)
+    o = case.oracle_state()
+    ctx = fc.gpu_context(case, precision=fl.MIXED_FP32)
+    nsteps = 12
+    base = case.lattice()
+    tab = [case.lattice(current_flag=0)]
+    for k in range(nsteps):
+        drift = k in (5, 6)
+        for half in (0, 1):
+            lat = case.lattice(K0=base.K0 * (1.0 - 0.002 * k), g=(1 if (2 * k + half) % 4 < 2 else -1) * base.g)
+            if drift:
+                lat.ku, lat.drift_flag, lat.current_flag = 0.0, 1, 0
+            tab.append(lat)
+    d = case.desc_kw
+    rc, _, wakeF = po.wake_function(2, 1, 3e-3, 5.8e7, 2.4e-6, d["slice_n"], d["num_slice"] * d["lambda0"])
+    assert rc == 0
+    rc, e_o, b_o, t_o = o.run(tab, 0, nsteps, True, 4, wakeF)
+    assert rc == 0
+    e_g, b_g, t_g = ctx.run([fc.to_lib_lattice(l) for l in tab], 0, nsteps, 4, wakeF)
+    tot_g = e_g[:, :1].sum(1) + e_g[:, 1:].sum(1)
+    tot_o = e_o[:, :1].sum(1) + e_o[:, 1:].sum(1)
+    assert np.max(np.abs(tot_g - tot_o) / tot_o) < 1e-4
+    bg = np.abs(b_g[:, 0] + 1j * b_g[:, 1]) / b_g[:, 2]
+    assert np.max(np.abs(bg - b_o[:, 0])) < 1e-3
     assert fc.rel_err(t_g, t_o) < 1e-4
     p = ctx.download_particles()
     assert fc.rel_err(p["gamma"] - case.state["gamma"], o.gamma - case.state["gamma"]) < 1e-3
