@@ -34,6 +34,9 @@ WORKLOADS = {
     "cfg1": "time-dependent 256 slices x 4096 particles, 101x101 grid, harmonic 1, flat 3 kA, 10 m (385 periods)",
     "cfg2": "time-dependent 2048 slices, 256-16384 particles/slice (11,670,128), 129x129, harmonics 1,3,5, "
             "Gaussian 4 kA, 20 m (770 periods)",
+    "cfg3": "time-dependent 4096 slices x 8192 particles (33,554,432), 151x151, harmonic 1, Gaussian 3 kA, six 5 m "
+            "units with 1 m drifts, -1%/unit taper from the fourth, FODO quads, circular AC wake, 29 m "
+            "(1116 periods)",
 }
 
 

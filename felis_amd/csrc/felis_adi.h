@@ -34,18 +34,16 @@ __host__ __device__ inline int adi_scan_E(int nx, int ny) {
   return (n + 15) / 16;
 }
 
-// skewed LDS layout of the (ny-2) x (nx-2) y right-hand side
+// skewed LDS layout of `width` columns of the (ny-2)-row y right-hand side
 struct DyLayout {
   int E, P, sc, sr;
-  __host__ __device__ DyLayout(int nx, int ny, int E_) : E(E_) {
-    const int nxi = nx - 2;
+  __host__ __device__ DyLayout(int width, int E_) : E(E_) {
     sc = (E % 2 == 0) ? 1 : 0;
-    P = nxi + sc * ((nxi - 1) / E);
+    P = width + sc * ((width - 1) / E);
     sr = ((E * P) % 2 == 0) ? 1 : 0;
-    (void)ny;
   }
   __host__ __device__ int at(int r, int c) const { return r * P + c + sc * (c / E) + sr * (r / E); }
-  __host__ __device__ int size(int ny) const { return at(ny - 3, 0) + P; }
+  __host__ __device__ int size(int rows) const { return at(rows - 1, 0) + P; }
 };
 
 // Per harmonic table block (complex FP32), e-major entries [e*16 + lane]:
@@ -66,11 +64,23 @@ struct AdiTabOff {
   }
 };
 
-inline size_t adi_scan_lds(int nx, int ny) {
+// LDS bytes of a workgroup that holds `cols` interior columns of the system
+inline size_t adi_scan_lds(int nx, int ny, int cols) {
   const int E = adi_scan_E(nx, ny);
-  DyLayout L(nx, ny, E);
+  DyLayout L(cols, E);
   AdiTabOff o(E, ny);
-  return ((size_t)L.size(ny) + o.staged) * sizeof(cpx<float>);
+  return ((size_t)L.size(ny - 2) + o.staged) * sizeof(cpx<float>);
+}
+
+// Interior columns whose y right-hand side stays in LDS: all of them when
+// they fit, else the first 8 lanes' 8E columns (the rest goes through the
+// system's scratch in HBM and is solved second); 0 when neither fits.
+inline int adi_scan_cols(int nx, int ny, size_t lds_max) {
+  const int E = adi_scan_E(nx, ny), nxi = nx - 2;
+  if (E > 10) return 0;
+  if (adi_scan_lds(nx, ny, nxi) + 256 <= lds_max + 1024) return nxi;
+  if (8 * E < nxi && adi_scan_lds(nx, ny, 8 * E) + 256 <= lds_max + 1024) return 8 * E;
+  return 0;
 }
 
 // Host: the tables above from the hoisted Thomas coefficients (r, a, q, p per
@@ -204,18 +214,21 @@ __device__ inline void buf_load_row(__amdgpu_buffer_rsrc_t r, int byte_off, cpx<
 }
 
 // One workgroup (NT threads: 512 or 1024) per system sys = (pol*nh + h)*slice_n + sl.
+// cw: interior columns whose y right-hand side the x sweep keeps in LDS (a
+// multiple of E, or all); the remaining nxi - cw columns go to the system's
+// part of `scratch` and are brought into LDS after the first cw are solved.
 template <int E, int NT>
 __global__ void __launch_bounds__(NT)
     k_adi_scan(GridArgs g, const cpx<float> *__restrict__ tab, int cf, cpx<float> *__restrict__ U,
                const cpx<float> *__restrict__ Sc, const int4 *__restrict__ win, double *__restrict__ slice_e,
-               const cpx<float> *__restrict__ gain) {
+               const cpx<float> *__restrict__ gain, int cw, cpx<float> *__restrict__ scratch) {
   extern __shared__ double lds_d[];
-  __shared__ double red[8];
+  __shared__ double red[NT / 64];
   const int nx = g.nx, ny = g.ny, nxi = nx - 2, nyi = ny - 2;
-  const DyLayout L(nx, ny, E);
+  const DyLayout L(cw, E);
   const AdiTabOff o(E, ny);
   cpx<float> *DY = reinterpret_cast<cpx<float> *>(lds_d);
-  cpx<float> *T = DY + L.size(ny);
+  cpx<float> *T = DY + L.size(nyi);
   const int sys = blockIdx.x;
   const int sl = sys % g.slice_n, ph = sys / g.slice_n, h = ph % g.nh, slot = sl + 1;
   const float H = (float)g.harm[h];
@@ -229,6 +242,8 @@ __global__ void __launch_bounds__(NT)
   cpx<float> *Un = U + ((long long)ph * g.S + phys_slot(slot, rb, g.S)) * g.N;
   const cpx<float> *Sn = Sc + ((long long)ph * g.S + slot) * g.N;
   const cpx<float> *Th = tab + (long long)h * o.block;
+  const int w1 = nxi - cw;  // columns solved second, through scratch
+  cpx<float> *Gs = scratch + (long long)sys * nyi * w1;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
   for (int k = tid; k < o.staged; k += blockDim.x) T[k] = Th[k];
   const int grp = lane >> 4, sub = lane & 15;
@@ -311,8 +326,10 @@ __global__ void __launch_bounds__(NT)
     // times the y sweep's first multiplier for this row
     const cpx<float> left = mk(dpp_shr<1>(y[E - 1].re), dpp_shr<1>(y[E - 1].im));
     const cpx<float> right = mk(dpp_shl<1>(y[0].re), dpp_shl<1>(y[0].im));
-    // row j-1 of the layout, columns i0-1 .. i0-1+E-1 are consecutive
-    cpx<float> *dst = DY + L.at(jj - 1, i0 - 1);
+    // row j-1, columns i0-1 .. i0-1+E-1: consecutive in the LDS layout (a
+    // lane's columns are all below cw or all above) or in the scratch rows
+    const bool in_lds = i0 - 1 < cw;
+    cpx<float> *dst = in_lds ? DY + L.at(jj - 1, i0 - 1) : Gs + (long long)(jj - 1) * w1 + (i0 - 1 - cw);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int i = i0 + e, k = e * 16 + sub;
@@ -334,47 +351,62 @@ __global__ void __launch_bounds__(NT)
   }
   __syncthreads();
   // ---- y-implicit half step down the columns, stored straight to U with
-  // the slice energy accumulated in FP64 ----
+  // the slice energy accumulated in FP64: the columns held in LDS, then
+  // (when the system does not fit) the rest from scratch ----
   double es = 0.0;
-  for (int cb = 0; cb < nxi; cb += nw * 4) {
-    const int col = cb + wv * 4 + grp;  // 0-based interior column
-    const bool colok = col < nxi;
-    const int cc = colok ? col : nxi - 1;
-    // rows sub*E .. sub*E+E-1 of column cc are P apart in the layout
-    const cpx<float> *src = DY + L.at(sub * E, cc);
-    cpx<float> y[E];
-    cpx<float> B = mk(0.f, 0.f);
+  auto ysweep = [&](const DyLayout &Ly, int c0, int width) {
+    for (int cb = 0; cb < width; cb += nw * 4) {
+      const int col = cb + wv * 4 + grp;  // column within the range
+      const bool colok = col < width;
+      const int cc = colok ? col : width - 1;
+      // rows sub*E .. sub*E+E-1 of column cc are P apart in the layout
+      const cpx<float> *src = DY + Ly.at(sub * E, cc);
+      cpx<float> y[E];
+      cpx<float> B = mk(0.f, 0.f);
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int r = sub * E + e;
-      const cpx<float> b = r < nyi ? src[e * L.P] : mk(0.f, 0.f);
-      B = cfma(T[o.yap + e * 16 + sub], B, b);
-      y[e] = B;
-    }
-    const cpx<float> din = scan_fwd(B, T + o.scan + 128 + sub);
-    B = mk(0.f, 0.f);
+      for (int e = 0; e < E; ++e) {
+        const int r = sub * E + e;
+        const cpx<float> b = r < nyi ? src[e * Ly.P] : mk(0.f, 0.f);
+        B = cfma(T[o.yap + e * 16 + sub], B, b);
+        y[e] = B;
+      }
+      const cpx<float> din = scan_fwd(B, T + o.scan + 128 + sub);
+      B = mk(0.f, 0.f);
 #pragma unroll
-    for (int e = E - 1; e >= 0; --e) {
-      const int k = e * 16 + sub;
-      B = cfma(T[o.ypp + k], B, cfma(T[o.yAf + k], din, y[e]));
-      y[e] = B;
-    }
-    const cpx<float> uin = scan_bwd(B, T + o.scan + 192 + sub);
-    cpx<float> *out = Un + (1 + sub * E) * nx + cc + 1;
+      for (int e = E - 1; e >= 0; --e) {
+        const int k = e * 16 + sub;
+        B = cfma(T[o.ypp + k], B, cfma(T[o.yAf + k], din, y[e]));
+        y[e] = B;
+      }
+      const cpx<float> uin = scan_bwd(B, T + o.scan + 192 + sub);
+      cpx<float> *out = Un + (1 + sub * E) * nx + c0 + cc + 1;
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int r = sub * E + e;
-      if (colok && r < nyi) {
-        cpx<float> v = cfma(T[o.yGb + e * 16 + sub], uin, y[e]);
-        if (has_gain) v = v * gs;
-        out[e * nx] = v;
-        es += (double)v.re * v.re + (double)v.im * v.im;
+      for (int e = 0; e < E; ++e) {
+        const int r = sub * E + e;
+        if (colok && r < nyi) {
+          cpx<float> v = cfma(T[o.yGb + e * 16 + sub], uin, y[e]);
+          if (has_gain) v = v * gs;
+          out[e * nx] = v;
+          es += (double)v.re * v.re + (double)v.im * v.im;
+        }
       }
     }
+  };
+  ysweep(L, 0, cw);
+  if (w1 > 0) {
+    // the scratch rows (written above by this workgroup) into LDS
+    __syncthreads();
+    const DyLayout L1(w1, E);
+    for (int q = tid; q < nyi * w1; q += NT) {
+      const int r = q / w1, c = q - r * w1;
+      DY[L1.at(r, c)] = Gs[q];
+    }
+    __syncthreads();
+    ysweep(L1, cw, w1);
   }
   // the boundary ring is untouched by the ADI but counts in the energy
   const int nring = 2 * nx + 2 * (ny - 2);
-  for (int k = tid; k < nring; k += blockDim.x) {
+  for (int k = tid; k < nring; k += NT) {
     int idx;
     if (k < nx) idx = k;
     else if (k < 2 * nx) idx = (ny - 1) * nx + (k - nx);

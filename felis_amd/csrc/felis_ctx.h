@@ -64,6 +64,7 @@ struct felis_ctx {
   long long *chunk_start = nullptr;
   int lds_max = 65536;  // dynamic LDS bytes a workgroup may use
   int n_cu = 256;       // compute units of the device
+  int adi_cols = 0;     // interior columns the batched ADI keeps in LDS (adi_scan_fits)
   felis::Tiles chunks() const { return felis::Tiles{chunk_seg, chunk_start, chunk_cnt}; }
 
   // per-kernel timing (felis_profile)

@@ -68,22 +68,29 @@ int launch_adi_scan(felis_ctx *ctx, const cpx<float> *tab, int cf, const cpx<flo
   GridArgs g = ctx->grid();
   const felis_desc &d = ctx->d;
   const int nsys = ctx->npol() * d.n_harm * d.slice_n;  // the x systems first: a dead Uy is skipped
-  const size_t lds = adi_scan_lds(d.nx, d.ny);
+  const int cw = ctx->adi_cols;
+  const size_t lds = adi_scan_lds(d.nx, d.ny, cw);
   const int E = adi_scan_E(d.nx, d.ny);
   cpx<float> *U = (cpx<float> *)ctx->U;
+  cpx<float> *scratch = (cpx<float> *)ctx->Uc;  // columns solved second when a system is split
   KTimer kt(ctx, FELIS_K_ADI_FUSED);
-  const bool wide = ctx->tune.adi_threads == 1024;
+  // 1024 threads need the kernel in 128 registers: whole systems of E <= 9
+  const bool wide = ctx->tune.adi_threads == 1024 && E <= 9 && cw == d.nx - 2;
 #define ADI_CASE(e)                                                                                         \
   case e:                                                                                                   \
     if (wide)                                                                                               \
       hipLaunchKernelGGL((k_adi_scan<e, 1024>), dim3(nsys), dim3(1024), lds, ctx->stream, g, tab, cf, U, Sc, \
-                         win, ctx->slice_e, gain);                                                          \
+                         win, ctx->slice_e, gain, cw, scratch);                                             \
     else                                                                                                    \
       hipLaunchKernelGGL((k_adi_scan<e, 512>), dim3(nsys), dim3(512), lds, ctx->stream, g, tab, cf, U, Sc,   \
-                         win, ctx->slice_e, gain);                                                          \
+                         win, ctx->slice_e, gain, cw, scratch);                                             \
     break;
   switch (E) {
     ADI_CASE(1) ADI_CASE(2) ADI_CASE(3) ADI_CASE(4) ADI_CASE(5) ADI_CASE(6) ADI_CASE(7) ADI_CASE(8) ADI_CASE(9)
+    case 10:
+      hipLaunchKernelGGL((k_adi_scan<10, 512>), dim3(nsys), dim3(512), lds, ctx->stream, g, tab, cf, U, Sc, win,
+                         ctx->slice_e, gain, cw, scratch);
+      break;
     default:
       ctx->err = "grid too wide for the LDS-resident ADI";
       return FELIS_ERR_UNSUPPORTED;
@@ -97,13 +104,17 @@ int launch_adi_scan(felis_ctx *ctx, const cpx<float> *tab, int cf, const cpx<flo
   return FELIS_OK;
 }
 
-// true when the system's right-hand side and tables fit the LDS of one
-// workgroup; sets the kernel's dynamic LDS limit
+// true when the system's tables and its y right-hand side (all of it, or
+// the first 8E columns with the rest through scratch) fit the LDS of one
+// workgroup; records the columns kept in LDS and sets the kernel's dynamic
+// LDS limit
 bool adi_scan_fits(felis_ctx *ctx) {
   const felis_desc &d = ctx->d;
-  const size_t lds = adi_scan_lds(d.nx, d.ny);
+  const int cols = adi_scan_cols(d.nx, d.ny, (size_t)ctx->lds_max);
+  if (cols == 0) return false;
+  ctx->adi_cols = cols;
+  const size_t lds = adi_scan_lds(d.nx, d.ny, cols);
   const int E = adi_scan_E(d.nx, d.ny);
-  if (E > 9 || lds + 256 > (size_t)ctx->lds_max + 1024) return false;
   const void *fn[2] = {nullptr, nullptr};
   switch (E) {
 #define ADI_FN(e)                                   \
@@ -111,12 +122,12 @@ bool adi_scan_fits(felis_ctx *ctx) {
     fn[0] = (const void *)k_adi_scan<e, 512>;       \
     fn[1] = (const void *)k_adi_scan<e, 1024>;      \
     break;
-    ADI_FN(1) ADI_FN(2) ADI_FN(3) ADI_FN(4) ADI_FN(5) ADI_FN(6) ADI_FN(7) ADI_FN(8)
-    default: fn[0] = (const void *)k_adi_scan<9, 512>; fn[1] = (const void *)k_adi_scan<9, 1024>; break;
+    ADI_FN(1) ADI_FN(2) ADI_FN(3) ADI_FN(4) ADI_FN(5) ADI_FN(6) ADI_FN(7) ADI_FN(8) ADI_FN(9)
+    default: fn[0] = (const void *)k_adi_scan<10, 512>; break;
 #undef ADI_FN
   }
   return hipFuncSetAttribute(fn[0], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess &&
-         hipFuncSetAttribute(fn[1], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess;
+         (!fn[1] || hipFuncSetAttribute(fn[1], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess);
 }
 
 template <int NH>

@@ -182,15 +182,16 @@ def test_run_mixed_matches_oracle(harm, first_cf):
     assert fc.rel_err(p["gamma"] - case.state["gamma"], o.gamma - case.state["gamma"]) < 1e-3
 
 
-@pytest.mark.parametrize("n", [33, 151], ids=["fused_adi", "two_pass_adi"])
+@pytest.mark.parametrize("n", [33, 151, 171], ids=["fused_adi", "split_lds_adi", "two_pass_adi"])
 def test_run_mixed_lattice_and_wake(n):
     """The batched loop through the features of the multi-undulator workload
     (4096 slices, 151 x 151): a linear K taper, alternating quad gradients, a
     two-period drift between undulator sections (no current, no slippage) and
     wake updates every four periods from the circular AC resistive-wall wake
     (radius 3 mm, 5.8e7 S/m, ctau 2.4 um), against the FP64 oracle loop. A
-    151 x 151 grid does not fit the LDS-resident ADI and takes the two-pass
-    solve. Tolerances as in test_run_mixed_matches_oracle."""
+    151 x 151 system does not fit one workgroup's LDS whole: the fused ADI
+    keeps 80 columns in LDS and passes the rest through scratch; 171 x 171
+    takes the two-pass solve. Tolerances as in test_run_mixed_matches_oracle."""
     case = fc.make_case(slice_n=6, harmonics=(1,), ppslice=256, nx=n, ny=n)
     o = case.oracle_state()
     ctx = fc.gpu_context(case, precision=fl.MIXED_FP32)
