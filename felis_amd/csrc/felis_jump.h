@@ -10,7 +10,8 @@
 //
 //   source   Sw[s] = sum_{i<J} Sc[s+i]     the slices field slot s passes
 //                                           while it slips to slot s+J
-//   field    Ub[s] = 1/J sum_{i<J} U[s-i]  the field slots that pass slice s
+//   field    Ub[s] = 1/J sum_{i<J} U_i[s-i] the field slots that pass slice s,
+//                                           each at its time in the jump
 //                                           (logical slots < 1 hold no field)
 //
 // Both run over slots in chunks with a halo, one thread per (cell, pol x
@@ -121,12 +122,21 @@ static __global__ void __launch_bounds__(256) k_src_window(int nx, int ny, int S
   }
 }
 
-// Ub[ph][phys(s)] = 1/J sum_{i<J, s-i>=1} U[ph][phys(s-i)] for the chunk's
-// slots, at one cell of the widened beam box; U and Ub share the ring.
+// The push field of a jump: slice s meets field slot s-i in period i of the
+// jump (i < J), and in the fine steps that slot's field there is the field
+// after i+1 of the J solves. With the field before the jump's solve (Uo) and
+// after it (Un), linear in between,
+//   Ub[s] = 1/J sum_{i<J, s-i>=1} [(1 - w_i) Uo(s-i) + w_i Un(s-i)], w_i = (i+1)/J.
+// With A(s) = sum_i U(s-i) and B(s) = sum_i (i+1) U(s-i), run along the slots
+// in FP64 (A(s+1) = A + U(s+1) - U(s-J+1), B(s+1) = B + A + U(s+1) -
+// (J+1) U(s-J+1)), the pass over Uo (before the solve, add = 0) writes
+// (A - B/J)/J and the pass over Un (after it, add = 1) adds B/J^2. J = 1
+// gives Un: the fine step. U and Ub share the ring; one thread per (cell of
+// the widened beam box, pol x harmonic, chunk of slots).
 static __global__ void __launch_bounds__(256) k_field_window(int nx, int ny, int S, int slice_n, int ring_base, int J,
                                                       const int4 *__restrict__ bbox,
                                                       const cpx<float> *__restrict__ U,
-                                                      cpx<float> *__restrict__ Ub) {
+                                                      cpx<float> *__restrict__ Ub, int add) {
   const long long N = (long long)nx * ny;
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N) return;
@@ -137,24 +147,30 @@ static __global__ void __launch_bounds__(256) k_field_window(int nx, int ny, int
   const int s1 = min(slice_n, s0 + kJumpChunk - 1);
   const cpx<float> *src = U + (long long)blockIdx.y * S * N + c;
   cpx<float> *dst = Ub + (long long)blockIdx.y * S * N + c;
-  auto at = [&](int s) { return src[(long long)phys_slot(s, ring_base, S) * N]; };
+  auto at = [&](int s) { return s >= 1 ? src[(long long)phys_slot(s, ring_base, S) * N] : mk(0.f, 0.f); };
   const double inv = 1.0 / J;
-  double sr = 0.0, si = 0.0;
-  for (int s = max(1, s0 - J + 1); s < s0; ++s) {
-    const cpx<float> v = at(s);
-    sr += v.re;
-    si += v.im;
+  double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+  for (int q = 0; q < J; ++q) {
+    const cpx<float> v = at(s0 - q);
+    ar += v.re;
+    ai += v.im;
+    br += (q + 1) * (double)v.re;
+    bi += (q + 1) * (double)v.im;
   }
   for (int s = s0; s <= s1; ++s) {
-    const cpx<float> v = at(s);
-    sr += v.re;
-    si += v.im;
-    dst[(long long)phys_slot(s, ring_base, S) * N] = mk((float)(sr * inv), (float)(si * inv));
-    const int so = s - J + 1;
-    if (so >= 1) {
-      const cpx<float> w = at(so);
-      sr -= w.re;
-      si -= w.im;
+    cpx<float> *o = dst + (long long)phys_slot(s, ring_base, S) * N;
+    if (add) {
+      const cpx<float> b = *o;
+      *o = mk((float)(b.re + br * inv * inv), (float)(b.im + bi * inv * inv));
+    } else {
+      *o = mk((float)((ar - br * inv) * inv), (float)((ai - bi * inv) * inv));
+    }
+    if (s < s1) {
+      const cpx<float> un = at(s + 1), uo = at(s - J + 1);
+      br += ar + un.re - (J + 1) * (double)uo.re;
+      bi += ai + un.im - (J + 1) * (double)uo.im;
+      ar += un.re - uo.re;
+      ai += un.im - uo.im;
     }
   }
 }

@@ -637,17 +637,24 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
             (err = felis_decode_corr_dev(ctx, jc->Y, jc->gain)))
           break;
       }
-      // the field solve keeps the windowed source: the correction and the
-      // training residual read it after the push
-      if ((err = launch_adi_scan(ctx, jc->tab, 2, (cpx<float> *)ctx->Sw, ctx->win2))) break;
-      // the push field: the J field slots that pass each slice
+      // the push field: the J field slots that pass each slice, each at its
+      // time in the jump, from the field before the solve and after it
       hipLaunchKernelGGL(k_win_bbox, dim3(1), dim3(256), 0, ctx->stream, ctx->win, d.slice_n, d.nx, d.ny,
                          kJumpMargin, ctx->box + 1);
       {
         KTimer kt(ctx, FELIS_K_JUMP_WINDOW);
         hipLaunchKernelGGL(k_field_window, dim3(ncb, ctx->npol() * nh, nchunk), dim3(256), 0, ctx->stream, d.nx, d.ny,
                            ctx->S, d.slice_n, gw.ring_base, J, ctx->box + 1, (const cpx<float> *)ctx->U,
-                           (cpx<float> *)ctx->Ub);
+                           (cpx<float> *)ctx->Ub, 0);
+      }
+      // the field solve keeps the windowed source: the correction and the
+      // training residual read it after the push
+      if ((err = launch_adi_scan(ctx, jc->tab, 2, (cpx<float> *)ctx->Sw, ctx->win2))) break;
+      {
+        KTimer kt(ctx, FELIS_K_JUMP_WINDOW);
+        hipLaunchKernelGGL(k_field_window, dim3(ncb, ctx->npol() * nh, nchunk), dim3(256), 0, ctx->stream, d.nx, d.ny,
+                           ctx->S, d.slice_n, gw.ring_base, J, ctx->box + 1, (const cpx<float> *)ctx->U,
+                           (cpx<float> *)ctx->Ub, 1);
       }
     } else if (fused) {
       if ((err = launch_adi_scan(ctx, adi_tab, ls->current_flag, (cpx<float> *)ctx->Sc, ctx->win))) break;
