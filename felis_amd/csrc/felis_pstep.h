@@ -111,6 +111,24 @@ __device__ inline void transverse_f32(const LatArgs &l, float dz, float gam, flo
   const float ig = 1.0f / gam, ig2 = ig * ig;
   const float dd = ku * K0 * K0 * 0.25f * ig2 * ue;
   const float fx = bb * ig * xCoef + cc, fy = bb * ig * yCoef - cc;
+  if (aa == 0.f && dd == 0.f) {
+    // ue K0 = 0: the planes decouple into u' = ig p, p' = -f u. For a linear
+    // system RK4 is the Taylor polynomial of exp(dz A) to fourth order, and
+    // here A^2 = -w I (w = ig f), so the step is
+    //   u <- c u + s ig p,  p <- c p - s f u,
+    //   c = 1 - dz^2 w/2 + dz^4 w^2/24,  s = dz (1 - dz^2 w/6).
+    auto plane = [&](float &u, float &p, float f) {
+      const float hw = dz * dz * (ig * f);
+      const float c = fmaf(hw, fmaf(hw, 1.0f / 24.0f, -0.5f), 1.0f);
+      const float s = dz * fmaf(hw, -1.0f / 6.0f, 1.0f);
+      const float u0 = u;
+      u = fmaf(c, u0, (s * ig) * p);
+      p = fmaf(c, p, -(s * f) * u0);
+    };
+    plane(x, px, fx);
+    plane(y, py, fy);
+    return;
+  }
   float kx = 0, ky = 0, kpx = 0, kpy = 0, sx = 0, sy = 0, spx = 0, spy = 0;
   const float cm[5] = {0, 0, 0.5f, 0.5f, 1};
   const float wt[5] = {0, 1, 2, 2, 1};
@@ -135,8 +153,8 @@ __device__ inline void transverse_f32(const LatArgs &l, float dz, float gam, flo
 }
 
 // push_theta_gamma_3D of one particle (particlePush.cpp:260-379): the field
-// coupling in FP32, the slippage phase ku - ks/(2 gamma^2) K2, gamma and
-// theta in FP64. One sincos per particle: the RK stages rotate it by their
+// coupling in FP32, the slippage rate ku - ks/(2 gamma0^2) K2 in FP64 with the
+// RK stages' changes of it in FP32, gamma and theta in FP64. One sincos per particle: the RK stages rotate it by their
 // small phase increments; harmonics by complex powers. On return eth_out
 // holds e^{i theta} of the new theta (when the particle was pushed: true).
 template <int NH, bool UY>
@@ -234,27 +252,34 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
   const float e_mc2 = (float)(kECharge / kEMass / kC / kC);
   const double halfPhi = l.Phi * 0.5;
   const cpx<float> rotm = expi_f32(-halfPhi);  // e^{-i Phi/2}
-  const double cmv[5] = {0, 0, 0.5, 0.5, 1};
-  const double wt[5] = {0, 1, 2, 2, 1};
+  const float cmv[5] = {0, 0, 0.5f, 0.5f, 1};
+  const float wt[5] = {0, 1, 2, 2, 1};
+  // The slippage rate ku - ks K2 / (2 gamma^2) is a difference of nearly
+  // equal terms and is formed in FP64 once, at gamma0: rate0 = ku - Ak with
+  // Ak = ks K2 / (2 gamma0^2). At a stage's gamma = gamma0 (1 + eps) the rate
+  // is rate0 + Ak (1 - r^2), r = 1 / (1 + eps), and Ak (1 - r^2) =
+  // Ak eps (2 + eps) r^2 is a small term that FP32 carries to its own
+  // relative precision, like the field terms. The stages' small terms and
+  // gamma increments are summed in FP32; theta and gamma are updated in FP64.
   // e^{i(theta0 + Phi/2)} once; stage m rotates it by e^{i del}, del = dz kth
-  // c_m (increments above 0.25 rad take the full evaluation), and takes
-  // 1/gamma from 1/gamma0 by three Newton steps (exact in FP64 for a relative
-  // energy change below 1e-2)
+  // c_m (increments above 0.25 rad take the full evaluation).
+  const double Ak = ks * 0.5 * ig0 * ig0 * K2;
+  const double rate0 = ku - Ak;
+  const float Akf = (float)Ak, rate0f = (float)rate0, ig0f = (float)ig0, dzf = (float)dz;
+  const float c20 = (float)(ks * 0.5 * ig0 * ig0);
   const cpx<float> e0 = expi_f32(th0 + halfPhi);
-  double kth = 0.0, kg = 0.0, sth = 0.0, sg = 0.0;
+  float kth = 0.f, kg = 0.f, sth = 0.f, sg = 0.f;
 #pragma unroll
   for (int m = 1; m < 5; ++m) {
-    const double del = dz * kth * cmv[m];
-    const double gm = gam0 + dz * kg * cmv[m];
+    const float del = dzf * kth * cmv[m];
     cpx<float> e1 = e0;
-    double igd = ig0;
+    float r = 1.f, eps = 0.f;
     if (m > 1) {
-      e1 = fabs(del) <= 0.25 ? e0 * expi_small((float)del) : expi_f32(th0 + del + halfPhi);
-      igd = ig0 * (2.0 - gm * ig0);
-      igd = igd * (2.0 - gm * igd);
-      igd = igd * (2.0 - gm * igd);
+      e1 = fabsf(del) <= 0.25f ? e0 * expi_small(del) : expi_f32(th0 + (double)del + halfPhi);
+      eps = dzf * kg * cmv[m] * ig0f;
+      r = __builtin_amdgcn_rcpf(1.f + eps);
     }
-    const float igm = (float)(0.5 * igd);
+    const float igm = 0.5f * ig0f * r;
     cpx<float> eh[NH];
     harm_powers<NH>(e1, a.hstep, eh);
     float sumTh = 0.f, sumG = 0.f;
@@ -266,20 +291,22 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
     }
     const cpx<float> eth = e1 * rotm;  // e^{i theta}
     const float sumEz = 2.0f * (Em.re * eth.re - Em.im * eth.im);
-    const double c2 = ks * 0.5 * igd * igd;
-    kth = (ku - c2 * K2) - (double)((float)c2 * (sumU2 + sq * sumTh));
-    kg = (double)(ks_f * sq * sumG + e_mc2 * sumEz);
-    sth += wt[m] * kth;
-    sg += wt[m] * kg;
+    const float r2 = r * r;
+    // kth - rate0
+    const float small = Akf * eps * (2.f + eps) * r2 - (c20 * r2) * (sumU2 + sq * sumTh);
+    kth = rate0f + small;
+    kg = ks_f * sq * sumG + e_mc2 * sumEz;
+    sth = fmaf(wt[m], small, sth);
+    sg = fmaf(wt[m], kg, sg);
   }
   const double dz6 = dz * (1.0 / 6.0);
-  const double dth = dz6 * sth;
+  const double dth = fma(dz, rate0, dz6 * (double)sth);
   if (dth > g.dPhi || dth < -g.dPhi) {
     atomicOr(err, EB_GUARD);
     return false;
   }
   th = th0 + dth;
-  gam = gam0 + (dz6 * sg - dz * wakeE);
+  gam = gam0 + (dz6 * (double)sg - dz * wakeE);
   eth_out = fabs(dth) <= 0.25 ? (e0 * rotm) * expi_small((float)dth) : expi_f32(th);
   return true;
 }
