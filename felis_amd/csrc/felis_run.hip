@@ -245,7 +245,12 @@ bool uniform_section(const felis_lattice *lat_tab, int k, int J) {
   return true;
 }
 
-// buffers of the jump's window passes, allocated on first use and kept
+// slot chunks of the jump's window passes
+int jump_chunks(const felis_ctx *ctx) { return (ctx->d.slice_n + kJumpChunk - 1) / kJumpChunk; }
+
+// buffers of the jump's window passes, allocated on first use and kept; box
+// holds the beam box, the push box, then the per-chunk windows of the
+// source pass and of the push-field pass
 int jump_buffers(felis_ctx *ctx) {
   if (ctx->Sw) return FELIS_OK;
   const size_t fsz = (size_t)2 * ctx->d.n_harm * ctx->S * ctx->N * ctx->esize();
@@ -253,7 +258,7 @@ int jump_buffers(felis_ctx *ctx) {
   int4 *w2 = nullptr, *bx = nullptr;
   if (hipMalloc(&sw, fsz) != hipSuccess || hipMalloc(&ub, fsz) != hipSuccess ||
       hipMalloc((void **)&w2, (size_t)ctx->S * sizeof(int4)) != hipSuccess ||
-      hipMalloc((void **)&bx, 2 * sizeof(int4)) != hipSuccess) {
+      hipMalloc((void **)&bx, (2 + 2 * (size_t)jump_chunks(ctx)) * sizeof(int4)) != hipSuccess) {
     (void)hipFree(sw);
     (void)hipFree(ub);
     (void)hipFree(w2);
@@ -478,7 +483,6 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
   hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, ctx->stream, (unsigned *)ctx->slice_e,
                      (long long)(2 * nh * (size_t)d.slice_n * sizeof(double) / sizeof(unsigned)), 0u);
   int shifted = 0;  // slots the last iteration slipped the field by
-  bool energy_full = false;  // the field changed after the slice energies were summed
   bool gate = jc != nullptr;
   int err = FELIS_OK;
   const unsigned ncb_all = (unsigned)((ctx->N + 255) / 256);
@@ -488,7 +492,7 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
     const bool wake_now = d.mode == 2 && wakeF && wake_step > 0 && it % wake_step == 0;
     // a step of a uniform undulator section after a one-slot slippage, with
     // no wake update and the jump gate shut: the captured step
-    if (use_graph && k > 0 && shifted == 1 && !energy_full && !wake_now && !gate && d.mode == 2 &&
+    if (use_graph && k > 0 && shifted == 1 && !wake_now && !gate && d.mode == 2 &&
         same_lattice(*ls, *l0) && same_lattice(*l0, *l1) && ls->current_flag && !l1->drift_flag) {
       ctx->lat_uy(l1);
       if (!sg.exec || !same_lattice(sg.key, *l1) || sg.npol != ctx->npol()) {
@@ -568,8 +572,6 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
       KTimer kt(ctx, FELIS_K_ENERGY);
       hipLaunchKernelGGL(k_reduce_diag, dim3((unsigned)(ne + nbf + 7 * ns)), dim3(256), 0, ctx->stream, g,
                          ctx->slice_e, shifted, tile_red, nred, species_tile, ns, row, (const int *)nullptr, 0);
-      if (energy_full && (err = ctx->energy(row))) break;
-      energy_full = false;
     }
     // jump gate: J periods left in one undulator section with no wake update
     // inside, the jump budget not spent, and the fundamental bunching of the
@@ -619,17 +621,23 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
     if (ls->current_flag && (err = ctx->ez_solve(ls))) break;
     GridArgs gw = ctx->grid();
     const unsigned ncb = (unsigned)((ctx->N + 255) / 256);
-    const unsigned nchunk = (unsigned)((d.slice_n + kJumpChunk - 1) / kJumpChunk);
+    const unsigned nchunk = (unsigned)jump_chunks(ctx);
+    int4 *cwin_src = jump ? ctx->box + 2 : nullptr, *cwin_push = jump ? ctx->box + 2 + nchunk : nullptr;
     if (jump) {
       // source of the J slices each field slot passes, then the ADI over J dz
       hipLaunchKernelGGL(k_win_bbox, dim3(1), dim3(256), 0, ctx->stream, ctx->win, d.slice_n, d.nx, d.ny, 0,
                          ctx->box);
       hipLaunchKernelGGL(k_win_union, dim3((unsigned)((ctx->S + 255) / 256)), dim3(256), 0, ctx->stream, ctx->win,
                          d.slice_n, J, ctx->win2);
+      hipLaunchKernelGGL(k_chunk_windows, dim3((nchunk + 63) / 64), dim3(64), 0, ctx->stream, ctx->win2, d.slice_n, 0,
+                         d.nx, d.ny, cwin_src);
+      hipLaunchKernelGGL(k_chunk_windows, dim3((nchunk + 63) / 64), dim3(64), 0, ctx->stream, ctx->win, d.slice_n,
+                         kJumpMargin, d.nx, d.ny, cwin_push);
       {
         KTimer kt(ctx, FELIS_K_JUMP_WINDOW);
         hipLaunchKernelGGL(k_src_window, dim3(ncb, ctx->npol() * nh, nchunk), dim3(256), 0, ctx->stream, d.nx, d.ny, ctx->S,
-                           d.slice_n, J, ctx->box, ctx->win, (const cpx<float> *)ctx->Sc, (cpx<float> *)ctx->Sw);
+                           d.slice_n, J, ctx->box, cwin_src, ctx->win, (const cpx<float> *)ctx->Sc,
+                           (cpx<float> *)ctx->Sw);
       }
       // the learned correction: per-slice inputs -> network -> emission factor
       if (jc->use_net) {
@@ -643,18 +651,18 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
                          kJumpMargin, ctx->box + 1);
       {
         KTimer kt(ctx, FELIS_K_JUMP_WINDOW);
-        hipLaunchKernelGGL(k_field_window, dim3(ncb, ctx->npol() * nh, nchunk), dim3(256), 0, ctx->stream, d.nx, d.ny,
-                           ctx->S, d.slice_n, gw.ring_base, J, ctx->box + 1, (const cpx<float> *)ctx->U,
-                           (cpx<float> *)ctx->Ub, 0);
+        hipLaunchKernelGGL(k_field_window<0>, dim3(ncb, ctx->npol() * nh, nchunk), dim3(256), 0, ctx->stream, d.nx,
+                           d.ny, ctx->S, d.slice_n, gw.ring_base, J, ctx->box + 1, cwin_push, (const cpx<float> *)ctx->U,
+                           (cpx<float> *)ctx->Ub);
       }
       // the field solve keeps the windowed source: the correction and the
       // training residual read it after the push
       if ((err = launch_adi_scan(ctx, jc->tab, 2, (cpx<float> *)ctx->Sw, ctx->win2))) break;
       {
         KTimer kt(ctx, FELIS_K_JUMP_WINDOW);
-        hipLaunchKernelGGL(k_field_window, dim3(ncb, ctx->npol() * nh, nchunk), dim3(256), 0, ctx->stream, d.nx, d.ny,
-                           ctx->S, d.slice_n, gw.ring_base, J, ctx->box + 1, (const cpx<float> *)ctx->U,
-                           (cpx<float> *)ctx->Ub, 1);
+        hipLaunchKernelGGL(k_field_window<1>, dim3(ncb, ctx->npol() * nh, nchunk), dim3(256), 0, ctx->stream, d.nx,
+                           d.ny, ctx->S, d.slice_n, gw.ring_base, J, ctx->box + 1, cwin_push, (const cpx<float> *)ctx->U,
+                           (cpx<float> *)ctx->Ub);
       }
     } else if (fused) {
       if ((err = launch_adi_scan(ctx, adi_tab, ls->current_flag, (cpx<float> *)ctx->Sc, ctx->win))) break;
@@ -686,9 +694,9 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
       if (jc->use_net) {
         GridArgs gc = ctx->grid();
         KTimer kt(ctx, FELIS_K_JUMP_WINDOW);
-        hipLaunchKernelGGL(k_emit_correct, dim3(ncb, ctx->npol() * nh, nchunk), dim3(256), 0, ctx->stream, gc, ctx->box,
-                           (const cpx<float> *)jc->gain, (const cpx<float> *)ctx->Sw, (cpx<float> *)ctx->U);
-        energy_full = true;
+        hipLaunchKernelGGL(k_emit_correct, dim3(d.slice_n, ctx->npol() * nh), dim3(256), 0, ctx->stream, gc, ctx->win2,
+                           (const cpx<float> *)jc->gain, (const cpx<float> *)ctx->Sw, (cpx<float> *)ctx->U,
+                           ctx->slice_e);
       }
       jc->jumps++;
       ctx->last_jump = J;

@@ -5,6 +5,7 @@
 #pragma once
 
 #include "felis_common.h"
+#include "felis_jump.h"
 #include "felis_pstep.h"
 
 namespace felis {
@@ -43,12 +44,15 @@ __device__ inline void block_sums_store(double (&acc)[NQ], double *out) {
 }
 
 // One 256-thread workgroup per slice over the particles of every species.
-template <int NH>
+// UY = false: Uy is exactly zero (see k_deposit_slot), its sums are zero and
+// it is not read. e^{iH theta} is a power of e^{i theta} in FP32.
+template <int NH, bool UY>
 static __global__ void __launch_bounds__(256)
     k_slice_moments(GridArgs g, int n_species, const long long *__restrict__ seg_off,
                     const double *__restrict__ seg_w, const double *__restrict__ theta,
                     const float *__restrict__ gamma, const float *__restrict__ x, const float *__restrict__ y,
-                    const cpx<float> *__restrict__ Ux, const cpx<float> *__restrict__ Uy, double *__restrict__ out) {
+                    const cpx<float> *__restrict__ Ux, const cpx<float> *__restrict__ Uy, double inv_dx,
+                    double inv_dy, double *__restrict__ out) {
   constexpr int NQ = 9 + 8 * NH;
   const int sl = blockIdx.x, slot = sl + 1;
   double acc[NQ];
@@ -67,7 +71,7 @@ static __global__ void __launch_bounds__(256)
       acc[2] += w * dg * dg;
       acc[3] += w * dg * e1.re;
       acc[4] += w * dg * e1.im;
-      const double fx_ = ((double)x[n] - g.minX) / g.dx, fy_ = ((double)y[n] - g.minY) / g.dy;
+      const double fx_ = ((double)x[n] - g.minX) * inv_dx, fy_ = ((double)y[n] - g.minY) * inv_dy;
       const int i = (int)fx_, j = (int)fy_;
       const bool in = i >= 0 && i < g.nx - 1 && j >= 0 && j < g.ny - 1;
       const float wx1 = in ? (float)(fx_ - i) : 0.f, wy1 = in ? (float)(fy_ - j) : 0.f;
@@ -75,7 +79,7 @@ static __global__ void __launch_bounds__(256)
       const long long c0 = in ? pbase + (long long)j * g.nx + i : 0;
 #pragma unroll
       for (int h = 0; h < NH; ++h) {
-        const cpx<float> eh = g.harm[h] == 1 ? e1 : expi_f32(th * g.harm[h]);
+        const cpx<float> eh = g.harm[h] == 1 ? e1 : cpow_int(e1, g.harm[h]);
         acc[5 + 8 * h] += w * eh.re;
         acc[6 + 8 * h] += w * eh.im;
         if (in) {
@@ -83,8 +87,9 @@ static __global__ void __launch_bounds__(256)
           const cpx<float> *b0 = Uy + (long long)h * g.S * g.N + c0;
           const cpx<float> ux =
               a0[0] * (wx0 * wy0) + a0[g.nx] * (wx0 * wy1) + a0[1] * (wx1 * wy0) + a0[g.nx + 1] * (wx1 * wy1);
-          const cpx<float> uy =
-              b0[0] * (wx0 * wy0) + b0[g.nx] * (wx0 * wy1) + b0[1] * (wx1 * wy0) + b0[g.nx + 1] * (wx1 * wy1);
+          const cpx<float> uy = UY ? b0[0] * (wx0 * wy0) + b0[g.nx] * (wx0 * wy1) + b0[1] * (wx1 * wy0) +
+                                         b0[g.nx + 1] * (wx1 * wy1)
+                                   : mk(0.f, 0.f);
           const cpx<float> px = ux * eh, py = uy * eh;
           acc[7 + 8 * h] += w * px.re;
           acc[8 + 8 * h] += w * px.im;
@@ -182,11 +187,13 @@ static __global__ void __launch_bounds__(256) k_field_cmp(GridArgs g, const cpx<
 
 // Per (pol x harmonic, slice) system after a jump of J periods: the jump's
 // emission E = 2 Sw[s - J] / H of the field now at slot s = slice + 1 (Sw is
-// indexed by the slot the field was solved at) against the difference of
-// state b and state a there: sum conj(E) (b - a) (re, im) and sum |E|^2.
+// indexed by the slot the field was solved at, and holds it only inside
+// that slot's window win2) against the difference of state b and state a
+// there: sum conj(E) (b - a) (re, im) and sum |E|^2.
 static __global__ void __launch_bounds__(256) k_emit_cmp(GridArgs g, int J, const cpx<float> *__restrict__ A,
                                                          const cpx<float> *__restrict__ B,
-                                                         const cpx<float> *__restrict__ Sw, double *__restrict__ out) {
+                                                         const cpx<float> *__restrict__ Sw,
+                                                         const int4 *__restrict__ win2, double *__restrict__ out) {
   const int sys = blockIdx.x, sl = sys % g.slice_n, ph = sys / g.slice_n, h = ph % g.nh;
   const int s = sl + 1, s0 = s - J;
   double acc[3] = {0.0, 0.0, 0.0};
@@ -194,8 +201,10 @@ static __global__ void __launch_bounds__(256) k_emit_cmp(GridArgs g, int J, cons
     const long long off = ((long long)ph * g.S + phys_slot(s, g.ring_base, g.S)) * g.N;
     const cpx<float> *src = Sw + ((long long)ph * g.S + s0) * g.N;
     const double k2 = 2.0 / g.harm[h];
+    const int4 w = win2[s0];
     for (long long k = threadIdx.x; k < g.N; k += blockDim.x) {
-      const cpx<float> a = A[off + k], b = B[off + k], e = src[k];
+      const int i = (int)(k % g.nx), j = (int)(k / g.nx);
+      const cpx<float> a = A[off + k], b = B[off + k], e = in_box(w, i, j) ? src[k] : mk(0.f, 0.f);
       const double er = k2 * e.re, ei = k2 * e.im;
       const double dr = (double)b.re - a.re, di = (double)b.im - a.im;
       acc[0] += er * dr + ei * di;

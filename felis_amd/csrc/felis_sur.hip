@@ -42,10 +42,16 @@ void part_arrays(felis_ctx *ctx, void **p, size_t *bytes) {
 template <int NH>
 void launch_moments(felis_ctx *ctx, double *out) {
   GridArgs g = ctx->grid();
-  hipLaunchKernelGGL((k_slice_moments<NH>), dim3(ctx->d.slice_n), dim3(256), 0, ctx->stream, g, ctx->d.n_species,
-                     ctx->seg_off, ctx->seg_w, ctx->theta, (const float *)ctx->gamma, (const float *)ctx->x,
-                     (const float *)ctx->y, (const cpx<float> *)ctx->U,
-                     (const cpx<float> *)((char *)ctx->U + ctx->pol_bytes()), out);
+  const cpx<float> *Ux = (const cpx<float> *)ctx->U, *Uy = (const cpx<float> *)((char *)ctx->U + ctx->pol_bytes());
+  const double inv_dx = 1.0 / ctx->d.dx, inv_dy = 1.0 / ctx->d.dy;
+  if (ctx->uy_live)
+    hipLaunchKernelGGL((k_slice_moments<NH, true>), dim3(ctx->d.slice_n), dim3(256), 0, ctx->stream, g,
+                       ctx->d.n_species, ctx->seg_off, ctx->seg_w, ctx->theta, (const float *)ctx->gamma,
+                       (const float *)ctx->x, (const float *)ctx->y, Ux, Uy, inv_dx, inv_dy, out);
+  else
+    hipLaunchKernelGGL((k_slice_moments<NH, false>), dim3(ctx->d.slice_n), dim3(256), 0, ctx->stream, g,
+                       ctx->d.n_species, ctx->seg_off, ctx->seg_w, ctx->theta, (const float *)ctx->gamma,
+                       (const float *)ctx->x, (const float *)ctx->y, Ux, Uy, inv_dx, inv_dy, out);
 }
 
 }  // namespace
@@ -234,7 +240,7 @@ int felis_jump_emission_residual(felis_ctx *ctx, int slot, double *out) {
   SCHK(hipMalloc((void **)&dev, (size_t)nsys * 3 * sizeof(double)));
   GridArgs g = ctx->grid();
   hipLaunchKernelGGL(k_emit_cmp, dim3(nsys), dim3(256), 0, ctx->stream, g, ctx->last_jump,
-                     (const cpx<float> *)ctx->U, (const cpx<float> *)sn.U, (const cpx<float> *)ctx->Sw, dev);
+                     (const cpx<float> *)ctx->U, (const cpx<float> *)sn.U, (const cpx<float> *)ctx->Sw, ctx->win2, dev);
   int e = FELIS_OK;
   if (hipGetLastError() != hipSuccess ||
       hipMemcpyAsync(out, dev, (size_t)nsys * 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream) !=

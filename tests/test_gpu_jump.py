@@ -11,7 +11,8 @@ Parity and tolerances:
     numpy restatement from the downloaded state to FP32 rounding (1e-4
     relative of each sum's scale);
   - a network whose outputs are a constant (log|e|, arg e) adds exactly
-    (e - 1) times the jump's emission to its field (1e-4);
+    (e - 1) times the jump's emission to its field (1e-4), and the run's
+    energy after such a jump is the field's (1e-6);
   - jumps of 5 periods follow the fine run's field energy within 10% over
     the first 100 periods (the shot-noise-dominated start, the jump's
     largest error without a correction).
@@ -237,6 +238,38 @@ def test_constant_network_scales_the_jump_emission():
         assert live.sum() > d.slice_n // 2
         r = (fx[live, 0] + 1j * fx[live, 1]) / fx[live, 2]
         assert np.max(np.abs(r - (e - 1))) < 1e-4 * abs(e - 1), h
+    net.close()
+    ctx.close()
+
+
+def test_corrected_jump_reports_the_field_energy():
+    """The corrected jump adds the correction's energy change to its slot
+    energies instead of summing the field again: the energy a run reports
+    for the state after the jump (the next step's row) equals the energy
+    summed from that field, with and without the network. The field comes
+    from a second run from the same state, which the deposit's float atomics
+    sum in another order: 1e-6 relative."""
+    deck, d, ctx = _ctx()
+    nh = d.n_harm
+    F = 5 + 5 * nh
+    net = fl.MLP([F, 16, 2 * nh], capacity=d.slice_n, seed=3)
+    net.set_layer(0, np.zeros((F, 16), np.float32), np.zeros(16, np.float32))
+    bias = np.array([v for h in range(nh) for v in (0.1 * (h + 1), 0.3 - 0.2 * h)], np.float32)
+    net.set_layer(1, np.zeros((16, 2 * nh), np.float32), bias)
+    tab = deck.lattice(0, 16)
+    ctx.run(tab[:21], 0, 10, outputs=False)
+    ctx.save_state(0)
+    tab2 = tab[20:33]
+    for use_net in (True, False):
+        kw = dict(periods=5, bunching_max=math.inf, max_jumps=1, net=net if use_net else None)
+        ctx.restore_state(0)
+        _, _, _, k = ctx.run_surrogate(tab2[:11], 10, 5, **kw)
+        assert k[0] == 1
+        e_field = ctx.slice_energy().sum()  # the field after the jump
+        ctx.restore_state(0)
+        e, _, _, k = ctx.run_surrogate(tab2, 10, 6, **kw)
+        assert k[0] == 1 and k[5] == 0
+        assert abs(e[5].sum() / e_field - 1) < 1e-6, use_net
     net.close()
     ctx.close()
 
