@@ -166,12 +166,15 @@ def gpu_parity(args, g, cpu):
     return float(np.max(np.abs(tot_g - tot_o) / np.abs(tot_o)))
 
 
-def surrogate_runs(args, periods: int = 10, gate: float = 0.05):
+def surrogate_runs(args, periods: int = 10, gate: float = 0.1, net_gate: float = 0.05):
     """The whole run of the deck (every period of the undulator) three ways
     from the same initial state: the batched step; jumps of `periods` periods
-    while |b1| < gate; the same with the committed correction network. Wall
-    times of the step loop and the relative error of the total field energy
-    (the integrated FEL power) against the batched run."""
+    while |b1| < gate; the same with the committed correction network (applied
+    while |b1| < net_gate, the range it was trained on). Wall times of the
+    step loop and the relative error of the total field energy (the
+    integrated FEL power) against the batched run. For the 2048-slice
+    workload also the same beam in a 40 m undulator, batched against jumps
+    with the network, to saturation (the first maximum of the power)."""
     from felis_amd import _lib as fl
     from felis_amd import surrogate as su
     deck = ROOT / "felis_amd" / "decks" / f"{args.config}.inp"
@@ -182,7 +185,30 @@ def surrogate_runs(args, periods: int = 10, gate: float = 0.05):
     rf = su.run_curves(deck, seed=args.seed)
     rj = su.run_curves(deck, seed=args.seed, periods=periods, gate=gate)
     net = model.network(capacity=slices)
-    rn = su.run_curves(deck, seed=args.seed, periods=periods, gate=gate, net=net)
+    rn = su.run_curves(deck, seed=args.seed, periods=periods, gate=gate, net=net, net_gate=net_gate)
+    sat = None
+    if args.config == "cfg2":
+        import tempfile
+        from felis_amd.decks import make_decks as md
+        with tempfile.TemporaryDirectory() as td:
+            p40 = Path(td) / "cfg2_40m.inp"
+            p40.write_text(md.deck(**dict(md.DECKS["cfg2.inp"], length=40.0)))
+            dk40 = fl.Deck(p40)
+            dzs = dk40.desc(fl.MIXED_FP32).dz  # m per step
+            dk40.close()
+            sf = su.run_curves(p40, seed=args.seed)
+            sn = su.run_curves(p40, seed=args.seed, periods=periods, gate=gate, net=net, net_gate=net_gate)
+        z = np.arange(sf["steps"]) * dzs
+        zs, Ps, found = su.saturation(z, sf["E"])
+        ok = ~np.isnan(sn["E"])
+        idx = np.nonzero(ok)[0]
+        En = sn["E"].copy()
+        En[~ok] = np.interp(np.nonzero(~ok)[0], idx, sn["E"][idx])
+        zj, Pj, _ = su.saturation(z, En)
+        sat = {"undulator_m": 40.0, "periods": int(sf["steps"]), "saturated": bool(found),
+               "batched_seconds": sf["seconds"], "jump_learned_seconds": sn["seconds"],
+               "speedup_of_batched": sf["seconds"] / sn["seconds"], "jumps": int(np.sum(sn["kind"] == 1)),
+               "z_sat_m": zs, "z_sat_rel_err": zj / zs - 1, "power_sat_rel_err": Pj / Ps - 1}
     net.close()
 
     def err(r):
@@ -192,9 +218,9 @@ def surrogate_runs(args, periods: int = 10, gate: float = 0.05):
                 "jumps": int(np.sum(r["kind"] == 1)), "periods_in_jumps": int(np.sum(r["kind"] > 0)),
                 "final_power_rel_err": float(r["E"][-1] / rf["E"][-1] - 1), "max_power_rel_err": float(
                     np.max(np.abs(rel)))}
-    return {"periods_per_jump": periods, "bunching_gate": gate, "periods": int(rf["steps"]),
-            "batched_seconds": rf["seconds"], "jump": err(rj), "jump_learned": err(rn),
-            "model": "felis_amd/models/jump_corr_J10.npz", "model_meta": model.meta}
+    return {"periods_per_jump": periods, "bunching_gate": gate, "network_bunching_gate": net_gate,
+            "periods": int(rf["steps"]), "batched_seconds": rf["seconds"], "jump": err(rj), "jump_learned": err(rn),
+            "saturation": sat, "model": "felis_amd/models/jump_corr_J10.npz", "model_meta": model.meta}
 
 
 def calibrate_copy(mib: int):
