@@ -10,7 +10,8 @@
 //                  deposit (solveField.cpp:246-312) into an LDS window over the
 //                  slice's beam, stored once, and the radial space-charge
 //                  source of the last species (solveField.cpp:322-359).
-//   k_reduce_diag  the step's diagnostics from the partial sums, in a fixed
+//   k_reduce_parts, k_reduce_final
+//                  the step's diagnostics from the partial sums, in a fixed
 //                  order.
 #pragma once
 
@@ -756,19 +757,23 @@ __global__ void __launch_bounds__(1024)
 
 // out[o] for o < 2*nh: energy of pol o/nh, harmonic o%nh over the slices that
 // stay after the slippage; next 2*nh+1: bFactor sums; then 7 twiss sums per
-// species. tile_red holds moment q of tile t at [q * ntiles + t]. One
-// workgroup per output: strided partial sums (four independent chains per
-// thread), then a fixed tree.
+// species. tile_red holds moment q of tile t at [q * ntiles + t]. In two
+// passes with a fixed order: np (at most kRedParts) workgroups per output
+// each sum a contiguous part of its row (strided partial sums, four
+// independent chains per thread, then a fixed tree) into parts[o][p]; one
+// thread per output then adds its parts in order (one part: the first pass
+// writes the output).
+constexpr int kRedParts = 16;
+
 static __global__ void __launch_bounds__(256)
-    k_reduce_diag(GridArgs g, const double *__restrict__ slice_e, int skip, const double *__restrict__ tile_red,
-                  int nred, const int *__restrict__ species_tile, int n_species, double *__restrict__ out,
-                  const int *__restrict__ step_dev, int per) {
+    k_reduce_parts(GridArgs g, const double *__restrict__ slice_e, int skip, const double *__restrict__ tile_red,
+                   const int *__restrict__ species_tile, int n_species, double *__restrict__ parts,
+                   double *__restrict__ out, const int *__restrict__ step_dev, int per) {
   __shared__ double sh[256];
-  if (step_dev) out += (long long)(*step_dev) * per;  // a graph-captured step: the row of the current step
-  const int o = blockIdx.x, tid = threadIdx.x, nh = g.nh;
+  const int o = blockIdx.x, p = blockIdx.y, np = gridDim.y, tid = threadIdx.x, nh = g.nh;
   const long long ntiles = species_tile[n_species];
   const double *row;
-  int k0, k1;
+  long long k0, k1;
   if (o < 2 * nh) {
     row = slice_e + (long long)o * g.slice_n;
     k0 = 0;
@@ -776,30 +781,48 @@ static __global__ void __launch_bounds__(256)
   } else if (o < 4 * nh + 1) {
     row = tile_red + (long long)(o - 2 * nh) * ntiles;
     k0 = 0;
-    k1 = (int)ntiles;
+    k1 = ntiles;
   } else {
     const int q = o - (4 * nh + 1), sp = q / 7, c = q % 7;
     row = tile_red + (long long)(2 * nh + 1 + c) * ntiles;
     k0 = species_tile[sp];
     k1 = species_tile[sp + 1];
   }
+  const long long len = k1 > k0 ? k1 - k0 : 0;
+  const long long a = k0 + len * p / np, b = k0 + len * (p + 1) / np;
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  int k = k0 + tid;
-  for (; k + 768 < k1; k += 1024) {
+  long long k = a + tid;
+  for (; k + 768 < b; k += 1024) {
     s0 += row[k];
     s1 += row[k + 256];
     s2 += row[k + 512];
     s3 += row[k + 768];
   }
-  for (; k < k1; k += 256) s0 += row[k];
+  for (; k < b; k += 256) s0 += row[k];
   sh[tid] = (s0 + s1) + (s2 + s3);
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if (tid < w) sh[tid] += sh[tid + w];
     __syncthreads();
   }
-  if (tid == 0) out[o] = sh[0];
-  (void)nred;
+  if (tid == 0) {
+    if (np > 1) {
+      parts[(long long)o * kRedParts + p] = sh[0];
+    } else {  // one part: the output itself
+      if (step_dev) out += (long long)(*step_dev) * per;  // a graph-captured step: the row of the current step
+      out[o] = sh[0];
+    }
+  }
+}
+
+static __global__ void k_reduce_final(int nout, int np, const double *__restrict__ parts, double *__restrict__ out,
+                                      const int *__restrict__ step_dev, int per) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= nout) return;
+  if (step_dev) out += (long long)(*step_dev) * per;  // a graph-captured step: the row of the current step
+  double s = 0.0;
+  for (int p = 0; p < np; ++p) s += parts[(long long)o * kRedParts + p];
+  out[o] = s;
 }
 
 }  // namespace felis

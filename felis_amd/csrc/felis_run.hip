@@ -197,6 +197,24 @@ int deposit_slot(felis_ctx *ctx, const DepArgs &dep, int lds) {
   NH_DISPATCH(launch_deposit_slot, ctx->d.n_harm, ctx, dep, lds)
 }
 
+// the step's diagnostics row (energy, bFactor, twiss) from the slice
+// energies and the tile moments: parts, then their fixed-order sums
+void reduce_diag(felis_ctx *ctx, int skip, const double *tile_red, const int *species_tile, double *parts,
+                 double *out, const int *step_dev, int per) {
+  const felis_desc &d = ctx->d;
+  const int nout = 4 * d.n_harm + 1 + 7 * d.n_species;
+  // parts per output: one workgroup per output is latency-bound on long
+  // rows (measured: 16 parts save 24 us a step at 45,600 tiles, and cost
+  // 1.4 us at 4,100)
+  const int np = ctx->ntiles >= 16384 ? kRedParts : 1;
+  GridArgs g = ctx->grid();
+  hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)nout, (unsigned)np), dim3(256), 0, ctx->stream, g, ctx->slice_e,
+                     skip, tile_red, species_tile, d.n_species, parts, out, step_dev, per);
+  if (np > 1)
+    hipLaunchKernelGGL(k_reduce_final, dim3((unsigned)((nout + 63) / 64)), dim3(64), 0, ctx->stream, nout, np, parts,
+                       out, step_dev, per);
+}
+
 template <int NH>
 hipError_t set_deposit_lds(int lds) {
   hipError_t e = hipFuncSetAttribute((const void *)k_deposit_slot<NH, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -389,7 +407,7 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
   }
   const int nh = d.n_harm, ns = d.n_species;
   const int nred = 2 * nh + 1 + 7;
-  double *tile_red = nullptr, *scratch_e = nullptr;
+  double *tile_red = nullptr, *scratch_e = nullptr, *red_parts = nullptr;
   int *species_tile = nullptr;
   cpx<float> *adi_tab = nullptr;
   std::vector<cpx<float>> h_tab;
@@ -397,6 +415,7 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
   auto cleanup = [&]() {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(tile_red);
+    (void)hipFree(red_parts);
     (void)hipFree(species_tile);
     (void)hipFree(adi_tab);
     (void)hipFree(scratch_e);
@@ -413,6 +432,7 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
       hipMalloc((void **)&species_tile, (ns + 1) * sizeof(int)) != hipSuccess ||
       hipMalloc((void **)&adi_tab, h_tab.size() * sizeof(cpx<float>)) != hipSuccess ||
       hipMalloc((void **)&scratch_e, 2 * nh * sizeof(double)) != hipSuccess ||
+      hipMalloc((void **)&red_parts, (size_t)(4 * nh + 1 + 7 * ns) * kRedParts * sizeof(double)) != hipSuccess ||
       hipMemcpy(adi_tab, h_tab.data(), h_tab.size() * sizeof(cpx<float>), hipMemcpyHostToDevice) != hipSuccess) {
     cleanup();
     ctx->err = "batched step buffers";
@@ -511,9 +531,7 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
           break;
         }
         ctx->ring_dev_active = ring_dev;
-        GridArgs g = ctx->grid();
-        hipLaunchKernelGGL(k_reduce_diag, dim3((unsigned)(ne + nbf + 7 * ns)), dim3(256), 0, ctx->stream, g,
-                           ctx->slice_e, 1, tile_red, nred, species_tile, ns, dev_out, step_dev, (int)per);
+        reduce_diag(ctx, 1, tile_red, species_tile, red_parts, dev_out, step_dev, (int)per);
         int ce = ctx->ez_solve(ls);
         if (!ce) ce = launch_adi_scan(ctx, adi_tab, 1, (cpx<float> *)ctx->Sc, ctx->win);
         if (!ce) ce = particles(ctx, a, tile_red, ctx->U);
@@ -568,10 +586,8 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
         if ((err = dep_args(ctx, ls, &dep)) || (err = deposit_slot(ctx, dep, dep_lds))) break;
       }
     } else {
-      GridArgs g = ctx->grid();
       KTimer kt(ctx, FELIS_K_ENERGY);
-      hipLaunchKernelGGL(k_reduce_diag, dim3((unsigned)(ne + nbf + 7 * ns)), dim3(256), 0, ctx->stream, g,
-                         ctx->slice_e, shifted, tile_red, nred, species_tile, ns, row, (const int *)nullptr, 0);
+      reduce_diag(ctx, shifted, tile_red, species_tile, red_parts, row, nullptr, 0);
     }
     // jump gate: J periods left in one undulator section with no wake update
     // inside, the jump budget not spent, and the fundamental bunching of the
