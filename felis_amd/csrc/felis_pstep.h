@@ -581,8 +581,21 @@ __global__ void __launch_bounds__(1024)
       }                                                                                                         \
     }                                                                                                           \
   }
+      // the next particle's loads are issued before this one is deposited, so
+      // each thread has a particle in flight while it works (measured: the
+      // deposit 9% faster; two ahead is slower, at 128 registers)
+      float xq = x[r0], yq = y[r0], gq = gamma[r0];
+      double thq = theta[r0];
       for (long long n = r0; n < r1; ++n) {
-        const double fx_ = ((double)x[n] - g.minX) * d.inv_dx, fy_ = ((double)y[n] - g.minY) * d.inv_dy;
+        const float xn = xq, yn = yq, gn = gq;
+        const double thn = thq;
+        if (n + 1 < r1) {
+          xq = x[n + 1];
+          yq = y[n + 1];
+          gq = gamma[n + 1];
+          thq = theta[n + 1];
+        }
+        const double fx_ = ((double)xn - g.minX) * d.inv_dx, fy_ = ((double)yn - g.minY) * d.inv_dy;
         const int idxI = (int)fx_, idxJ = (int)fy_;
         if (!(idxI >= 0 && idxI < g.nx && idxJ >= 0 && idxJ < g.ny)) continue;
         const int k2 = idxJ * g.nx + idxI;
@@ -600,8 +613,8 @@ __global__ void __launch_bounds__(1024)
         float wx[2], wy[2];
         wx[1] = (float)(fx_ - idxI); wx[0] = 1.f - wx[1];
         wy[1] = (float)(fy_ - idxJ); wy[0] = 1.f - wy[1];
-        const double th = theta[n];
-        const float amp0 = __fdividef(macro_u, (float)(g.gref + (double)gamma[n]));
+        const double th = thn;
+        const float amp0 = __fdividef(macro_u, (float)(g.gref + (double)gn));
         float xi2 = 0.f;
         cpx<float> expP = mk(1.f, 0.f);
         if (any_even) {
