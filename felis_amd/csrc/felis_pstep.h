@@ -453,9 +453,16 @@ __global__ void __launch_bounds__(1024)
   for (int sp = 0; sp < n_species; ++sp) {
     const int seg = sp * g.slice_n + sl;
     const long long b = seg_off[seg], e = seg_off[seg + 1];
+    // the next iteration's loads in flight while this one is binned
+    float xq = b + tid < e ? x[b + tid] : 0.f, yq = b + tid < e ? y[b + tid] : 0.f;
     for (long long n = b + tid; n < e; n += nt) {
+      const float xn = xq, yn = yq;
+      if (n + nt < e) {
+        xq = x[n + nt];
+        yq = y[n + nt];
+      }
       // the same cell index as the deposit below
-      const int idxI = (int)(((double)x[n] - g.minX) * d.inv_dx), idxJ = (int)(((double)y[n] - g.minY) * d.inv_dy);
+      const int idxI = (int)(((double)xn - g.minX) * d.inv_dx), idxJ = (int)(((double)yn - g.minY) * d.inv_dy);
       if (idxI >= 0 && idxI < g.nx - 1 && idxJ >= 0 && idxJ < g.ny - 1) {
         mi = min(mi, idxI); Mi = max(Mi, idxI); mj = min(mj, idxJ); Mj = max(Mj, idxJ);
       }
