@@ -726,8 +726,8 @@ __global__ void __launch_bounds__(1024)
         const int idxR = (int)(rr * inv_dr + 0.5);
         const cpx<float> em = conjg(expi_f32(theta[n]));  // e^{-i theta}
         if (idxR > 0 && idxR < nr) {
-          const double amp = d.coef_Ez * macro / (1.0 * idxR);
           if (F > 1) {
+            const double amp = d.coef_Ez * macro / (1.0 * idxR);
             // azimuthal modes need the angle of every particle: no combining
             FELIS_RADIAL_FLUSH();
             key = -1;
@@ -748,11 +748,15 @@ __global__ void __launch_bounds__(1024)
             FELIS_RADIAL_FLUSH();
             key = idxR;
             sum[0] = mk(0.0, 0.0);
-            wr1 = (rr * rr * inv_dr * inv_dr - idxR * idxR) / (2.0 * idxR);
-            wr0 = 1.0 - wr1;
+            // the bin's amplitude coef_Ez macro / idxR, the same for the whole
+            // run, goes into the run's radial weights
+            const double amp = d.coef_Ez * macro / (1.0 * idxR);
+            const double w1 = (rr * rr * inv_dr * inv_dr - idxR * idxR) / (2.0 * idxR);
+            wr1 = amp * w1;
+            wr0 = amp * (1.0 - w1);
           }
           // i e^{-i theta} = (sin theta, cos theta) = (-em.im, em.re)
-          sum[0] = sum[0] + mk(-(double)em.im, (double)em.re) * amp;
+          sum[0] = sum[0] + mk(-(double)em.im, (double)em.re);
         } else if (idxR == 0) {
           if (key != 0) {
             FELIS_RADIAL_FLUSH();
