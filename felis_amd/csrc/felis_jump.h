@@ -14,10 +14,12 @@
 //                                           each at its time in the jump
 //                                           (logical slots < 1 hold no field)
 //
-// Both run over slots in chunks with a halo, one thread per (cell, pol x
-// harmonic, chunk), with the running sum in FP64; only the bounding box of
-// the beam's deposit windows is visited, its cells taken by consecutive threads (the source of a slot is its deposit
-// window, and the push reads the field only around its particles).
+// Both run over slots in chunks of kJumpChunk with a halo, one thread per
+// (cell, pol x harmonic, chunk), with the running sum in FP64. Only the
+// bounding box of the beam's deposit windows is visited, its cells taken by
+// consecutive threads, and of a chunk only the cells inside the union of the
+// windows its slots are read in (the source of a slot is its deposit window,
+// and the push reads the field only around its particles).
 #pragma once
 
 #include "felis_common.h"
