@@ -22,8 +22,22 @@ namespace felis {
 
 constexpr int kRun = 8;  // consecutive particles one deposit thread combines
 
+// transversePush constants of one lattice row (particlePush.cpp:76-90),
+// formed on the host: the same for every particle of a launch
+struct TransArgs {
+  float aa, bb, cc, ddc, ue, xCoef, yCoef;  // dd = ddc ig^2 ue
+  int ka;
+};
+
 struct PStepArgs {
   LatArgs l0, l1;  // transversePush at (it,0); everything else at (it,0.5)
+  TransArgs t0, t1;
+  // push_theta_gamma constants of l1 (particlePush.cpp:260-300), formed on
+  // the host: K2 = 1 + p^2 + K2a (1 + K2b r^2), xi = xic / gamma0^2,
+  // xi2 = xi2c |p|_ue, sq = K0 sqrt(1 + ue^2), rotm = e^{-i Phi/2}
+  double K2a, K2b, xic, halfPhi;
+  float xi2c, sq, kaf;
+  cpx<float> rotm, etaX, etaY;
   int drift;       // driftFlag of (it,0.5)
   int nred;        // partial-sum row length: 2*nh + 1 + 7
   double gamR;
@@ -73,7 +87,7 @@ __device__ inline cpx<T> cpow_int(cpx<T> b, int n) {
 // e^{i psi} in FP32 with the reduction in FP64: psi = q pi/2 + r with
 // |r| <= pi/4, then the Taylor series of sin r (to r^9) and cos r (to r^10),
 // whose truncation errors there are below 2e-9, and the quadrant's swap.
-__device__ inline cpx<float> expi_f32(double psi) {
+__host__ __device__ inline cpx<float> expi_f32(double psi) {
   const double q = rint(psi * 0.6366197723675814);
   const float r = (float)fma(-q, 1.5707963267948966, psi);
   const float r2 = r * r;
@@ -94,24 +108,34 @@ __device__ inline cpx<float> expi_small(float d) {
 }
 
 // transversePush of one particle in FP32 (particlePush.cpp:76-111)
-__device__ inline void transverse_f32(const LatArgs &l, float dz, float gam, float &x, float &y, float &px,
-                                      float &py) {
+__host__ inline TransArgs trans_args(const LatArgs &l) {
+  TransArgs t;
   const float ku = (float)l.ku, K0 = (float)l.K0, ue = (float)l.ue;
-  const float aa = K0 * K0 * 0.25f * ue * ku;
-  const float bb = K0 * K0 * 0.25f * (1.0f + ue * ue) * ku * ku;
-  const float cc = (float)(kECharge * l.g / kEMass / kC);
-  const int ka = (int)l.K0_alpha;
-  float xCoef = 0.f, yCoef = 0.f;
+  t.aa = K0 * K0 * 0.25f * ue * ku;
+  t.bb = K0 * K0 * 0.25f * (1.0f + ue * ue) * ku * ku;
+  t.cc = (float)(kECharge * l.g / kEMass / kC);
+  t.ddc = ku * K0 * K0 * 0.25f;
+  t.ue = ue;
+  t.ka = (int)l.K0_alpha;
+  t.xCoef = 0.f;
+  t.yCoef = 0.f;
   if (l.und_type == 1) {
-    if (ka == 1) xCoef = 1.f;
-    else if (ka == -1) yCoef = 1.f;
+    if (t.ka == 1) t.xCoef = 1.f;
+    else if (t.ka == -1) t.yCoef = 1.f;
   } else {
-    xCoef = 1.f;
-    yCoef = 1.f;
+    t.xCoef = 1.f;
+    t.yCoef = 1.f;
   }
+  return t;
+}
+
+__device__ inline void transverse_f32(const TransArgs &t, float dz, float gam, float &x, float &y, float &px,
+                                      float &py) {
+  const float aa = t.aa, bb = t.bb, cc = t.cc;
+  const int ka = t.ka;
   const float ig = 1.0f / gam, ig2 = ig * ig;
-  const float dd = ku * K0 * K0 * 0.25f * ig2 * ue;
-  const float fx = bb * ig * xCoef + cc, fy = bb * ig * yCoef - cc;
+  const float dd = t.ddc * ig2 * t.ue;
+  const float fx = bb * ig * t.xCoef + cc, fy = bb * ig * t.yCoef - cc;
   if (aa == 0.f && dd == 0.f) {
     // ue K0 = 0: the planes decouple into u' = ig p, p' = -f u. For a linear
     // system RK4 is the Taylor polynomial of exp(dz A) to fourth order, and
@@ -164,8 +188,8 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
                                    const cpx<double> *__restrict__ Ez, const double *__restrict__ bes, float x0,
                                    float y0, float px, float py, double &th, double &gam, cpx<float> &eth_out,
                                    unsigned *err) {
-  const double ku = l.ku, ks = g.ks, K0 = l.K0, ue = l.ue;
-  const float ka = (float)l.K0_alpha;
+  const double ku = l.ku, ks = g.ks;
+  const float ka = a.kaf;
   const double dz = g.dz;
   const double fx_ = ((double)x0 - g.minX) * a.inv_dx, fy_ = ((double)y0 - g.minY) * a.inv_dy;
   const int idxI = (int)fx_;
@@ -175,9 +199,9 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
   const float wy1 = (float)(fy_ - idxJ), wy0 = 1.f - wy1;
   const float r2 = x0 * x0 + y0 * y0, pr2 = px * px + py * py;
   const double th0 = th, gam0 = gam;
-  const double K2 = 1.0 + (double)pr2 + (1.0 + ue * ue) * K0 * K0 * 0.5 * (1.0 + ku * ku * 0.5 * (double)r2);
+  const double K2 = 1.0 + (double)pr2 + a.K2a * (1.0 + a.K2b * (double)r2);
   const double ig0 = 1.0 / gam0;
-  const float xi = (float)(ks / ku * K0 * K0 * 0.125 * (1.0 - ue * ue) * ig0 * ig0);
+  const float xi = (float)(a.xic * ig0 * ig0);
   // longitudinal space charge, first mode (sc_l = 1 in the batched path)
   cpx<float> Em = mk(0.f, 0.f);
   {
@@ -203,14 +227,14 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
   float xi2 = 0.f;
   cpx<float> expP = mk(1.f, 0.f);
   if (any_even) {
-    const float uef = (float)ue;
-    xi2 = (float)(sqrt(2.0) * K0 / (1.0 + K0 * K0 * 0.5 * (1.0 + ue * ue))) *
+    const float uef = (float)l.ue;
+    xi2 = a.xi2c *
           sqrtf((1 + ka) * (px * px + uef * uef * py * py) + (1 - ka) * (px * px * uef * uef + py * py));
     const float B = atanf(2.0f * ka * uef * (px - py) /
                           ((1.0f + ka) * (px + uef * uef * py) + (1 - ka) * (px * uef * uef + py)));
-    expP = expi_f32((double)B - l.Phi * 0.5);
+    expP = expi_f32((double)B - a.halfPhi);
   }
-  const cpx<float> etaX = mk((float)l.etaXr, (float)l.etaXi), etaY = mk((float)l.etaYr, (float)l.etaYi);
+  const cpx<float> etaX = a.etaX, etaY = a.etaY;
   cpx<float> cu[NH];
   float sumU2 = 0.f;
 #pragma unroll
@@ -248,11 +272,11 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
     }
     cu[h] = UY ? etaX * fx * ux - etaY * fy * uy : etaX * fx * ux;
   }
-  const float sq = (float)(K0 * sqrt(ue * ue + 1.0));
+  const float sq = a.sq;
   const float ks_f = (float)ks;
   const float e_mc2 = (float)(kECharge / kEMass / kC / kC);
-  const double halfPhi = l.Phi * 0.5;
-  const cpx<float> rotm = expi_f32(-halfPhi);  // e^{-i Phi/2}
+  const double halfPhi = a.halfPhi;
+  const cpx<float> rotm = a.rotm;  // e^{-i Phi/2}
   const float cmv[5] = {0, 0, 0.5f, 0.5f, 1};
   const float wt[5] = {0, 1, 2, 2, 1};
   // The slippage rate ku - ks K2 / (2 gamma^2) is a difference of nearly
@@ -341,8 +365,8 @@ __global__ void __launch_bounds__(kTile)
     float xx = x[n], yy = y[n], ppx = px[n], ppy = py[n];
     const float dzh = (float)(g.dz * 0.5);
     const float gf = (float)gm;
-    transverse_f32(a.l0, dzh, gf, xx, yy, ppx, ppy);
-    transverse_f32(a.l1, dzh, gf, xx, yy, ppx, ppy);
+    transverse_f32(a.t0, dzh, gf, xx, yy, ppx, ppy);
+    transverse_f32(a.t1, dzh, gf, xx, yy, ppx, ppy);
     cpx<float> eth = mk(1.f, 0.f);
     bool have_eth = false;
     if (!a.drift) {

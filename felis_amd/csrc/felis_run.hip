@@ -341,6 +341,21 @@ int pstep_args(felis_ctx *ctx, const felis_lattice *l0, const felis_lattice *l1,
   a.l0.und_type = l0->und_type;
   a.l1.K0 = l1->K0; a.l1.ue = l1->ue; a.l1.K0_alpha = l1->K0_alpha; a.l1.ku = l1->ku; a.l1.g = l1->g;
   a.l1.und_type = l1->und_type;
+  a.t0 = trans_args(a.l0);
+  a.t1 = trans_args(a.l1);
+  if (!l1->drift_flag) {
+    const double K0 = a.l1.K0, ue = a.l1.ue, ku = a.l1.ku;
+    a.K2a = (1.0 + ue * ue) * K0 * K0 * 0.5;
+    a.K2b = ku * ku * 0.5;
+    a.xic = d.ks / ku * K0 * K0 * 0.125 * (1.0 - ue * ue);
+    a.halfPhi = a.l1.Phi * 0.5;
+    a.xi2c = (float)(std::sqrt(2.0) * K0 / (1.0 + K0 * K0 * 0.5 * (1.0 + ue * ue)));
+    a.sq = (float)(K0 * std::sqrt(ue * ue + 1.0));
+    a.kaf = (float)a.l1.K0_alpha;
+    a.rotm = expi_f32(-a.halfPhi);
+    a.etaX = mk((float)a.l1.etaXr, (float)a.l1.etaXi);
+    a.etaY = mk((float)a.l1.etaYr, (float)a.l1.etaYi);
+  }
   a.drift = l1->drift_flag;
   a.nred = nred;
   a.gamR = d.gamR;
