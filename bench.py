@@ -263,8 +263,8 @@ def main():
                     help="only run a device copy of MIB MiB three times (profiler counter scale)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (slices shard across GPUs "
-              "only by design, see the sharding notes); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (sharding the slices across "
+              "GPUs is a design, not code); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)

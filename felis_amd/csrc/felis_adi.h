@@ -220,8 +220,8 @@ __device__ inline void buf_load_row(__amdgpu_buffer_rsrc_t r, int byte_off, cpx<
 template <int E, int NT>
 __global__ void __launch_bounds__(NT)
     k_adi_scan(GridArgs g, const cpx<float> *__restrict__ tab, int cf, cpx<float> *__restrict__ U,
-               const cpx<float> *__restrict__ Sc, const int4 *__restrict__ win, double *__restrict__ slice_e,
-               const cpx<float> *__restrict__ gain, int cw, cpx<float> *__restrict__ scratch) {
+               const cpx<float> *__restrict__ Sc, const int4 *__restrict__ win, double *__restrict__ slice_e, int cw,
+               cpx<float> *__restrict__ scratch) {
   extern __shared__ double lds_d[];
   __shared__ double red[NT / 64];
   const int nx = g.nx, ny = g.ny, nxi = nx - 2, nyi = ny - 2;
@@ -232,10 +232,6 @@ __global__ void __launch_bounds__(NT)
   const int sys = blockIdx.x;
   const int sl = sys % g.slice_n, ph = sys / g.slice_n, h = ph % g.nh, slot = sl + 1;
   const float H = (float)g.harm[h];
-  // optional complex gain on the stored field (the learned correction of a
-  // multi-period jump); null on every plain step
-  const bool has_gain = gain != nullptr;
-  const cpx<float> gs = has_gain ? gain[sys] : mk(1.f, 0.f);
   // the ring position is the same for every lane (read from memory when a
   // captured step replays): keep it scalar
   const int rb = __builtin_amdgcn_readfirstlane(ring_of(g));
@@ -384,8 +380,7 @@ __global__ void __launch_bounds__(NT)
       for (int e = 0; e < E; ++e) {
         const int r = sub * E + e;
         if (colok && r < nyi) {
-          cpx<float> v = cfma(T[o.yGb + e * 16 + sub], uin, y[e]);
-          if (has_gain) v = v * gs;
+          const cpx<float> v = cfma(T[o.yGb + e * 16 + sub], uin, y[e]);
           out[e * nx] = v;
           es += (double)v.re * v.re + (double)v.im * v.im;
         }

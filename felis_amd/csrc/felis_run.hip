@@ -63,8 +63,7 @@ int enqueue_step(felis_ctx *ctx, const felis_lattice *lat, double *e_out, double
 
 // The batched ADI over every (slice, harmonic, pol) system with the scan
 // tables of adi_scan_tables (device copy in tab).
-int launch_adi_scan(felis_ctx *ctx, const cpx<float> *tab, int cf, const cpx<float> *Sc, const int4 *win,
-                    const cpx<float> *gain = nullptr) {
+int launch_adi_scan(felis_ctx *ctx, const cpx<float> *tab, int cf, const cpx<float> *Sc, const int4 *win) {
   GridArgs g = ctx->grid();
   const felis_desc &d = ctx->d;
   const int nsys = ctx->npol() * d.n_harm * d.slice_n;  // the x systems first: a dead Uy is skipped
@@ -80,16 +79,16 @@ int launch_adi_scan(felis_ctx *ctx, const cpx<float> *tab, int cf, const cpx<flo
   case e:                                                                                                   \
     if (wide)                                                                                               \
       hipLaunchKernelGGL((k_adi_scan<e, 1024>), dim3(nsys), dim3(1024), lds, ctx->stream, g, tab, cf, U, Sc, \
-                         win, ctx->slice_e, gain, cw, scratch);                                             \
+                         win, ctx->slice_e, cw, scratch);                                             \
     else                                                                                                    \
       hipLaunchKernelGGL((k_adi_scan<e, 512>), dim3(nsys), dim3(512), lds, ctx->stream, g, tab, cf, U, Sc,   \
-                         win, ctx->slice_e, gain, cw, scratch);                                             \
+                         win, ctx->slice_e, cw, scratch);                                             \
     break;
   switch (E) {
     ADI_CASE(1) ADI_CASE(2) ADI_CASE(3) ADI_CASE(4) ADI_CASE(5) ADI_CASE(6) ADI_CASE(7) ADI_CASE(8) ADI_CASE(9)
     case 10:
       hipLaunchKernelGGL((k_adi_scan<10, 512>), dim3(nsys), dim3(512), lds, ctx->stream, g, tab, cf, U, Sc, win,
-                         ctx->slice_e, gain, cw, scratch);
+                         ctx->slice_e, cw, scratch);
       break;
     default:
       ctx->err = "grid too wide for the LDS-resident ADI";
@@ -242,7 +241,7 @@ struct JumpCtl {
   int *kind = nullptr;          // per step: 0 fine, 1 jump start, 2 inside a jump
   cpx<float> *tab = nullptr;    // ADI scan tables for a field step of J * dz
   int jumps = 0;
-  // per-slice moments, network inputs and outputs, per-system gain, totals
+  // per-slice moments, network inputs and outputs, per-system emission factor, totals
   double *M = nullptr, *tot = nullptr;
   float *X = nullptr, *Y = nullptr;
   cpx<float> *gain = nullptr;

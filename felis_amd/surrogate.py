@@ -2,23 +2,24 @@
 
 A jump (felis_run_surrogate) advances every slice J undulator periods with
 one field solve and one particle push of step J*dz, keeping the slippage of
-the J fine steps through two window passes (each field slot takes the
-source of the slices it slips past, each slice is pushed in the mean of the
-field slots that slip past it). Its residual against J fine steps from the
-same state is, per (slice, harmonic), very nearly a complex factor on the
-jump's new field: a gain of the emitted field, smooth in the state. The
-correction network (felis_mlp: BF16 operands on the matrix cores, FP32
-accumulation, FP32 master weights and Adam, all on the GPU) maps each
-slice's state before the jump (felis_slice_features: current, field
-intensity at the beam, bunching, coherent fraction and phase per harmonic,
-energy spread and modulation) to that factor, as (log|g_h|, arg g_h) per
-harmonic; the jump's ADI multiplies its new field by it.
+the J fine steps through window passes: each field slot takes the source of
+the J slices it slips past, and each slice is pushed in the field slots that
+slip past it, each weighted in time between the field before and after the
+jump's solve by the period in which it meets the slice. Its residual against
+J fine steps from the same state is, per (slice, harmonic), mostly in the
+field the jump emits: E = 2 Sw / H, Sw the windowed source. The correction
+network (felis_mlp: BF16 operands on the matrix cores, FP32 accumulation,
+FP32 master weights and Adam, all on the GPU) maps each slice's state
+before the jump (felis_slice_features: current, field intensity at the
+beam, bunching, coherent fraction and phase per harmonic, energy spread and
+modulation) to a complex factor e_h on that emission, as (log|e_h|,
+arg e_h) per harmonic; after the push the jump's field gains (e_h - 1) E.
 
-The plain batched step is both the fallback and the judge: jumps run only
-while the fundamental bunching |b1| stays below the gate (the linear-gain
-regime), every training target is the fine steps' own result, and
-evaluate() scores a run with jumps against the plain run from the same
-initial state.
+The plain batched step is both the fallback and the measure of the error:
+jumps run only while the fundamental bunching |b1| stays below the gate
+(the linear-gain regime), every training target is the fine steps' own
+result, and evaluate() scores a run with jumps against the plain run from
+the same initial state.
 
 Pipeline (all on the GPU through the C-ABI):
   training_decks()  random peak current (2-5 kA), energy spread (0.5e-4 to
@@ -28,7 +29,8 @@ Pipeline (all on the GPU through the C-ABI):
   pairs()           along a fine run of each deck, at every J-th step while
                     |b1| < gate: the features of the state, J fine steps and
                     one jump from that state (device snapshots), and the
-                    least-squares gain per (slice, harmonic) as the target
+                    least-squares emission factor per (slice, harmonic) as
+                    the target (emission_targets)
   train()           Adam on the weighted mean squared error
   save() / load()   the network as .npz (dims, FP32 weights, input affine)
 """

@@ -166,10 +166,11 @@ int felis_run(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
 /* -------- multi-period jumps (no counterpart in the reference) ---------- */
 /* While the gate is open, one iteration advances every slice `periods`
  * undulator periods: one field solve and one particle push of step
- * periods * dz, with the slippage of the fine steps kept by two window
- * passes (each field slot takes the source of the slices it slips past; each
- * slice is pushed in the mean of the field slots that slip past it), then a
- * slippage of `periods` slots. The gate needs `periods` steps left in one
+ * periods * dz, with the slippage of the fine steps kept by window passes
+ * (each field slot takes the source of the slices it slips past; each slice
+ * is pushed in the field slots that slip past it, each weighted in time
+ * between the field before and after the jump's solve by the period in which
+ * it meets the slice), then a slippage of `periods` slots. The gate needs `periods` steps left in one
  * undulator section with no wake update inside, and the fundamental
  * bunching |b1| of the state below bunching_max; once it fails on the
  * bunching, the rest of the run takes fine steps (the plain batched step). */
