@@ -335,6 +335,7 @@ extern "C" int felis_create(const felis_desc *desc, felis_ctx **out) {
       (e = dalloc(ctx, &ctx->adix, ctx->h_adix.size() * sizeof(cpx<double>))) ||
       (e = dalloc(ctx, &ctx->adiy, ctx->h_adiy.size() * sizeof(cpx<double>))) ||
       (e = dalloc(ctx, &ctx->bessel, ctx->h_bessel.size() * sizeof(double))) ||
+      (e = dalloc(ctx, &ctx->bessel_f, ctx->h_bessel.size() * sizeof(float))) ||
       (e = dalloc(ctx, &ctx->wakeE, (size_t)d.slice_n * sizeof(double))) ||
       (e = dalloc(ctx, &ctx->wakeF, (size_t)d.slice_n * sizeof(double))) ||
       (e = dalloc(ctx, &ctx->den, (size_t)d.slice_n * sizeof(double))) ||
@@ -356,6 +357,8 @@ extern "C" int felis_create(const felis_desc *desc, felis_ctx **out) {
   (void)hipMemcpyAsync(ctx->adix, ctx->h_adix.data(), ctx->h_adix.size() * sizeof(cpx<double>), hipMemcpyHostToDevice, ctx->stream);
   (void)hipMemcpyAsync(ctx->adiy, ctx->h_adiy.data(), ctx->h_adiy.size() * sizeof(cpx<double>), hipMemcpyHostToDevice, ctx->stream);
   (void)hipMemcpyAsync(ctx->bessel, ctx->h_bessel.data(), ctx->h_bessel.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+  const std::vector<float> bes_f(ctx->h_bessel.begin(), ctx->h_bessel.end());
+  (void)hipMemcpyAsync(ctx->bessel_f, bes_f.data(), bes_f.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
     felis_destroy(ctx);
     return FELIS_ERR_HIP;

@@ -45,6 +45,7 @@ struct felis_ctx {
   bool sig_set = false;
   felis::cpx<double> *abc = nullptr, *adix = nullptr, *adiy = nullptr;
   double *bessel = nullptr, *wakeE = nullptr, *wakeF = nullptr, *den = nullptr;
+  float *bessel_f = nullptr;  // the Bessel table in FP32, for the batched push
   double *slice_e = nullptr, *red_out = nullptr;
   unsigned *errw = nullptr;
   int ring_base = 0;

@@ -48,6 +48,21 @@ struct PStepArgs {
   int hstep[FELIS_MAX_HARM];
 };
 
+// 1/x in FP64 for a positive normal x: the hardware reciprocal and two
+// Newton steps (each squares the relative error), half the instructions of
+// the IEEE division sequence
+__device__ inline double rcp_f64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+}
+
+// the Bessel table (FP32 copy) at order `order`, linear in the argument
+__device__ inline float bessel_interp_f(const float *__restrict__ tab, int bo, int idx, int order, float w0, float w1) {
+  return tab[idx * bo + order] * w0 + tab[(idx + 1) * bo + order] * w1;
+}
+
 // e^{i H_h psi} for every harmonic from e^{i psi}, by multiplying along the
 // ascending harmonic list (steps of 1 and 2 use e and e^2 directly)
 template <int NH>
@@ -185,7 +200,7 @@ __device__ inline void transverse_f32(const TransArgs &t, float dz, float gam, f
 template <int NH, bool UY>
 __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const LatArgs &l, int slot, double wakeE,
                                    const cpx<float> *__restrict__ Ux, const cpx<float> *__restrict__ Uy,
-                                   const cpx<double> *__restrict__ Ez, const double *__restrict__ bes, float x0,
+                                   const cpx<double> *__restrict__ Ez, const float *__restrict__ bes, float x0,
                                    float y0, float px, float py, double &th, double &gam, cpx<float> &eth_out,
                                    unsigned *err) {
   const double ku = l.ku, ks = g.ks;
@@ -200,7 +215,7 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
   const float r2 = x0 * x0 + y0 * y0, pr2 = px * px + py * py;
   const double th0 = th, gam0 = gam;
   const double K2 = 1.0 + (double)pr2 + a.K2a * (1.0 + a.K2b * (double)r2);
-  const double ig0 = 1.0 / gam0;
+  const double ig0 = rcp_f64(gam0);
   const float xi = (float)(a.xic * ig0 * ig0);
   // longitudinal space charge, first mode (sc_l = 1 in the batched path)
   cpx<float> Em = mk(0.f, 0.f);
@@ -256,15 +271,15 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
       atomicOr(err, EB_BESSEL);
     } else if (H % 2 == 1) {
       const float sign = ((H - 1) / 2 % 2 == 0) ? 1.f : -1.f;
-      const float J1 = (float)bessel_interp(bes, g.bo, idx, (H - 1) / 2, w0, w1);
-      const float J2 = (float)bessel_interp(bes, g.bo, idx, (H + 1) / 2, w0, w1);
+      const float J1 = bessel_interp_f(bes, g.bo, idx, (H - 1) / 2, w0, w1);
+      const float J2 = bessel_interp_f(bes, g.bo, idx, (H + 1) / 2, w0, w1);
       fx = mk(sign * (ka * J1 - J2), 0.f);
       fy = mk(sign * (ka * J1 + J2), 0.f);
     } else {
       const float sign = (H / 2 % 2 == 0) ? 1.f : -1.f;
-      const float J1 = (float)bessel_interp(bes, g.bo, idx, (H - 2) / 2, w0, w1);
-      const float J2 = (float)bessel_interp(bes, g.bo, idx, H / 2, w0, w1);
-      const float J3 = (float)bessel_interp(bes, g.bo, idx, (H + 2) / 2, w0, w1);
+      const float J1 = bessel_interp_f(bes, g.bo, idx, (H - 2) / 2, w0, w1);
+      const float J2 = bessel_interp_f(bes, g.bo, idx, H / 2, w0, w1);
+      const float J3 = bessel_interp_f(bes, g.bo, idx, (H + 2) / 2, w0, w1);
       const float s = sign * xi2 * H * 0.5f;
       const cpx<float> expM = conjg(expP);
       fx = (expP * (ka * J1 - J2) + expM * (ka * J2 - J3)) * s;
@@ -343,7 +358,7 @@ __global__ void __launch_bounds__(kTile)
     k_pstep(GridArgs g0, PStepArgs a, Tiles tl, double *__restrict__ theta, float *__restrict__ gamma,
             float *__restrict__ x, float *__restrict__ y, float *__restrict__ px, float *__restrict__ py,
             const double *__restrict__ seg_w, const cpx<float> *__restrict__ Ux, const cpx<float> *__restrict__ Uy,
-            const cpx<double> *__restrict__ Ez, const double *__restrict__ bes, const double *__restrict__ wakeE,
+            const cpx<double> *__restrict__ Ez, const float *__restrict__ bes, const double *__restrict__ wakeE,
             double *__restrict__ tile_red, unsigned *err) {
   __shared__ double red[(2 * NH + 1 + 7) * kTile];
   GridArgs g = g0;
@@ -421,7 +436,7 @@ __global__ void __launch_bounds__(kTile)
       acc[2 * h + 1] = (double)eh[h].im * w;
     }
     acc[2 * NH] = w;
-    const double inv = 1.0 / gm;
+    const double inv = rcp_f64(gm);
     const double xp = (double)ppx * inv, yp = (double)ppy * inv, xv = xx, yv = yy;
     acc[2 * NH + 1] = xv * xv;
     acc[2 * NH + 2] = yv * yv;

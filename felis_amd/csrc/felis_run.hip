@@ -137,12 +137,12 @@ int launch_particles(felis_ctx *ctx, const PStepArgs &a, double *tile_red, void 
   if (ctx->uy_live)
     hipLaunchKernelGGL((k_pstep<NH, true>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, g, a, ctx->tiles(),
                        ctx->theta, (float *)ctx->gamma, (float *)ctx->x, (float *)ctx->y, (float *)ctx->px,
-                       (float *)ctx->py, ctx->seg_w, Ux, Uy, (const cpx<double> *)ctx->Ez, ctx->bessel, ctx->wakeE,
+                       (float *)ctx->py, ctx->seg_w, Ux, Uy, (const cpx<double> *)ctx->Ez, ctx->bessel_f, ctx->wakeE,
                        tile_red, ctx->errw);
   else
     hipLaunchKernelGGL((k_pstep<NH, false>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, g, a, ctx->tiles(),
                        ctx->theta, (float *)ctx->gamma, (float *)ctx->x, (float *)ctx->y, (float *)ctx->px,
-                       (float *)ctx->py, ctx->seg_w, Ux, Uy, (const cpx<double> *)ctx->Ez, ctx->bessel, ctx->wakeE,
+                       (float *)ctx->py, ctx->seg_w, Ux, Uy, (const cpx<double> *)ctx->Ez, ctx->bessel_f, ctx->wakeE,
                        tile_red, ctx->errw);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
