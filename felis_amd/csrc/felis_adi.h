@@ -346,6 +346,24 @@ __global__ void __launch_bounds__(NT)
     }
   }
   __syncthreads();
+  // the boundary ring is untouched by the ADI but counts in the energy: its
+  // loads go out before the y sweep and are summed after it
+  const int nring = 2 * nx + 2 * (ny - 2);
+  constexpr int kRing = (2 * 162 + 2 * 160 + NT - 1) / NT;  // ring cells per thread, grids up to 162 wide
+  cpx<float> rv[kRing];
+#pragma unroll
+  for (int q = 0; q < kRing; ++q) {
+    const int k0 = tid + q * NT, k = k0 < nring ? k0 : 0;  // a valid cell for every lane
+    int idx;
+    if (k < nx) idx = k;
+    else if (k < 2 * nx) idx = (ny - 1) * nx + (k - nx);
+    else {
+      const int r = k - 2 * nx;
+      idx = (1 + (r >> 1)) * nx + ((r & 1) ? nx - 1 : 0);
+    }
+    const cpx<float> v = Un[idx];
+    rv[q] = k0 < nring ? v : mk(0.f, 0.f);
+  }
   // ---- y-implicit half step down the columns, stored straight to U with
   // the slice energy accumulated in FP64: the columns held in LDS, then
   // (when the system does not fit) the rest from scratch ----
@@ -399,19 +417,8 @@ __global__ void __launch_bounds__(NT)
     __syncthreads();
     ysweep(L1, cw, w1);
   }
-  // the boundary ring is untouched by the ADI but counts in the energy
-  const int nring = 2 * nx + 2 * (ny - 2);
-  for (int k = tid; k < nring; k += NT) {
-    int idx;
-    if (k < nx) idx = k;
-    else if (k < 2 * nx) idx = (ny - 1) * nx + (k - nx);
-    else {
-      int r = k - 2 * nx;
-      idx = (1 + (r >> 1)) * nx + ((r & 1) ? nx - 1 : 0);
-    }
-    cpx<float> v = Un[idx];
-    es += (double)v.re * v.re + (double)v.im * v.im;
-  }
+#pragma unroll
+  for (int q = 0; q < kRing; ++q) es += (double)rv[q].re * rv[q].re + (double)rv[q].im * rv[q].im;
   for (int off = 32; off > 0; off >>= 1) es += __shfl_down(es, off);
   if (lane == 0) red[wv] = es;
   __syncthreads();
