@@ -355,14 +355,17 @@ def saturation(z, P):
 
 def run_curves(deck_path, seed: int, periods: int | None = None, gate: float = 0.05, net=None,
                n_steps: int | None = None, net_gate: float = 0.0):
-    """One run of the deck: plain batched (periods None) or with jumps.
-    Returns steps, total field energy per step (NaN inside jumps), |b1|,
-    kinds and wall seconds of the step loop."""
+    """One run of the deck: plain batched (periods None) or with jumps, with
+    the workload's autotuned launch shapes. Returns steps, total field energy
+    per step (NaN inside jumps), |b1|, kinds and wall seconds of the step
+    loop."""
+    from . import tune
     deck = fl.Deck(deck_path)
     d = deck.desc(fl.MIXED_FP32)
     nh = d.n_harm
     ctx = fl.Context(d)
     deck.load(ctx, seed=seed)
+    tune.apply(ctx)  # the workload's autotuned launch shapes (defaults when it has none)
     N = deck.max_step if n_steps is None else n_steps
     tab = deck.lattice(0, N)
     ctx.sync()
