@@ -63,15 +63,26 @@ static __global__ void __launch_bounds__(256)
     const int seg = sp * g.slice_n + sl;
     const long long b = seg_off[seg], e = seg_off[seg + 1];
     const double w = seg_w[seg];
-    for (long long n = b + threadIdx.x; n < e; n += blockDim.x) {
-      const double th = theta[n], dg = (double)gamma[n];
+    // the next iteration's loads in flight while this one is summed
+    const long long n0 = b + threadIdx.x;
+    double thq = n0 < e ? theta[n0] : 0.0;
+    float gq = n0 < e ? gamma[n0] : 0.f, xq = n0 < e ? x[n0] : 0.f, yq = n0 < e ? y[n0] : 0.f;
+    for (long long n = n0; n < e; n += blockDim.x) {
+      const double th = thq, dg = (double)gq;
+      const float xn = xq, yn = yq;
+      if (n + blockDim.x < e) {
+        thq = theta[n + blockDim.x];
+        gq = gamma[n + blockDim.x];
+        xq = x[n + blockDim.x];
+        yq = y[n + blockDim.x];
+      }
       const cpx<float> e1 = expi_f32(th);
       acc[0] += w;
       acc[1] += w * dg;
       acc[2] += w * dg * dg;
       acc[3] += w * dg * e1.re;
       acc[4] += w * dg * e1.im;
-      const double fx_ = ((double)x[n] - g.minX) * inv_dx, fy_ = ((double)y[n] - g.minY) * inv_dy;
+      const double fx_ = ((double)xn - g.minX) * inv_dx, fy_ = ((double)yn - g.minY) * inv_dy;
       const int i = (int)fx_, j = (int)fy_;
       const bool in = i >= 0 && i < g.nx - 1 && j >= 0 && j < g.ny - 1;
       const float wx1 = in ? (float)(fx_ - i) : 0.f, wy1 = in ? (float)(fy_ - j) : 0.f;
