@@ -5,7 +5,9 @@ Every tuning the autotuner may pick runs the same arithmetic in another
 launch shape or order: the field energy of every period and the final field
 agree with the default tuning's to 1e-5 relative (float atomics of the
 deposit summed in another order; the deposit's LDS budget of 32 KiB forces
-its harmonics into chunks), and the particles' phases to 1e-4 rad.
+its harmonics into chunks, and 4 KiB holds no slice's window, so the
+deposit clears its window in Sc and adds with global atomics), and the
+particles' phases to 1e-4 rad.
 """
 from pathlib import Path
 
@@ -46,8 +48,10 @@ def ref():
                                dict(step_graph=0, deposit_threads=512, deposit_lds=65536),
                                dict(step_graph=1, deposit_threads=256, deposit_lds=32768),
                                dict(adi_threads=1024),
-                               dict(step_graph=1, deposit_threads=512, deposit_lds=65536, adi_threads=1024)],
-                         ids=["graph", "dep256", "dep512_64k", "graph_dep256_32k", "adi1024", "graph_dep512_adi1024"])
+                               dict(step_graph=1, deposit_threads=512, deposit_lds=65536, adi_threads=1024),
+                               dict(deposit_threads=512, deposit_lds=4096)],
+                         ids=["graph", "dep256", "dep512_64k", "graph_dep256_32k", "adi1024", "graph_dep512_adi1024",
+                              "dep_window_past_lds"])
 def test_tunings_agree_with_default(ref, t):
     r = _run(t)
     assert tune._rel_energy_err(r["e"], ref["e"]) < 1e-5
