@@ -469,8 +469,9 @@ __global__ void __launch_bounds__(kTile)
 // Sc needs no clearing between steps (the field solve reads the window only).
 // Window cells on the boundary ring and the reference's flat-index spill
 // (A.5) use atomics; ring cells are never read by the batched field solve.
-// A window larger than the LDS budget (hp = 0) is first cleared in Sc and
-// then deposited with global atomics.
+// A window larger than the LDS budget for its harmonics goes through LDS in
+// strips of rows, one harmonic per pass; one of which not even a row fits
+// is first cleared in Sc and then deposited with global atomics.
 // NPOL = 1 deposits the x polarisation only: the y source is exactly zero
 // (eta_Y = 0: a planar undulator with K0_alpha = 1) and Uy is zero.
 template <int NH, int NPOL>
@@ -526,6 +527,13 @@ __global__ void __launch_bounds__(1024)
   int hp = W > 0 ? lds_bytes / (W * NPOL * (int)sizeof(cpx<float>)) : NH;
   if (hp > NH) hp = NH;
   const int step = hp > 0 ? hp : 1;
+  // A window larger than the budget for one harmonic goes through LDS in
+  // strips of node rows that fit, one harmonic per pass over the particles;
+  // only when not even one row fits are the window's interior cells cleared
+  // in Sc and added to with global atomics.
+  const int rows_fit = ww > 0 ? lds_bytes / (ww * NPOL * (int)sizeof(cpx<float>)) : 0;
+  const bool in_lds = hp > 0 || rows_fit > 0;
+  const int sh = hp == 0 && rows_fit > 0 ? rows_fit : max(wh, 1);  // node rows per strip
   // the only interior Sc cells this deposit can make nonzero
   if (tid == 0) win_out[slot] = empty ? make_int4(1, 0, 1, 0) : make_int4(wi0, wi0 + ww - 1, wj0, wj0 + wh - 1);
   bool any_even = false;
@@ -568,7 +576,7 @@ __global__ void __launch_bounds__(1024)
   }
   const cpx<float> etaX = mk((float)d.etaXr, (float)d.etaXi), etaY = mk((float)d.etaYr, (float)d.etaYi);
   cpx<float> *win = reinterpret_cast<cpx<float> *>(lds_d);
-  if (hp == 0 && W > 0) {
+  if (!in_lds && W > 0) {
     // the window's interior cells start from zero for the atomics below
     for (int q = tid; q < NH * NPOL * W; q += nt) {
       const int hh = q / (NPOL * W), pol = (q / W) % NPOL, r = q % W;
@@ -581,8 +589,12 @@ __global__ void __launch_bounds__(1024)
   }
   for (int h0 = 0; h0 < NH; h0 += step) {
     const int hc = min(step, NH - h0);
-    if (hp > 0) {
-      for (int q = tid; q < hc * NPOL * W; q += nt) win[q] = mk(0.f, 0.f);
+    for (int sr0 = 0; sr0 < max(wh, 1); sr0 += sh) {
+    // this strip's node rows sr0 .. sr0+srh-1 of the window (all of them
+    // when it fits); cells outside the window take global atomics, once
+    const int srh = min(sh, wh - sr0), Ws = ww * max(srh, 0);
+    if (in_lds) {
+      for (int q = tid; q < hc * NPOL * Ws; q += nt) win[q] = mk(0.f, 0.f);
       __syncthreads();
     }
     for (int sp = 0; sp < n_species; ++sp) {
@@ -604,17 +616,18 @@ __global__ void __launch_bounds__(1024)
       _Pragma("unroll") for (int hh = 0; hh < NH; ++hh) {                                                       \
         if (hh >= hc) break;                                                                                    \
         const cpx<float> vx = acc[c][hh][0], vy = acc[c][hh][1];                                                \
-        if (kin) {                                                                                              \
-          const int q = (kJ + jj - wj0) * ww + (kI + ii - wi0);                                                 \
-          float *p0 = reinterpret_cast<float *>(lds_d) + 2 * ((hh * NPOL + 0) * W + q);                         \
+        const int row = kJ + jj - wj0 - sr0;                                                                    \
+        if (kin && row >= 0 && row < srh) {                                                                     \
+          const int q = row * ww + (kI + ii - wi0);                                                             \
+          float *p0 = reinterpret_cast<float *>(lds_d) + 2 * ((hh * NPOL + 0) * Ws + q);                        \
           atomicAdd(p0, vx.re);                                                                                 \
           atomicAdd(p0 + 1, vx.im);                                                                             \
           if (NPOL == 2) {                                                                                      \
-            float *p1 = reinterpret_cast<float *>(lds_d) + 2 * ((hh * NPOL + 1) * W + q);                       \
+            float *p1 = reinterpret_cast<float *>(lds_d) + 2 * ((hh * NPOL + 1) * Ws + q);                      \
             atomicAdd(p1, vy.re);                                                                               \
             atomicAdd(p1 + 1, vy.im);                                                                           \
           }                                                                                                     \
-        } else {                                                                                                \
+        } else if (!kin && sr0 == 0) {                                                                          \
           const long long o =                                                                                   \
               (long long)(h0 + hh) * g.S * g.N + (long long)slot * g.N + (long long)kJ * g.nx + kI + jj * g.nx + ii; \
           unsafeAtomicAdd(&Scx[o].re, vx.re);                                                                   \
@@ -654,7 +667,7 @@ __global__ void __launch_bounds__(1024)
           key = k2;
           kI = idxI;
           kJ = idxJ;
-          kin = hp > 0 && idxI < g.nx - 1 && idxJ < g.ny - 1;
+          kin = in_lds && idxI < g.nx - 1 && idxJ < g.ny - 1;
         }
         float wx[2], wy[2];
         wx[1] = (float)(fx_ - idxI); wx[0] = 1.f - wx[1];
@@ -708,12 +721,12 @@ __global__ void __launch_bounds__(1024)
 #undef FELIS_DEPOSIT_FLUSH
       }
     }
-    if (hp > 0) {
+    if (in_lds) {
       __syncthreads();
-      for (int q = tid; q < hc * NPOL * W; q += nt) {
+      for (int q = tid; q < hc * NPOL * Ws; q += nt) {
         const cpx<float> v = win[q];
-        const int hh = q / (NPOL * W), pol = (q / W) % NPOL, r = q % W;
-        const int jy = wj0 + r / ww, ix = wi0 + r % ww;
+        const int hh = q / (NPOL * Ws), pol = (q / Ws) % NPOL, r = q % Ws;
+        const int jy = wj0 + sr0 + r / ww, ix = wi0 + r % ww;
         cpx<float> *dst = (pol ? Scy : Scx) + (long long)(h0 + hh) * g.S * g.N + (long long)slot * g.N +
                           (long long)jy * g.nx + ix;
         if (jy > 0 && jy < g.ny - 1 && ix > 0 && ix < g.nx - 1) {
@@ -724,6 +737,7 @@ __global__ void __launch_bounds__(1024)
         }
       }
       __syncthreads();
+    }
     }
   }
   // radial source of the last species, stored whole for this slot. Each wave
