@@ -910,8 +910,6 @@ int felis_ctx::ez_solve(const felis_lattice *lat) {
       return FELIS_ERR_UNSUPPORTED;
   }
   LAUNCHCHK();
-  hipLaunchKernelGGL(k_ez_slot0, dim3(1), dim3(64), 0, stream, g, ez);
-  LAUNCHCHK();
   return FELIS_OK;
 }
 

@@ -663,6 +663,10 @@ __global__ void __launch_bounds__(64) k_ez_solve(GridArgs g, const double *__res
   const double *w = coef + (long long)(ll * F + f) * 2 * nr, *bp = w + nr;
   const long long es = (long long)g.S * F * nr;
   cpx<double> *ez = Ez + ll * es;
+  // slot 0's last entry, which the reference clears after the solve: no
+  // block reads it (the elimination below takes it as zero) or writes slot 0
+  if (sid == 0)
+    for (int t = lane; t < L * F; t += 64) Ez[(long long)(t / F) * es + (t % F) * nr + nr - 1] = mk(0.0, 0.0);
   double av[E];
   cpx<double> bv[E];
   double A = 1.0;
@@ -703,13 +707,6 @@ __global__ void __launch_bounds__(64) k_ez_solve(GridArgs g, const double *__res
       const cpx<double> v = dprev - z * aj;
       ez[(long long)slot * F * nr + f * nr + j] = mk(v.re / bp[j], v.im / bp[j]);
     }
-  }
-}
-
-static __global__ void k_ez_slot0(GridArgs g, cpx<double> *__restrict__ Ez) {
-  for (int t = threadIdx.x; t < g.L * g.F; t += blockDim.x) {
-    const int ll = t / g.F, f = t % g.F;
-    Ez[(long long)ll * g.S * g.F * g.nr + f * g.nr + g.nr - 1] = mk(0.0, 0.0);
   }
 }
 
