@@ -248,3 +248,50 @@ def test_find_parameter_matches_reference(deck):
         assert (got is not None) == bool(found), (blk, rank, key, got, val)
         if found:
             assert got == val, (blk, rank, key, got, val)
+
+
+@pytest.mark.parametrize("und_type,ka,g", [(1, 1, 0.0), (1, 1, 25.0), (1, -1, 8.0), (2, 0, -12.0)])
+def test_transverse_rk4_is_the_matrix_taylor_polynomial(und_type, ka, g):
+    """particlePush.cpp:32-117 with ue = 0 (no x-y coupling): the planes are
+    the oscillators u' = p/gamma, p' = -f u, whose RK4 step of h = dz/2 is
+    exactly the fourth-order Taylor polynomial of exp(h A),
+        u <- c u + s p/gamma,  p <- c p - s f u,  w = f/gamma,
+        c = 1 - h^2 w/2 + h^4 w^2/24,  s = h (1 - h^2 w/6),
+    the form the batched push evaluates; and that polynomial is exp(h A)
+    (scipy) to its truncation, O((h^2 w)^3). A dropped stage weight or a
+    wrong sign in the focusing term fails both."""
+    from scipy.linalg import expm
+    rng = np.random.default_rng(11)
+    n = 32
+    x, y = rng.normal(scale=1e-4, size=(2, n))
+    px, py = rng.normal(scale=2e-3, size=(2, n))
+    gamma = 4000.0 + 200.0 * rng.random(n)
+    dz = 0.026
+    st = make_state(seg_off=np.array([0, n]), x=x.copy(), y=y.copy(), px=px.copy(), py=py.copy(),
+                    gamma=gamma.copy(), dz=dz)
+    lat = po.Lattice()
+    lat.K0, lat.ue, lat.K0_alpha, lat.ku, lat.g, lat.und_type = 2.5, 0.0, float(ka), 2 * np.pi / dz, g, und_type
+    assert st.transverse_push(lat) == 0
+    h = dz / 2
+    bb = lat.K0 ** 2 * 0.25 * lat.ku ** 2
+    cc = 1.60217662e-19 * g / 9.10938291e-31 / 299792458.0
+    xc, yc = (1.0, 1.0) if und_type == 2 else ((1.0, 0.0) if ka == 1 else (0.0, 1.0))
+
+    def plane(u, p, f, gam):
+        w = f / gam
+        c = 1 - h * h * w / 2 + h ** 4 * w * w / 24
+        s = h * (1 - h * h * w / 6)
+        return c * u + s * p / gam, c * p - s * f * u
+
+    fx = bb / gamma * xc + cc
+    fy = bb / gamma * yc - cc
+    xw, pxw = plane(x, px, fx, gamma)
+    yw, pyw = plane(y, py, fy, gamma)
+    for got, want in ((st.x, xw), (st.y, yw), (st.px, pxw), (st.py, pyw)):
+        assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
+    for k in range(3):
+        for u0, p0, f, u1, p1 in ((x, px, fx, st.x, st.px), (y, py, fy, st.y, st.py)):
+            A = np.array([[0.0, 1.0 / gamma[k]], [-f[k], 0.0]])
+            e = expm(h * A) @ np.array([u0[k], p0[k]])
+            tol = (h * h * abs(f[k]) / gamma[k]) ** 3 * max(abs(u0[k]), abs(p0[k]) * h / gamma[k]) + 1e-18
+            assert abs(u1[k] - e[0]) <= tol and abs(p1[k] - e[1]) <= tol * gamma[k] / h + 1e-18
