@@ -9,20 +9,30 @@
  *   orc_periodic_particles, orc_total_energy, orc_twiss, orc_bfactor,
  *   orc_update_wake, orc_wake_function, orc_update_k_quadg, orc_load_seed_3d,
  *   orc_shift_field (its MPI half)   -> golden vectors written by the
- *       reference's own compiled sources (tests/golden/, oracle/_ref).
- *   orc_abc_setup, orc_bessel_table  -> closed forms and scipy.special.jv.
- *   orc_solve_field_u_3d             -> dense complex solves of the same
- *       tridiagonal systems (numpy.linalg.solve), free-space Gaussian-beam
- *       diffraction, unitarity with the absorbing layer off.
- *   orc_transverse_push              -> exact matrix-exponential solution of
- *       the linear ODE (scipy.linalg.expm), RK4 O(dz^5) local error.
- *   orc_push_theta_gamma_3d, orc_drift_theta_gamma
- *                                    -> zero-field closed form, wake loss,
- *       Madey low-gain curve, pendulum invariant, energy balance with the
- *       field solver.
- *   orc_solve_sc_3d                  -> single-particle corner weights,
- *       quiet-start cancellation, energy balance with the push; Ez radial
- *       solve against a dense solve.
+ *       reference's own compiled sources (tests/golden/, oracle/_ref;
+ *       test_oracle_golden.py).
+ *   orc_bessel_table                 -> scipy.special.jv.
+ *   orc_abc_setup                    -> the profile's closed form with and
+ *       without the layer; a tilted beam is absorbed, never amplified
+ *       (test_oracle_pinning.py).
+ *   orc_solve_field_u_3d             -> dense complex solves of the row
+ *       equations with the special last row; Gaussian-beam diffraction
+ *       (width, Gouy phase) at second order in the cell size; sum |U|^2
+ *       conserved without the layer.
+ *   orc_transverse_push              -> the step map is the fourth-order
+ *       Taylor polynomial of exp(hM) for the linear system, coupled (ue != 0)
+ *       and uncoupled; symplectic to O(h^5) for K0_alpha = 1.
+ *   orc_push_theta_gamma_3d          -> zero-field closed form (K2 with r and
+ *       p, wake loss, grid truncation); one RK4 step against an adaptive
+ *       integrator of the same ODE with O(dz^5) local error; the Madey
+ *       low-gain curve of a frozen field.
+ *   orc_drift_theta_gamma            -> zero- and uniform-field closed forms.
+ *   orc_solve_sc_3d                  -> single-particle corner writes with
+ *       scipy's Bessel functions, quiet-start cancellation; the radial Ez
+ *       solve against a dense solve with the slot-0 read; the field energy it
+ *       adds equals the work the push's field does on the beam.
+ *   test_oracle_mutations.py plants mistakes in copies of this file and
+ *       shows that the checks above catch each of them.
  *   orc_push_theta_gamma_1d, orc_solve_sc_1d, orc_solve_field_u_1d
  *                                    -> zero-field slippage, quiet-start
  *       cancellation and U += Sc closed forms; their coupling and source
