@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes as C
 import json
+import re
 import math
 import sys
 import tempfile
@@ -212,6 +213,137 @@ def main():
             rows.append([blk, rank, key, int(r), buf.value.decode() if r else None])
         result[dname] = rows
     (HERE / "findparam.json").write_text(json.dumps(result, indent=1))
+
+    # deck -> lattice table: the reference's FindParameters reads the blocks of
+    # deck_lattice.inp, parameterSetting.cpp:287-429 expands them into units and
+    # quads (restated below), parameterSetting.cpp:168-174 sets the state before
+    # the loop and the reference's updateK_quadG.cpp walks 1200 periods.
+    deck = HERE / "deck_lattice.inp"
+
+    def fp(block, rank, key):
+        buf = C.create_string_buffer(512)
+        r = R.ref_find_parameters(block.encode(), rank, key.encode(), str(deck).encode(), buf)
+        return buf.value.decode() if r else None
+
+    def atof(v):
+        m = re.match(r"\s*[-+]?(\d+\.?\d*|\.\d+)([eE][-+]?\d+)?", v or "")
+        return float(m.group(0)) if m else 0.0
+
+    und, rank = [], 1
+    while True:
+        t = fp("Undulator", rank, "undulator_type")
+        if t is None:
+            break
+        utype = 1 if t.strip().startswith("BiPolar") else 2
+        n = int(atof(fp("Undulator", rank, "numbers")))
+        air = 1 if (fp("Undulator", rank, "in_air") or "").strip().startswith("ON") else 0
+        lam = atof(fp("Undulator", rank, "lambdaU")) * 0.01
+        lin = atof(fp("Undulator", rank, "linear_taper")) / math.sqrt(2.0) if fp("Undulator", rank, "linear_taper") else 0.0
+        qt = atof(fp("Undulator", rank, "quad_taper")) / math.sqrt(2.0) if fp("Undulator", rank, "quad_taper") else 0.0
+        slope = atof(fp("Undulator", rank, "slope_K")) / math.sqrt(2.0) if fp("Undulator", rank, "slope_K") else 0.0
+        qid = int(atof(fp("Undulator", rank, "quad_taper_start_index"))) if fp("Undulator", rank, "quad_taper_start_index") else 30000
+        pol = atof(fp("Undulator", rank, "polarity"))
+        ka = int(atof(fp("Undulator", rank, "K0_alpha"))) if fp("Undulator", rank, "K0_alpha") else 1
+        K0 = atof(fp("Undulator", rank, "K0"))
+        us, ue_ = atof(fp("Undulator", rank, "unit_start")), atof(fp("Undulator", rank, "unit_end"))
+        ds, de = atof(fp("Undulator", rank, "undulator_start")), atof(fp("Undulator", rank, "undulator_end"))
+        rows = [[us, ue_, ds, de, K0]]
+        unit_len, und_off, und_len = ue_ - us, ds - us, de - ds
+        for i in range(1, n):
+            a = rows[i - 1][1]
+            b = a + unit_len
+            c = a + und_off
+            Ki = K0 + i * lin + ((i - qid) * (i - qid) * qt if i >= qid else 0.0)
+            rows.append([a, b, c, c + und_len, Ki])
+        for r in rows:
+            und.append(r + [slope, pol, lam, ka, utype, air, rank - 1])
+        rank += 1
+    quad, rank = [], 1
+    while fp("Quad", rank, "numbers") is not None:
+        n = int(atof(fp("Quad", rank, "numbers")))
+        us, ue_ = atof(fp("Quad", rank, "unit_start")), atof(fp("Quad", rank, "unit_end"))
+        qs, qe, g = atof(fp("Quad", rank, "quad_start")), atof(fp("Quad", rank, "quad_end")), atof(fp("Quad", rank, "g"))
+        unit_len, q_off, q_len = ue_ - us, qs - us, qe - qs
+        rows = [[us, ue_, qs, qe]]
+        for i in range(1, n):
+            a = rows[i - 1][1]
+            rows.append([a, a + unit_len, a + q_off, a + q_off + q_len])
+        quad += [[r[2], r[3], g, rank] for r in rows]
+        rank += 1
+    und = np.array(und, np.float64)
+    quad = np.array(quad, np.float64)
+    u0 = und[0]
+    state = np.array([u0[4], u0[6], u0[8], 2.0 * math.pi / u0[7], 0.0, u0[7], u0[9], 0.0])
+    init = state[:7].copy()
+    dz = u0[7] * 1
+    seq = []
+    for it in range(0, 1200):
+        for half in (0.0, 0.5):
+            R.ref_update_k_quadg(P(und), len(und), P(quad), len(quad), dz, it, half, P(state), P(flags))
+            seq.append(np.concatenate([state[:7], flags.astype(np.float64)]))
+    np.savez(HERE / "update_k_quadg_deck.npz", und=und, quad=quad, dz=dz, init=init, seq=np.array(seq))
+
+    # deck -> derived Domain scalars (parameterSetting.cpp:71-77, 89-111, 168-175,
+    # 186-196, 236-240 restated on the reference's FindParameters values) ----------
+    PLANK_H, C_LIGHT, MC2_MEV = 4.135667696e-15, 299792458.0, 0.510998918
+
+    def derived(path):
+        def f(block, key, default=None):
+            buf = C.create_string_buffer(512)
+            r = R.ref_find_parameters(block.encode(), 1, key.encode(), str(path).encode(), buf)
+            return buf.value.decode() if r else default
+        dim = int(atof(f("Domain", "dimension")))
+        static = (f("Domain", "mode") or "").strip().startswith("Static")
+        ks = atof(f("Domain", "photon_energy")) * 2 * math.pi / (PLANK_H * C_LIGHT)
+        lambda0 = 2 * math.pi / ks
+        Lz = atof(f("Save", "total_length"))
+        minX, maxX = atof(f("Domain", "minX")) * 1e-6, atof(f("Domain", "maxX")) * 1e-6
+        minY, maxY = atof(f("Domain", "minY")) * 1e-6, atof(f("Domain", "maxY")) * 1e-6
+        minZ, maxZ = atof(f("Domain", "minZ")) * 1e-6, atof(f("Domain", "maxZ")) * 1e-6
+        nx, ny = int(atof(f("Domain", "nx"))), int(atof(f("Domain", "ny")))
+        dx, dy = (maxX - minX) / (nx * 1.0), (maxY - minY) / (ny * 1.0)
+        nx, ny = nx + 1, ny + 1
+        if dim == 1:
+            nx, ny, dx, dy = 1, 1, 0.0, 0.0
+        num_slice = int(atof(f("Domain", "slices_in_bucket", "1")))
+        num_lu = int(atof(f("Domain", "lambdaUs_in_iteration", "1")))
+        gamma0 = atof(f("EBeam", "beam_energy")) / MC2_MEV + 1.0
+        lam_u = atof(f("Undulator", "lambdaU")) * 0.01
+        ku = 2 * math.pi / lam_u
+        ue = atof(f("Undulator", "polarity"))
+        K0 = atof(f("Undulator", "K0")) if f("Undulator", "K0") else math.sqrt(
+            (4 * gamma0 * gamma0 * ku / ks - 2) / (1.0 + ue * ue))
+        gamR = math.sqrt(0.5 * ks / ku * (1 + 0.5 * (1 + ue * ue) * K0 * K0))
+        nr = int(atof(f("Domain", "radial_grids"))) if f("Domain", "radial_grids") else int(math.sqrt(nx * nx + ny * ny))
+        dr = math.sqrt((maxX - minX) ** 2 + (maxY - minY) ** 2) / (1.0 * nx)
+        if dim == 1:
+            nr, dr = 1, 0.0
+        if static:
+            minZ, maxZ = 0.0, lambda0 * num_slice
+        dz = lam_u * num_lu
+        # seed (parameterSetting.cpp:198-229); D.polarity is never parsed (0)
+        Z0_OHM, E_CH, E_MASS = 376.730313668, 1.60217662e-19, 9.10938291e-31
+        spot = atof(f("Seed", "spot_sigma_R")) * 1e-6
+        load_h = int(atof(f("Seed", "loading_harmonics", "1")))
+        s_ue = atof(f("Seed", "polarity", "0"))
+        area = 2.0 * math.pi * spot * spot
+        a0 = math.sqrt(atof(f("Seed", "power")) * 2.0 * Z0_OHM / area) * E_CH / (E_MASS * C_LIGHT ** 2 * ks * load_h)
+        seed = dict(a0=a0, zR=2.0 * spot * spot * ks * load_h, spot_sig_r=spot, load_h=load_h,
+                    duration=atof(f("Seed", "rms_duration")) * 1e-15, focus=atof(f("Seed", "focus")),
+                    laser_alpha=atof(f("Seed", "laser_alpha", "-1")),
+                    laser_psi=float(np.angle((1.0 - s_ue * s_ue + 2j * s_ue) / (1.0 + s_ue * s_ue))))
+        return dict(seed=seed, nx=nx, ny=ny, dx=dx, dy=dy, min_x=minX, min_y=minY, min_z=minZ, max_z=maxZ, ks=ks,
+                    lambda0=lambda0, gamR=gamR, dr=dr, nr=nr, dz=dz, num_slice=num_slice,
+                    max_step=int(Lz / dz + 1), slice_n=int((maxZ - minZ) / (lambda0 * num_slice)),
+                    abc_n=int(atof(f("Domain", "ABC_N", "10"))), abc_sig=atof(f("Domain", "ABC_coef", "1")),
+                    bessel_n=int(atof(f("Bessel_table", "num_grids", "2001"))),
+                    sc_l=int(atof(f("Domain", "number_longitudinal_mode", "1"))),
+                    sc_f=int(atof(f("Domain", "number_azimuthal_mode", "1"))))
+
+    decks_d = {"deck_lattice.inp": deck}
+    for name in ("cfg0.inp", "cfg1.inp", "cfg2.inp", "cfg3.inp"):
+        decks_d[name] = ROOT / "felis_amd" / "decks" / name
+    (HERE / "deck_desc.json").write_text(json.dumps({k: derived(v) for k, v in decks_d.items()}, indent=1))
     print("wrote", len(out), "npz fixtures and findparam.json to", HERE)
 
 
