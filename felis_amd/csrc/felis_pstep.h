@@ -470,8 +470,10 @@ __global__ void __launch_bounds__(kTile)
 // Window cells on the boundary ring and the reference's flat-index spill
 // (A.5) use atomics; ring cells are never read by the batched field solve.
 // A window larger than the LDS budget for its harmonics goes through LDS in
-// strips of rows, one harmonic per pass; one of which not even a row fits
-// is first cleared in Sc and then deposited with global atomics.
+// strips of rows, one harmonic per pass. The host keeps the budget at one
+// node row of the widest window or more, so the branch below for a window of
+// which not even a row fits (clear in Sc, then global atomics) is a guard
+// that the loop does not take.
 // NPOL = 1 deposits the x polarisation only: the y source is exactly zero
 // (eta_Y = 0: a planar undulator with K0_alpha = 1) and Uy is zero.
 template <int NH, int NPOL>

@@ -411,10 +411,19 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
     ctx->err = "jumps need the LDS-resident ADI (grid too wide)";
     return FELIS_ERR_UNSUPPORTED;
   }
-  // the batched deposit adapts to any LDS budget (harmonics in chunks, else
-  // global atomics); the tuning may give it less than the device allows
+  // The batched deposit takes its windows through LDS in harmonic chunks or
+  // strips of node rows, and then builds the radial source there; the tuning
+  // may give it less LDS than the device allows. Its budget never goes below
+  // one node row of the widest window (nx cells of both polarisations) and
+  // one radial row (sc_f nr + 1 complex doubles), so both always fit.
+  const int dep_floor = std::max(d.nx * 2 * (int)sizeof(float) * 2, (d.sc_f * d.nr + 1) * 2 * (int)sizeof(double));
+  if (dep_floor > ctx->lds_max) {
+    ctx->err = "the radial source row (number_azimuthal_mode x radial_grids) exceeds the LDS of one workgroup";
+    return FELIS_ERR_UNSUPPORTED;
+  }
   int dep_lds = ctx->lds_max;
   if (ctx->tune.deposit_lds > 0) dep_lds = std::max(4096, std::min(ctx->lds_max, ctx->tune.deposit_lds));
+  dep_lds = std::max(dep_lds, (dep_floor + 255) / 256 * 256);
   if (deposit_lds_attr(d.n_harm, dep_lds) != hipSuccess) {
     ctx->err = "hipFuncSetAttribute(k_deposit_slot)";
     return FELIS_ERR_HIP;

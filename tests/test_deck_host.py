@@ -53,7 +53,7 @@ def test_deck_lattice_table_matches_the_reference():
     assert (seq[:, 4] > 0).any() and (seq[:, 4] < 0).any() and len(set(seq[:, 0])) > 100
 
 
-@pytest.mark.parametrize("name", ["deck_lattice.inp", "cfg0.inp", "cfg1.inp", "cfg2.inp", "cfg3.inp"])
+@pytest.mark.parametrize("name", ["deck_lattice.inp", "cfg0.inp", "cfg1.inp", "cfg2.inp", "cfg2s.inp", "cfg3.inp"])
 def test_deck_derived_scalars(name):
     """ks, lambda0, the grid (dx from the deck's nx, then nx + 1), the radial
     grid, gamR, dz, maxStep and sliceN (with the Static mode's window) as

@@ -5,9 +5,10 @@ Every tuning the autotuner may pick runs the same arithmetic in another
 launch shape or order: the field energy of every period and the final field
 agree with the default tuning's to 1e-5 relative (float atomics of the
 deposit summed in another order; the deposit's LDS budget of 32 KiB forces
-its harmonics into chunks, and 4 KiB holds no slice's window, so the
-deposit clears its window in Sc and adds with global atomics), and the
-particles' phases to 1e-4 rad.
+its harmonics into chunks, and at 4 KiB the windows go through LDS in strips
+of node rows, one harmonic per pass), and the particles' phases to 1e-4 rad.
+A 4 KiB budget with 256 radial grids is raised to hold the radial source row
+(257 complex doubles).
 """
 from pathlib import Path
 
@@ -25,9 +26,11 @@ SEED = 20261203
 N = 40
 
 
-def _run(t):
+def _run(t, nr=None):
     deck = fl.Deck(DECK)
     d = deck.desc(fl.MIXED_FP32)
+    if nr is not None:
+        d.nr = nr
     ctx = fl.Context(d)
     deck.load(ctx, seed=SEED)
     if t is not None:
@@ -41,6 +44,11 @@ def _run(t):
 @pytest.fixture(scope="module")
 def ref():
     return _run(None)
+
+
+@pytest.fixture(scope="module")
+def ref_nr256():
+    return _run(None, nr=256)
 
 
 @pytest.mark.parametrize("t", [dict(step_graph=1, deposit_threads=1024, deposit_lds=0),
@@ -57,6 +65,16 @@ def test_tunings_agree_with_default(ref, t):
     assert tune._rel_energy_err(r["e"], ref["e"]) < 1e-5
     assert np.max(np.abs(r["U"] - ref["U"])) < 1e-5 * np.max(np.abs(ref["U"]))
     assert np.max(np.abs(r["theta"] - ref["theta"])) < 1e-4
+
+
+def test_radial_row_fits_a_small_budget(ref_nr256):
+    """256 radial grids (the largest the context accepts) at a 4 KiB deposit
+    budget: the radial row (4112 bytes) does not fit 4 KiB, so the run raises
+    the budget to hold it, and agrees with the default tuning."""
+    r = _run(dict(deposit_threads=512, deposit_lds=4096), nr=256)
+    assert tune._rel_energy_err(r["e"], ref_nr256["e"]) < 1e-5
+    assert np.max(np.abs(r["U"] - ref_nr256["U"])) < 1e-5 * np.max(np.abs(ref_nr256["U"]))
+    assert np.max(np.abs(r["theta"] - ref_nr256["theta"])) < 1e-4
 
 
 def test_set_tuning_rejects_bad_shapes():

@@ -746,3 +746,21 @@ def test_drift_closed_forms():
     np.testing.assert_allclose(st.theta[inside] - th0[inside], want[inside], rtol=1e-11)
     assert st.theta[2] == th0[2]
     np.testing.assert_array_equal(st.gamma, g0)
+
+
+def test_guard_and_nan_in_the_oracle():
+    """particlePush.cpp:370-374: a field whose ponderomotive term alone moves
+    theta by more than 2 pi numSlice in one period trips the guard (the
+    reference exits; the oracle returns ORC_ERR_GUARD). A NaN theta passes the
+    reference's push and wrap unnoticed (NaN fails both comparisons of
+    rearrangeParticles.cpp:105); the product's error word adds that check."""
+    import fel_cases as fc
+    case = fc.make_case(slice_n=3, ppslice=64, harmonics=(1, 3))
+    case.state["Ux"][:, 2] *= 2e6
+    o = case.oracle_state()
+    assert o.push_theta_gamma(case.lattice()) == -3
+    case = fc.make_case(slice_n=3, ppslice=64, harmonics=(1, 3))
+    case.state["theta"][17] = np.nan
+    o = case.oracle_state()
+    assert o.push_theta_gamma(case.lattice()) == 0
+    assert o.periodic() == 0 and np.isnan(o.theta[17])
