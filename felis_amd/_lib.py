@@ -27,7 +27,7 @@ EXPORTS = [
     "felis_slice_energy", "felis_solve_field", "felis_transverse_push", "felis_push_theta_gamma",
     "felis_drift_theta_gamma", "felis_periodic_particles", "felis_shift_field", "felis_update_wake",
     "felis_set_wake", "felis_update_total_energy", "felis_update_bfactor", "felis_calculate_twiss",
-    "felis_run", "felis_find_parameter", "felis_deck_open", "felis_deck_close", "felis_deck_error",
+    "felis_run", "felis_run_noalloc", "felis_find_parameter", "felis_deck_open", "felis_deck_close", "felis_deck_error",
     "felis_deck_desc", "felis_deck_steps", "felis_deck_lattice", "felis_deck_wake_function",
     "felis_deck_beam_layout", "felis_deck_load_beam", "felis_deck_load_seed", "felis_profile",
     "felis_profile_read", "felis_device_info", "felis_set_beam_sigma", "felis_deck_beam_sigma",
@@ -131,6 +131,8 @@ def load() -> C.CDLL:
         L.felis_update_bfactor.argtypes = [C.c_void_p, _dp, _dp]
         L.felis_calculate_twiss.argtypes = [C.c_void_p, _dp]
         L.felis_run.argtypes = [C.c_void_p, C.POINTER(Lattice), C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+        L.felis_run_noalloc.argtypes = [C.c_void_p, C.POINTER(Lattice), C.c_int, C.c_int, C.c_int, _dp, _dp, _dp,
+                                        _dp]
         L.felis_state_save.argtypes = [C.c_void_p, C.c_int]
         L.felis_state_restore.argtypes = [C.c_void_p, C.c_int]
         L.felis_slice_moments.argtypes = [C.c_void_p, _dp]

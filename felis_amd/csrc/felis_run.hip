@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -71,7 +72,7 @@ int launch_adi_scan(felis_ctx *ctx, const cpx<float> *tab, int cf, const cpx<flo
   const size_t lds = adi_scan_lds(d.nx, d.ny, cw);
   const int E = adi_scan_E(d.nx, d.ny);
   cpx<float> *U = (cpx<float> *)ctx->U;
-  cpx<float> *scratch = (cpx<float> *)ctx->Uc;  // columns solved second when a system is split
+  cpx<float> *split = (cpx<float> *)ctx->Uc;  // columns solved second when a system is split
   KTimer kt(ctx, FELIS_K_ADI_FUSED);
   // 1024 threads need the kernel in 128 registers: whole systems of E <= 9
   const bool wide = ctx->tune.adi_threads == 1024 && E <= 9 && cw == d.nx - 2;
@@ -79,16 +80,16 @@ int launch_adi_scan(felis_ctx *ctx, const cpx<float> *tab, int cf, const cpx<flo
   case e:                                                                                                   \
     if (wide)                                                                                               \
       hipLaunchKernelGGL((k_adi_scan<e, 1024>), dim3(nsys), dim3(1024), lds, ctx->stream, g, tab, cf, U, Sc, \
-                         win, ctx->slice_e, cw, scratch);                                             \
+                         win, ctx->slice_e, cw, split);                                               \
     else                                                                                                    \
       hipLaunchKernelGGL((k_adi_scan<e, 512>), dim3(nsys), dim3(512), lds, ctx->stream, g, tab, cf, U, Sc,   \
-                         win, ctx->slice_e, cw, scratch);                                             \
+                         win, ctx->slice_e, cw, split);                                               \
     break;
   switch (E) {
     ADI_CASE(1) ADI_CASE(2) ADI_CASE(3) ADI_CASE(4) ADI_CASE(5) ADI_CASE(6) ADI_CASE(7) ADI_CASE(8) ADI_CASE(9)
     case 10:
       hipLaunchKernelGGL((k_adi_scan<10, 512>), dim3(nsys), dim3(512), lds, ctx->stream, g, tab, cf, U, Sc, win,
-                         ctx->slice_e, cw, scratch);
+                         ctx->slice_e, cw, split);
       break;
     default:
       ctx->err = "grid too wide for the LDS-resident ADI";
@@ -395,10 +396,22 @@ struct StepGraph {
   }
 };
 
+// Per-call buffers of the batched loop handed in by the caller instead of
+// allocated (felis_run_noalloc): tile partial sums, fixed-order reduction
+// parts, the first solve's energy scratch, the species' first tiles and the
+// ADI scan tables (adi_tab_elems entries).
+struct RunScratch {
+  double *tile_red, *red_parts, *scratch_e;
+  int *species_tile;
+  cpx<float> *adi_tab;
+  size_t adi_tab_elems;
+};
+
 // The batched FP32/FP64 loop; with jc, iterations whose gate is open advance
-// J periods at once (felis_jump.h).
+// J periods at once (felis_jump.h). With ext the loop allocates nothing: its
+// buffers come from ext and the captured-step replay is not used.
 int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps, int wake_step,
-              const double *wakeF, double *dev_out, size_t per, JumpCtl *jc) {
+              const double *wakeF, double *dev_out, size_t per, JumpCtl *jc, const RunScratch *ext = nullptr) {
   const felis_desc &d = ctx->d;
   if (d.sc_l != 1) {
     ctx->err = "the batched path models one longitudinal space-charge mode (number_longitudinal_mode = 1)";
@@ -435,13 +448,16 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
   cpx<float> *adi_tab = nullptr;
   std::vector<cpx<float>> h_tab;
   adi_scan_tables(d.nx, d.ny, nh, ctx->h_adix, ctx->h_adiy, ctx->h_abc, h_tab);
+  const bool own = ext == nullptr;
   auto cleanup = [&]() {
     (void)hipStreamSynchronize(ctx->stream);
-    (void)hipFree(tile_red);
-    (void)hipFree(red_parts);
-    (void)hipFree(species_tile);
-    (void)hipFree(adi_tab);
-    (void)hipFree(scratch_e);
+    if (own) {
+      (void)hipFree(tile_red);
+      (void)hipFree(red_parts);
+      (void)hipFree(species_tile);
+      (void)hipFree(adi_tab);
+      (void)hipFree(scratch_e);
+    }
     if (jc) {
       (void)hipFree(jc->tab);
       (void)hipFree(jc->M);
@@ -451,7 +467,25 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
     ctx->dz_mul = 1.0;
   };
 
-  if (hipMalloc((void **)&tile_red, (size_t)std::max(1, ctx->ntiles) * nred * sizeof(double)) != hipSuccess ||
+  if (!own) {
+    if (h_tab.size() > ext->adi_tab_elems) {
+      ctx->err = "ADI tables larger than the scratch carved for them";
+      return FELIS_ERR_ARG;
+    }
+    tile_red = ext->tile_red;
+    red_parts = ext->red_parts;
+    scratch_e = ext->scratch_e;
+    species_tile = ext->species_tile;
+    adi_tab = ext->adi_tab;
+    if (hipMemcpyAsync(adi_tab, h_tab.data(), h_tab.size() * sizeof(cpx<float>), hipMemcpyHostToDevice,
+                       ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(species_tile, ctx->species_tile.data(), (ns + 1) * sizeof(int), hipMemcpyHostToDevice,
+                       ctx->stream) != hipSuccess) {
+      cleanup();
+      ctx->err = "batched step tables";
+      return FELIS_ERR_HIP;
+    }
+  } else if (hipMalloc((void **)&tile_red, (size_t)std::max(1, ctx->ntiles) * nred * sizeof(double)) != hipSuccess ||
       hipMalloc((void **)&species_tile, (ns + 1) * sizeof(int)) != hipSuccess ||
       hipMalloc((void **)&adi_tab, h_tab.size() * sizeof(cpx<float>)) != hipSuccess ||
       hipMalloc((void **)&scratch_e, 2 * nh * sizeof(double)) != hipSuccess ||
@@ -461,15 +495,15 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
     ctx->err = "batched step buffers";
     return FELIS_ERR_HIP;
   }
-  if (hipMemcpy(species_tile, ctx->species_tile.data(), (ns + 1) * sizeof(int), hipMemcpyHostToDevice) !=
-      hipSuccess) {
+  if (own && hipMemcpy(species_tile, ctx->species_tile.data(), (ns + 1) * sizeof(int), hipMemcpyHostToDevice) !=
+                 hipSuccess) {
     cleanup();
     ctx->err = "species_tile upload";
     return FELIS_ERR_HIP;
   }
   // replay one captured step over the uniform stretches of the run when the
   // tuning asks for it, unless kernels are being timed one by one
-  const bool use_graph = fused && !ctx->profiling && ctx->tune.step_graph;
+  const bool use_graph = fused && !ctx->profiling && ctx->tune.step_graph && own;
   StepGraph sg;
   int *ring_dev = nullptr;
   if (use_graph && hipMalloc((void **)&ring_dev, 2 * sizeof(int)) != hipSuccess) {
@@ -757,6 +791,28 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
   return err;
 }
 
+// The plain FP64 loop: the per-phase operators of every step in the
+// reference's order, the step's outputs into row k of dev (per doubles).
+int run_plain(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps, int wake_step,
+              const double *wakeF, double *dev, size_t per, const double *energy, const double *bfac,
+              const double *twiss) {
+  const felis_desc &d = ctx->d;
+  const size_t ne = 2 * (size_t)d.n_harm, nb = 2 * (size_t)d.n_harm + 1;
+  int err = FELIS_OK;
+  for (int k = 0; k < n_steps && !err; ++k) {
+    int it = it0 + k;
+    if (d.mode == 2 && wakeF && wake_step > 0 && it % wake_step == 0) {
+      err = ctx->update_wake(wakeF, nullptr);
+      if (err) break;
+    }
+    double *row = dev + (size_t)k * per;
+    err = enqueue_step(ctx, &lat_tab[2 * k], energy ? row : nullptr, bfac ? row + ne : nullptr,
+                       twiss ? row + ne + nb : nullptr);
+    if (!err && ((k & 31) == 31 || k == n_steps - 1)) err = ctx->check_err();
+  }
+  return err;
+}
+
 // Shared by felis_run and felis_run_surrogate: the loop, then the per-step
 // outputs copied out of one device buffer (rows of periods inside a jump
 // are NaN).
@@ -778,17 +834,7 @@ int run_common(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_step
       err = run_mixed(ctx, lat_tab, it0, n_steps, wake_step, wakeF, dev, per, jc);
     }
   } else {
-    for (int k = 0; k < n_steps && !err; ++k) {
-      int it = it0 + k;
-      if (d.mode == 2 && wakeF && wake_step > 0 && it % wake_step == 0) {
-        err = ctx->update_wake(wakeF, nullptr);
-        if (err) break;
-      }
-      double *row = dev + (size_t)k * per;
-      err = enqueue_step(ctx, &lat_tab[2 * k], energy ? row : nullptr, bfac ? row + ne : nullptr,
-                         twiss ? row + ne + nb : nullptr);
-      if (!err && ((k & 31) == 31 || k == n_steps - 1)) err = ctx->check_err();
-    }
+    err = run_plain(ctx, lat_tab, it0, n_steps, wake_step, wakeF, dev, per, energy, bfac, twiss);
   }
   if (!err) {
     std::vector<double> h((size_t)n_steps * per);
@@ -876,4 +922,121 @@ extern "C" int felis_get_tuning(felis_ctx *ctx, felis_tuning *t) {
   if (!ctx || !t) return FELIS_ERR_ARG;
   *t = ctx->tune;
   return FELIS_OK;
+}
+
+// ------------------------------------------------------------------------
+// felis_run_noalloc: felis_run with its per-call device buffers taken from a
+// file-scope device array instead of hipMalloc. The array lives in every
+// device the library runs on, for the life of the process; its address on a
+// device comes from hipGetSymbolAddress, taken once per device with that
+// device current, and the kernels only ever see that address. Every call on a
+// device shares the array and leaves its contents behind: the loop clears
+// what it sums into with fill kernels, and calls that overlap on one device
+// race.
+constexpr size_t kNoAllocBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_dev[kMaxDevices] = {};
+
+// the device address of `scratch` on the current device
+int scratch_address(felis_ctx *ctx, unsigned char **out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    ctx->err = "felis_run_noalloc: no current device";
+    return FELIS_ERR_HIP;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      ctx->err = std::string("felis_run_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return FELIS_ERR_HIP;
+    }
+    g_scratch_dev[dev] = (unsigned char *)p;
+  }
+  *out = g_scratch_dev[dev];
+  return FELIS_OK;
+}
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+}  // namespace
+
+extern "C" int felis_run_noalloc(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps,
+                                 int wake_step, const double *wakeF, double *energy, double *bfac, double *twiss) {
+  if (!ctx || !lat_tab || n_steps < 0) return FELIS_ERR_ARG;
+  const felis_desc &d = ctx->d;
+  const int nh = d.n_harm, ns = d.n_species;
+  const size_t ne = 2 * (size_t)nh, nb = 2 * (size_t)nh + 1, nt = 7 * (size_t)ns, per = ne + nb + nt;
+  if (ctx->mixed() && !ctx->np) {
+    ctx->err = "no particles uploaded";
+    return FELIS_ERR_STATE;
+  }
+  // the carve: output rows, then (batched) tile sums, reduction parts,
+  // energy scratch, species tiles and the ADI scan tables
+  size_t need = align256(std::max<size_t>(1, (size_t)n_steps * per) * sizeof(double));
+  const size_t off_tile = need;
+  RunScratch rs{};
+  std::vector<cpx<float>> h_tab;
+  size_t off_parts = 0, off_e = 0, off_sp = 0, off_tab = 0;
+  if (ctx->mixed()) {
+    adi_scan_tables(d.nx, d.ny, nh, ctx->h_adix, ctx->h_adiy, ctx->h_abc, h_tab);
+    need += align256((size_t)std::max(1, ctx->ntiles) * (2 * nh + 8) * sizeof(double));
+    off_parts = need;
+    need += align256((size_t)(4 * nh + 1 + 7 * ns) * kRedParts * sizeof(double));
+    off_e = need;
+    need += align256(2 * (size_t)nh * sizeof(double));
+    off_sp = need;
+    need += align256((size_t)(ns + 1) * sizeof(int));
+    off_tab = need;
+    need += align256(std::max<size_t>(1, h_tab.size()) * sizeof(cpx<float>));
+  }
+  if (need > kNoAllocBytes) {
+    ctx->err = "felis_run_noalloc: the run needs " + std::to_string(need) + " bytes of scratch (" +
+               std::to_string(n_steps) + " periods, " + std::to_string(ctx->ntiles) +
+               " particle tiles); the library's array holds " + std::to_string(kNoAllocBytes) +
+               ": use felis_run or fewer periods per call";
+    return FELIS_ERR_ARG;
+  }
+  unsigned char *base = nullptr;
+  int err = scratch_address(ctx, &base);
+  if (err) return err;
+  double *dev = (double *)base;
+  if (ctx->mixed()) {
+    rs.tile_red = (double *)(base + off_tile);
+    rs.red_parts = (double *)(base + off_parts);
+    rs.scratch_e = (double *)(base + off_e);
+    rs.species_tile = (int *)(base + off_sp);
+    rs.adi_tab = (cpx<float> *)(base + off_tab);
+    rs.adi_tab_elems = h_tab.size();
+    err = run_mixed(ctx, lat_tab, it0, n_steps, wake_step, wakeF, dev, per, nullptr, &rs);
+  } else {
+    err = run_plain(ctx, lat_tab, it0, n_steps, wake_step, wakeF, dev, per, energy, bfac, twiss);
+  }
+  // the outputs straight from the rows into the caller's arrays
+  const struct {
+    double *dst;
+    size_t off, n;
+  } outs[3] = {{energy, 0, ne}, {bfac, ne, nb}, {twiss, ne + nb, nt}};
+  for (int q = 0; q < 3 && !err && n_steps > 0; ++q) {
+    if (!outs[q].dst) continue;
+    hipError_t he = hipMemcpy2DAsync(outs[q].dst, outs[q].n * sizeof(double), dev + outs[q].off,
+                                     per * sizeof(double), outs[q].n * sizeof(double), (size_t)n_steps,
+                                     hipMemcpyDeviceToHost, ctx->stream);
+    if (he != hipSuccess) {
+      ctx->err = std::string("run outputs: ") + hipGetErrorString(he);
+      err = FELIS_ERR_HIP;
+    }
+  }
+  const hipError_t he = hipStreamSynchronize(ctx->stream);
+  if (!err && he != hipSuccess) {
+    ctx->err = std::string("run outputs: ") + hipGetErrorString(he);
+    err = FELIS_ERR_HIP;
+  }
+  return err;
 }

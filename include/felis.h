@@ -162,6 +162,18 @@ int felis_calculate_twiss(felis_ctx *ctx, double *sums /* [n_species][7] */);
  * total weight), twiss[k][n_species*7]. wakeF (slice_n) may be NULL. */
 int felis_run(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps, int wake_step,
               const double *wakeF, double *energy, double *bfac, double *twiss);
+/* felis_run that allocates no device memory. No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The same
+ * loop and outputs as felis_run; its per-call device buffers (output rows,
+ * tile partial sums, reduction parts, ADI scan tables) come from one 16 MiB
+ * array in the library that each device holds for the life of the process.
+ * A run needing more (about n_steps * (32 n_harm + 56 n_species + 8) bytes
+ * plus, batched, particle tiles * (16 n_harm + 64) bytes) returns
+ * FELIS_ERR_ARG. The captured-step replay is not used. Calls that overlap on
+ * one device share the array and race. */
+int felis_run_noalloc(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps, int wake_step,
+                      const double *wakeF, double *energy, double *bfac, double *twiss);
 
 /* -------- multi-period jumps (no counterpart in the reference) ---------- */
 /* While the gate is open, one iteration advances every slice `periods`
