@@ -102,7 +102,7 @@ def run_gpu(args, precision=None, steps=None, warmup=None, keep_init=None):
         from felis_amd import tune
         tuning = tune.apply(ctx, args.tuning)
     init = None
-    if (not args.no_cpu) if keep_init is None else keep_init:
+    if (not args.no_cpu or not args.no_whole_run) if keep_init is None else keep_init:
         init = dict(particles=ctx.download_particles(), Ux=ctx.download_field(fl.UX), Uy=ctx.download_field(fl.UY))
     W = args.warmup if warmup is None else warmup
     K = args.steps if steps is None else steps
@@ -130,25 +130,51 @@ def run_gpu(args, precision=None, steps=None, warmup=None, keep_init=None):
                 wakeF=wakeF, np=ctx.np, npol=ctx.live_polarisations() if precision == "mixed" else 2)
 
 
+def host_cores():
+    """(physical cores of the host from lscpu, CPUs this process may run on)."""
+    import subprocess
+    phys = None
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, timeout=10).stdout
+        cores = {tuple(l.split(",")) for l in out.splitlines() if l and not l.startswith("#")}
+        phys = len(cores) or None
+    except (OSError, subprocess.SubprocessError):
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return phys, avail
+
+
 def run_cpu(args, g):
-    """The FP64 oracle on the first steps of the same workload (bounded sample)."""
+    """The FP64 oracle on a bounded sample of the same workload: one untimed
+    period from the initial state (period 0 of main.cpp:92-156 has no
+    deposit), then args.cpu_steps timed steady-state periods. Threads: the
+    CPU share this process is given (the physical cores of the host, capped
+    by the CPUs it may use and OMP_NUM_THREADS), one OpenMP thread per core
+    over the slices, as the reference's ranks split them."""
     from oracle import pyoracle as po
     d = g["desc"]
-    threads = int(args.cpu_threads or min(16, os.cpu_count() or 1))
+    phys, avail = host_cores()
+    threads = min(x for x in (phys, avail, int(os.environ.get("OMP_NUM_THREADS", "0")) or None) if x)
+    threads = int(args.cpu_threads or threads)
     po.set_threads(threads)
     seg_off, seg_w = g["deck"].beam_layout(d.n_species)
     st = po.state_from_desc(d, seg_off, seg_w, g["init"]["particles"], g["init"]["Ux"], g["init"]["Uy"])
-    lat = po.lattice_table(g["tab"][:2 * args.cpu_steps + 1])
+    n = 1 + args.cpu_steps
+    lat = po.lattice_table(g["tab"][:2 * n + 1])
+    rc, e0, _, _ = st.run(lat[:3], 0, 1, d.mode == 2, g["deck"].wake_step, g["wakeF"])
+    if rc != 0:
+        raise RuntimeError(f"oracle run failed: {rc}")
     t0 = time.perf_counter()
-    rc, energy, _, _ = st.run(lat, 0, args.cpu_steps, d.mode == 2, g["deck"].wake_step, g["wakeF"])
+    rc, e1, _, _ = st.run(lat[2:], 1, args.cpu_steps, d.mode == 2, g["deck"].wake_step, g["wakeF"])
     t1 = time.perf_counter()
     if rc != 0:
         raise RuntimeError(f"oracle run failed: {rc}")
-    return dict(seconds=t1 - t0, threads=threads, energy=energy)
+    return dict(seconds=t1 - t0, threads=threads, host_physical_cores=phys, host_cpus_available=avail,
+                energy=np.concatenate([e0, e1]))
 
 
 def gpu_parity(args, g, cpu):
-    """GPU from the same initial state over the sampled steps vs the oracle."""
+    """GPU from the same initial state over the sampled periods vs the oracle."""
     from felis_amd import _lib as fl
     d = g["desc"]
     ctx = fl.Context(d)
@@ -158,12 +184,57 @@ def gpu_parity(args, g, cpu):
                          part["gamma"])
     ctx.upload_field(fl.UX, g["init"]["Ux"])
     ctx.upload_field(fl.UY, g["init"]["Uy"])
-    e, _, _ = ctx.run(g["tab"][:2 * args.cpu_steps + 1], 0, args.cpu_steps, g["deck"].wake_step, g["wakeF"])
+    n = 1 + args.cpu_steps
+    e, _, _ = ctx.run(g["tab"][:2 * n + 1], 0, n, g["deck"].wake_step, g["wakeF"])
     ctx.close()
     nh = d.n_harm
     tot_g = e[:, :nh].sum(1) + e[:, nh:].sum(1)
     tot_o = cpu["energy"][:, :nh].sum(1) + cpu["energy"][:, nh:].sum(1)
     return float(np.max(np.abs(tot_g - tot_o) / np.abs(tot_o)))
+
+
+def whole_run_parity(args, g):
+    """The batched path against the plain FP64 path (itself oracle-matched to
+    1e-9) over every period of the deck, both on the GPU from the initial
+    state: the largest relative error of the integrated power of each
+    harmonic over the run, and of the fundamental's slice power at its end
+    (slices above 1e-3 of the peak)."""
+    from felis_amd import _lib as fl
+    d = g["desc"]
+    deck = g["deck"]
+    n = deck.max_step
+    tab = deck.lattice(0, n)
+    out = {}
+    curves = {}
+    slices = {}
+    for prec in (fl.MIXED_FP32, fl.PLAIN_FP64):
+        ctx = fl.Context(deck.desc(prec))
+        part = g["init"]["particles"]
+        seg_off, seg_w = deck.beam_layout(d.n_species)
+        ctx.upload_particles(seg_off, seg_w, part["x"], part["y"], part["px"], part["py"], part["theta"],
+                             part["gamma"])
+        ctx.upload_field(fl.UX, g["init"]["Ux"])
+        ctx.upload_field(fl.UY, g["init"]["Uy"])
+        if prec == fl.MIXED_FP32:
+            from felis_amd import tune
+            tune.apply(ctx, args.tuning)
+        e, _, _ = ctx.run(tab, 0, n, deck.wake_step, g["wakeF"])
+        curves[prec] = e
+        slices[prec] = ctx.slice_energy()[0, 0]
+        ctx.close()
+    nh = d.n_harm
+    em, ep = curves[fl.MIXED_FP32], curves[fl.PLAIN_FP64]
+    for h in range(nh):
+        ref = ep[:, h] + ep[:, nh + h]
+        on = ref > 0
+        err = np.abs(em[on, h] + em[on, nh + h] - ref[on]) / ref[on]
+        out[f"max_rel_err_power_h{d.harmony[h]}"] = float(err.max())
+    sp, sm = slices[fl.PLAIN_FP64], slices[fl.MIXED_FP32]
+    strong = sp > 1e-3 * sp.max()
+    out["max_rel_err_slice_power_h1_end"] = float(np.max(np.abs(sm[strong] - sp[strong]) / sp[strong]))
+    out["periods"] = n
+    out["against"] = "plain FP64 path on the GPU (oracle-matched to 1e-9 in tests/test_gpu_workloads.py)"
+    return out
 
 
 def surrogate_runs(args, periods: int = 10, gate: float = 0.1, net_gate: float = 0.05):
@@ -249,6 +320,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-whole-run", action="store_true",
+                    help="skip the whole-run comparison of the batched path against the plain FP64 path")
     ap.add_argument("--plain-steps", type=int, default=10,
                     help="periods of the plain FP64 path timed beside a mixed run (0: skip)")
     ap.add_argument("--no-surrogate", action="store_true",
@@ -321,9 +394,16 @@ def main():
         cpu = run_cpu(args, g)
         out["cpu_baseline"] = {"value": d.slice_n * args.cpu_steps / cpu["seconds"], "unit": "slice·periods/s",
                                "cores": cpu["threads"], "kind": "port",
-                               "sample": f"first {args.cpu_steps} periods of the same workload and initial state, "
-                                         f"FP64 oracle (oracle/felis_oracle.c), {cpu['threads']} threads"}
-        out["parity"] = {"steps": args.cpu_steps, "max_rel_err_total_energy": gpu_parity(args, g, cpu)}
+                               "host_physical_cores": cpu["host_physical_cores"],
+                               "host_cpus_available": cpu["host_cpus_available"],
+                               "thread_model": "one process, OpenMP threads over the slices (the reference splits "
+                                               "slices over MPI ranks, boundary.cpp:33-46)",
+                               "sample": f"periods 1-{args.cpu_steps} (steady state, after an untimed period 0) of "
+                                         f"the same workload and initial state, FP64 oracle "
+                                         f"(oracle/felis_oracle.c), {cpu['threads']} threads"}
+        out["parity"] = {"steps": 1 + args.cpu_steps, "max_rel_err_total_energy": gpu_parity(args, g, cpu)}
+    if mixed and not args.no_whole_run:
+        out.setdefault("parity", {})["whole_run_vs_plain_fp64"] = whole_run_parity(args, g)
     g["ctx"].close()
     print(json.dumps(out), flush=True)
 
