@@ -20,7 +20,8 @@
 
 namespace felis {
 
-constexpr int kRun = 8;  // consecutive particles one deposit thread combines
+constexpr int kRun = 8;       // consecutive particles one deposit thread combines
+constexpr int kMaxModes = 4;  // longitudinal space-charge modes of the batched path
 
 // transversePush constants of one lattice row (particlePush.cpp:76-90),
 // formed on the host: the same for every particle of a launch
@@ -197,7 +198,7 @@ __device__ inline void transverse_f32(const TransArgs &t, float dz, float gam, f
 // RK stages' changes of it in FP32, gamma and theta in FP64. One sincos per particle: the RK stages rotate it by their
 // small phase increments; harmonics by complex powers. On return eth_out
 // holds e^{i theta} of the new theta (when the particle was pushed: true).
-template <int NH, bool UY>
+template <int NH, bool UY, bool LM>
 __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const LatArgs &l, int slot, double wakeE,
                                    const cpx<float> *__restrict__ Ux, const cpx<float> *__restrict__ Uy,
                                    const cpx<double> *__restrict__ Ez, const float *__restrict__ bes, float x0,
@@ -217,19 +218,30 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
   const double K2 = 1.0 + (double)pr2 + a.K2a * (1.0 + a.K2b * (double)r2);
   const double ig0 = rcp_f64(gam0);
   const float xi = (float)(a.xic * ig0 * ig0);
-  // longitudinal space charge, first mode (sc_l = 1 in the batched path)
-  cpx<float> Em = mk(0.f, 0.f);
+  // longitudinal space charge, modes l = 0 .. L-1 (particlePush.cpp:287-304);
+  // on the axis the two radial points are summed unweighted. LM: more than
+  // one mode (otherwise the first only, in fewer registers)
+  constexpr int ML = LM ? kMaxModes : 1;
+  cpx<float> Em[ML];
+  const int L = LM ? g.L : 1;
   {
     const int F = g.F, nr = g.nr;
+    const long long es = (long long)g.S * F * nr;
     const double r = sqrt((double)r2);
     const double rr = r * a.inv_dr;
     const int idxR = (int)(rr + 0.5);
+#pragma unroll
+    for (int l = 0; l < ML; ++l) Em[l] = mk(0.f, 0.f);
     if (idxR >= 0 && idxR < nr) {
       const float wr1 = idxR > 0 ? __fdividef((float)(rr * rr - (double)idxR * idxR), 2.f * idxR) : 1.f;
       const float wr0 = idxR > 0 ? 1.f - wr1 : 1.f;
-      for (int f = 0; f < F; ++f) {
-        const cpx<double> *p = Ez + (long long)slot * nr * F + nr * f + idxR;
-        Em = Em + cvt<float>(p[0]) * wr0 + cvt<float>(p[1]) * wr1;
+#pragma unroll
+      for (int l = 0; l < ML; ++l) {
+        if (l >= L) break;
+        for (int f = 0; f < F; ++f) {
+          const cpx<double> *p = Ez + l * es + (long long)slot * nr * F + nr * f + idxR;
+          Em[l] = Em[l] + cvt<float>(p[0]) * wr0 + cvt<float>(p[1]) * wr1;
+        }
       }
     } else {
       atomicOr(err, EB_EMCARRY);
@@ -330,7 +342,14 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
       sumG += t.re * igm;
     }
     const cpx<float> eth = e1 * rotm;  // e^{i theta}
-    const float sumEz = 2.0f * (Em.re * eth.re - Em.im * eth.im);
+    float sumEz = 2.0f * (Em[0].re * eth.re - Em[0].im * eth.im);
+    cpx<float> ethl = eth;  // e^{i (l+1) theta}
+#pragma unroll
+    for (int l = 1; l < ML; ++l) {
+      if (l >= L) break;
+      ethl = ethl * eth;
+      sumEz += 2.0f * (Em[l].re * ethl.re - Em[l].im * ethl.im);
+    }
     const float r2 = r * r;
     // kth - rate0
     const float small = Akf * eps * (2.f + eps) * r2 - (c20 * r2) * (sumU2 + sq * sumTh);
@@ -352,8 +371,9 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
 }
 
 // One workgroup of kTile threads per tile, one particle per thread. UY =
-// false: Uy is exactly zero (see k_deposit_slot) and is not read.
-template <int NH, bool UY>
+// false: Uy is exactly zero (see k_deposit_slot) and is not read. LM: more
+// than one longitudinal space-charge mode.
+template <int NH, bool UY, bool LM>
 __global__ void __launch_bounds__(kTile)
     k_pstep(GridArgs g0, PStepArgs a, Tiles tl, double *__restrict__ theta, float *__restrict__ gamma,
             float *__restrict__ x, float *__restrict__ y, float *__restrict__ px, float *__restrict__ py,
@@ -385,7 +405,7 @@ __global__ void __launch_bounds__(kTile)
     cpx<float> eth = mk(1.f, 0.f);
     bool have_eth = false;
     if (!a.drift) {
-      have_eth = push_tg_f32<NH, UY>(g, a, a.l1, slot, wk, Ux, Uy, Ez, bes, xx, yy, ppx, ppy, th, gm, eth, err);
+      have_eth = push_tg_f32<NH, UY, LM>(g, a, a.l1, slot, wk, Ux, Uy, Ez, bes, xx, yy, ppx, ppy, th, gm, eth, err);
     } else {
       const double fx_ = ((double)xx - g.minX) * a.inv_dx, fy_ = ((double)yy - g.minY) * a.inv_dy;
       const int idxI = (int)fx_, idxJ = (int)fy_;
@@ -476,7 +496,7 @@ __global__ void __launch_bounds__(kTile)
 // that the loop does not take.
 // NPOL = 1 deposits the x polarisation only: the y source is exactly zero
 // (eta_Y = 0: a planar undulator with K0_alpha = 1) and Uy is zero.
-template <int NH, int NPOL>
+template <int NH, int NPOL, bool LM>
 __global__ void __launch_bounds__(1024)
     k_deposit_slot(GridArgs g, DepArgs d, int n_species, const long long *__restrict__ seg_off,
                    const double *__restrict__ seg_w, const double *__restrict__ theta,
@@ -747,12 +767,14 @@ __global__ void __launch_bounds__(1024)
   // of radial bins, which one shared copy would serialise); consecutive
   // particles at one radius (a beamlet) are summed in registers first. The
   // copies are added in a fixed order.
-  const int F = g.F, nr = g.nr, row = F * nr + 1;
+  // Modes l = 0 .. L-1 are rows l * row of the slot's source (L > 1 or F > 1
+  // deposit particle by particle).
+  const int F = g.F, nr = g.nr, L = LM ? g.L : 1, row = F * nr + 1, rows = L * row;
   const int nwv = nt >> 6, wid = tid >> 6;
   cpx<double> *hist = reinterpret_cast<cpx<double> *>(lds_d);
-  const bool priv = (long long)nwv * row * (long long)sizeof(cpx<double>) <= lds_bytes;
-  cpx<double> *mine = priv ? hist + (long long)wid * row : hist;
-  for (int q = tid; q < (priv ? nwv : 1) * row; q += nt) hist[q] = mk(0.0, 0.0);
+  const bool priv = (long long)nwv * rows * (long long)sizeof(cpx<double>) <= lds_bytes;
+  cpx<double> *mine = priv ? hist + (long long)wid * rows : hist;
+  for (int q = tid; q < (priv ? nwv : 1) * rows; q += nt) hist[q] = mk(0.0, 0.0);
   __syncthreads();
   {
     const int seg = (n_species - 1) * g.slice_n + sl;
@@ -781,22 +803,27 @@ __global__ void __launch_bounds__(1024)
         const int idxR = (int)(rr * inv_dr + 0.5);
         const cpx<float> em = conjg(expi_f32(theta[n]));  // e^{-i theta}
         if (idxR > 0 && idxR < nr) {
-          if (F > 1) {
+          if (F > 1 || (LM && L > 1)) {
+            // azimuthal modes need the angle of every particle, longitudinal
+            // modes (l+1) theta: no combining. Mode l adds
+            // i (l+1) coef_Ez macro / idxR e^{-i((l+1) theta + f phi)}
+            // (solveField.cpp:340-347)
             const double amp = d.coef_Ez * macro / (1.0 * idxR);
-            // azimuthal modes need the angle of every particle: no combining
             FELIS_RADIAL_FLUSH();
             key = -1;
             const double ph = (xx == 0.0) ? 0.0 : atan2(yy, xx);
             const double w1 = (rr * rr * inv_dr * inv_dr - idxR * idxR) / (2.0 * idxR), w0 = 1.0 - w1;
-            for (int f = 0; f < F; ++f) {
-              double sn, cs;
-              sincos(-(theta[n] + f * ph), &sn, &cs);
-              cpx<double> *hp2 = mine + f * nr + idxR;
-              atomicAdd(&hp2[0].re, -sn * amp * w0);
-              atomicAdd(&hp2[0].im, cs * amp * w0);
-              atomicAdd(&hp2[1].re, -sn * amp * w1);
-              atomicAdd(&hp2[1].im, cs * amp * w1);
-            }
+            for (int l = 0; l < L; ++l)
+              for (int f = 0; f < F; ++f) {
+                double sn, cs;
+                sincos(-((l + 1.0) * theta[n] + f * ph), &sn, &cs);
+                const double a = amp * (l + 1.0);
+                cpx<double> *hp2 = mine + l * row + f * nr + idxR;
+                atomicAdd(&hp2[0].re, -sn * a * w0);
+                atomicAdd(&hp2[0].im, cs * a * w0);
+                atomicAdd(&hp2[1].re, -sn * a * w1);
+                atomicAdd(&hp2[1].im, cs * a * w1);
+              }
             continue;
           }
           if (idxR != key) {
@@ -813,6 +840,19 @@ __global__ void __launch_bounds__(1024)
           // i e^{-i theta} = (sin theta, cos theta) = (-em.im, em.re)
           sum[0] = sum[0] + mk(-(double)em.im, (double)em.re);
         } else if (idxR == 0) {
+          if (LM && L > 1) {
+            // the axis row of every mode, without 1/idxR (solveField.cpp:348-356)
+            FELIS_RADIAL_FLUSH();
+            key = -1;
+            for (int l = 0; l < L; ++l) {
+              double sn, cs;
+              sincos(-(l + 1.0) * theta[n], &sn, &cs);
+              const double a = d.coef_Ez * macro * (l + 1.0);
+              atomicAdd(&mine[l * row].re, -sn * a);
+              atomicAdd(&mine[l * row].im, cs * a);
+            }
+            continue;
+          }
           if (key != 0) {
             FELIS_RADIAL_FLUSH();
             key = 0;
@@ -826,11 +866,11 @@ __global__ void __launch_bounds__(1024)
     }
   }
   __syncthreads();
-  for (int q = tid; q < row; q += nt) {
+  for (int q = tid; q < rows; q += nt) {
     cpx<double> v = hist[q];
     if (priv)
-      for (int w = 1; w < nwv; ++w) v = v + hist[(long long)w * row + q];
-    ScR[(long long)slot * row + q] = v;
+      for (int w = 1; w < nwv; ++w) v = v + hist[(long long)w * rows + q];
+    ScR[(long long)slot * rows + q] = v;
   }
 }
 

@@ -135,16 +135,20 @@ int launch_particles(felis_ctx *ctx, const PStepArgs &a, double *tile_red, void 
   GridArgs g = ctx->grid();
   KTimer kt(ctx, FELIS_K_PARTICLE_STEP);
   const cpx<float> *Ux = (const cpx<float> *)Ufield, *Uy = (const cpx<float> *)((char *)Ufield + ctx->pol_bytes());
-  if (ctx->uy_live)
-    hipLaunchKernelGGL((k_pstep<NH, true>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, g, a, ctx->tiles(),
-                       ctx->theta, (float *)ctx->gamma, (float *)ctx->x, (float *)ctx->y, (float *)ctx->px,
-                       (float *)ctx->py, ctx->seg_w, Ux, Uy, (const cpx<double> *)ctx->Ez, ctx->bessel_f, ctx->wakeE,
-                       tile_red, ctx->errw);
-  else
-    hipLaunchKernelGGL((k_pstep<NH, false>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, g, a, ctx->tiles(),
-                       ctx->theta, (float *)ctx->gamma, (float *)ctx->x, (float *)ctx->y, (float *)ctx->px,
-                       (float *)ctx->py, ctx->seg_w, Ux, Uy, (const cpx<double> *)ctx->Ez, ctx->bessel_f, ctx->wakeE,
-                       tile_red, ctx->errw);
+#define PSTEP_LAUNCH(UY, LM)                                                                                    \
+  hipLaunchKernelGGL((k_pstep<NH, UY, LM>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, g, a, ctx->tiles(),     \
+                     ctx->theta, (float *)ctx->gamma, (float *)ctx->x, (float *)ctx->y, (float *)ctx->px,          \
+                     (float *)ctx->py, ctx->seg_w, Ux, Uy, (const cpx<double> *)ctx->Ez, ctx->bessel_f, ctx->wakeE, \
+                     tile_red, ctx->errw)
+  const bool lm = ctx->d.sc_l > 1;
+  if (ctx->uy_live) {
+    if (lm) PSTEP_LAUNCH(true, true);
+    else PSTEP_LAUNCH(true, false);
+  } else {
+    if (lm) PSTEP_LAUNCH(false, true);
+    else PSTEP_LAUNCH(false, false);
+  }
+#undef PSTEP_LAUNCH
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ctx->err = std::string("particle step launch: ") + hipGetErrorString(e);
@@ -159,16 +163,20 @@ int launch_deposit_slot(felis_ctx *ctx, const DepArgs &dep, int lds) {
   KTimer kt(ctx, FELIS_K_DEPOSIT);
   cpx<float> *Scx = (cpx<float> *)ctx->Sc, *Scy = (cpx<float> *)((char *)ctx->Sc + ctx->pol_bytes());
   const int nt = ctx->tune.deposit_threads ? ctx->tune.deposit_threads : 1024;
-  if (ctx->uy_live)
-    hipLaunchKernelGGL((k_deposit_slot<NH, 2>), dim3(ctx->d.slice_n), dim3(nt), lds, ctx->stream, g, dep,
-                       ctx->d.n_species, ctx->seg_off, ctx->seg_w, ctx->theta, (const float *)ctx->gamma,
-                       (const float *)ctx->x, (const float *)ctx->y, (const float *)ctx->px, (const float *)ctx->py,
-                       ctx->bessel, Scx, Scy, ctx->ScR, ctx->win, lds, ctx->errw);
-  else
-    hipLaunchKernelGGL((k_deposit_slot<NH, 1>), dim3(ctx->d.slice_n), dim3(nt), lds, ctx->stream, g, dep,
-                       ctx->d.n_species, ctx->seg_off, ctx->seg_w, ctx->theta, (const float *)ctx->gamma,
-                       (const float *)ctx->x, (const float *)ctx->y, (const float *)ctx->px, (const float *)ctx->py,
-                       ctx->bessel, Scx, Scy, ctx->ScR, ctx->win, lds, ctx->errw);
+#define DEPOSIT_LAUNCH(NPOL, LM)                                                                             \
+  hipLaunchKernelGGL((k_deposit_slot<NH, NPOL, LM>), dim3(ctx->d.slice_n), dim3(nt), lds, ctx->stream, g, dep,   \
+                     ctx->d.n_species, ctx->seg_off, ctx->seg_w, ctx->theta, (const float *)ctx->gamma,          \
+                     (const float *)ctx->x, (const float *)ctx->y, (const float *)ctx->px, (const float *)ctx->py, \
+                     ctx->bessel, Scx, Scy, ctx->ScR, ctx->win, lds, ctx->errw)
+  const bool lm = ctx->d.sc_l > 1;
+  if (ctx->uy_live) {
+    if (lm) DEPOSIT_LAUNCH(2, true);
+    else DEPOSIT_LAUNCH(2, false);
+  } else {
+    if (lm) DEPOSIT_LAUNCH(1, true);
+    else DEPOSIT_LAUNCH(1, false);
+  }
+#undef DEPOSIT_LAUNCH
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     ctx->err = std::string("deposit launch: ") + hipGetErrorString(e);
@@ -217,10 +225,13 @@ void reduce_diag(felis_ctx *ctx, int skip, const double *tile_red, const int *sp
 
 template <int NH>
 hipError_t set_deposit_lds(int lds) {
-  hipError_t e = hipFuncSetAttribute((const void *)k_deposit_slot<NH, 1>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     lds);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute((const void *)k_deposit_slot<NH, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  const void *fn[4] = {(const void *)k_deposit_slot<NH, 1, false>, (const void *)k_deposit_slot<NH, 2, false>,
+                       (const void *)k_deposit_slot<NH, 1, true>, (const void *)k_deposit_slot<NH, 2, true>};
+  for (const void *f : fn) {
+    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 hipError_t deposit_lds_attr(int nh, int lds) {
   switch (nh) {
@@ -413,8 +424,8 @@ struct RunScratch {
 int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps, int wake_step,
               const double *wakeF, double *dev_out, size_t per, JumpCtl *jc, const RunScratch *ext = nullptr) {
   const felis_desc &d = ctx->d;
-  if (d.sc_l != 1) {
-    ctx->err = "the batched path models one longitudinal space-charge mode (number_longitudinal_mode = 1)";
+  if (d.sc_l > kMaxModes) {
+    ctx->err = "the batched path models up to 4 longitudinal space-charge modes (number_longitudinal_mode)";
     return FELIS_ERR_UNSUPPORTED;
   }
   // LDS-resident scan ADI when the system fits, else the two-pass ADI with a
@@ -428,8 +439,9 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
   // strips of node rows, and then builds the radial source there; the tuning
   // may give it less LDS than the device allows. Its budget never goes below
   // one node row of the widest window (nx cells of both polarisations) and
-  // one radial row (sc_f nr + 1 complex doubles), so both always fit.
-  const int dep_floor = std::max(d.nx * 2 * (int)sizeof(float) * 2, (d.sc_f * d.nr + 1) * 2 * (int)sizeof(double));
+  // one radial row per mode (sc_l (sc_f nr + 1) complex doubles), so both always fit.
+  const int dep_floor =
+      std::max(d.nx * 2 * (int)sizeof(float) * 2, d.sc_l * (d.sc_f * d.nr + 1) * 2 * (int)sizeof(double));
   if (dep_floor > ctx->lds_max) {
     ctx->err = "the radial source row (number_azimuthal_mode x radial_grids) exceeds the LDS of one workgroup";
     return FELIS_ERR_UNSUPPORTED;

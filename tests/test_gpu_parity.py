@@ -155,15 +155,18 @@ def test_update_wake():
     assert fc.rel_err(wE, o.wakeE) < 1e-14
 
 
-@pytest.mark.parametrize("harm,first_cf", [((1,), 0), ((1, 3, 5), 0), ((1, 2), 0), ((1, 3), 1)])
-def test_run_mixed_matches_oracle(harm, first_cf):
-    """Batched FP32/FP64 loop (fused kernels) against the FP64 oracle loop.
+@pytest.mark.parametrize("harm,first_cf,sc_l,sc_f", [((1,), 0, 1, 1), ((1, 3, 5), 0, 1, 1), ((1, 2), 0, 1, 1),
+                                                     ((1, 3), 1, 1, 1), ((1, 3), 1, 3, 1), ((1,), 1, 2, 2)])
+def test_run_mixed_matches_oracle(harm, first_cf, sc_l, sc_f):
+    """Batched FP32/FP64 loop (fused kernels) against the FP64 oracle loop,
+    with one, two and three longitudinal space-charge modes and two azimuthal
+    ones (solveField.cpp:315-398, particlePush.cpp:287-334).
 
     Tolerance: FP32 coupling, FP64 phase; after 12 periods the total energy
     agrees to 1e-4 relative and the bunching factors to 1e-3 of the beam's
     total weight (they start near the quiet-start noise floor)."""
     case = fc.make_case(slice_n=6, harmonics=harm, ppslice=256, nx=33, ny=31, ue=0.3 if 2 in harm else 0.0,
-                        und_type=2 if 2 in harm else 1)
+                        und_type=2 if 2 in harm else 1, sc_l=sc_l, sc_f=sc_f)
     o = case.oracle_state()
     ctx = fc.gpu_context(case, precision=fl.MIXED_FP32)
     nsteps = 12
