@@ -527,6 +527,19 @@ static int download_coord(felis_ctx *ctx, const void *src, double *dst, double o
   return FELIS_OK;
 }
 
+// Whether an in-grid particle (idxI in [0, nx-2], idxJ in [0, ny-2] after
+// truncation toward zero, so x from minX - dx on) can have idxR >= nr, where
+// the reference carries Em (particlePush.cpp:283-304). With the default
+// radial grid (parameterSetting.cpp:193-195) it cannot; a small
+// radial_grids can. Decided with a margin, so that no such particle goes
+// unprepared.
+static bool em_carry_possible(const felis_desc &d) {
+  if (d.dr <= 0) return false;
+  const double xm = std::max(std::fabs(d.min_x - d.dx), std::fabs(d.min_x + (d.nx - 1) * d.dx));
+  const double ym = std::max(std::fabs(d.min_y - d.dy), std::fabs(d.min_y + (d.ny - 1) * d.dy));
+  return std::sqrt(xm * xm + ym * ym) / d.dr + 0.5 >= d.nr * (1.0 - 1e-9);
+}
+
 // Allocates the particle arrays and the tile table for a segment layout.
 int felis_ctx::alloc_particles(const int64_t *seg_off, const double *seg_weight) {
   felis_ctx *ctx = this;
@@ -587,6 +600,15 @@ int felis_ctx::alloc_particles(const int64_t *seg_off, const double *seg_weight)
       (e = pal((void **)&ctx->chunk_start, cstart.size() * sizeof(long long))) ||
       (e = pal((void **)&ctx->chunk_cnt, ccnt.size() * sizeof(int))))
     return e;
+  ctx->em_carry = d.dimension == 3 && em_carry_possible(d);
+  ctx->emc = nullptr;
+  ctx->em_last = ctx->em_src = nullptr;
+  if (ctx->em_carry &&
+      ((e = pal(&ctx->emc, np * d.sc_l * 2 * sizeof(double))) ||
+       (e = pal((void **)&ctx->em_last, tseg.size() * sizeof(int))) ||
+       (e = pal((void **)&ctx->em_src, tseg.size() * sizeof(int)))))
+    return e;
+  if (ctx->em_carry) HIPCHK(hipMemset(ctx->emc, 0, np * d.sc_l * 2 * sizeof(double)));
   HIPCHK(hipMemcpy(ctx->seg_off, seg_off, (nseg + 1) * sizeof(long long), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(ctx->seg_w, seg_weight, nseg * sizeof(double), hipMemcpyHostToDevice));
   if (!tseg.empty()) {
@@ -777,11 +799,19 @@ static int launch_push(felis_ctx *ctx, const LatArgs &l) {
   if (!ctx->ntiles) return FELIS_OK;
   GridArgs g = ctx->grid();
   KTimer kt(ctx, FELIS_K_PUSH);
+  if (ctx->em_carry) {
+    const EmCarryPlain<R> pol{g, (const R *)ctx->x, (const R *)ctx->y, (const cpx<double> *)ctx->Ez};
+    hipLaunchKernelGGL((k_em_last<EmCarryPlain<R>>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, pol,
+                       ctx->tiles(), ctx->em_last);
+    hipLaunchKernelGGL(k_em_scan, dim3(1), dim3(1024), 0, ctx->stream, ctx->ntiles, ctx->em_last, ctx->em_src);
+    hipLaunchKernelGGL((k_em_fill<EmCarryPlain<R>>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, pol,
+                       ctx->tiles(), ctx->em_last, ctx->em_src, (cpx<double> *)ctx->emc);
+  }
   hipLaunchKernelGGL((k_push_tg<R, T>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, g, l, ctx->tiles(),
                      ctx->theta, (R *)ctx->gamma, (const R *)ctx->x, (const R *)ctx->y, (const R *)ctx->px,
                      (const R *)ctx->py, (const cpx<T> *)ctx->U,
                      (const cpx<T> *)((char *)ctx->U + ctx->pol_bytes()), (const cpx<double> *)ctx->Ez,
-                     ctx->bessel, ctx->wakeE, ctx->errw);
+                     ctx->bessel, ctx->wakeE, ctx->em_carry ? (const cpx<double> *)ctx->emc : nullptr, ctx->errw);
   LAUNCHCHK();
   return FELIS_OK;
 }

@@ -63,6 +63,12 @@ struct felis_ctx {
   int nchunks = 0;
   int *chunk_seg = nullptr, *chunk_cnt = nullptr;
   long long *chunk_start = nullptr;
+  // Em carry (felis_emcarry.h): set when an in-grid particle can lie at
+  // idxR >= nr; then emc holds np * sc_l complex doubles and em_last /
+  // em_src one int per tile
+  bool em_carry = false;
+  void *emc = nullptr;
+  int *em_last = nullptr, *em_src = nullptr;
   int lds_max = 65536;  // dynamic LDS bytes a workgroup may use
   int n_cu = 256;       // compute units of the device
   int adi_cols = 0;     // interior columns the batched ADI keeps in LDS (adi_scan_fits)

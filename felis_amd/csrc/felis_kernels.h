@@ -7,6 +7,7 @@
 #pragma once
 
 #include "felis_common.h"
+#include "felis_emcarry.h"
 
 namespace felis {
 
@@ -68,6 +69,31 @@ __device__ inline double bessel_interp(const double *__restrict__ tab, int bo, i
   return tab[idx * bo + order] * w0 + tab[(idx + 1) * bo + order] * w1;
 }
 
+// Em of the longitudinal space-charge modes for an in-grid particle at
+// radius^2 r2 in slot (particlePush.cpp:283-304): the two radial points
+// weighted by wr, summed unweighted on the axis. False when idxR >= nr,
+// where the reference keeps the Em of the particle it pushed before.
+__device__ inline bool em_gather_f64(const GridArgs &g, const cpx<double> *__restrict__ Ez, int slot, double r2,
+                                     cpx<double> (&Em)[4]) {
+  const int L = g.L, F = g.F, nr = g.nr;
+  const double r = sqrt(r2);
+  const int idxR = (int)(r / g.dr + 0.5);
+  const long long es = (long long)g.S * F * nr;
+  if (idxR >= nr) return false;
+  const double wr1 = idxR > 0 ? (r * r / (g.dr * g.dr) - idxR * idxR) / (2.0 * idxR) : 1.0;
+  const double wr0 = idxR > 0 ? 1.0 - wr1 : 1.0;
+  for (int ll = 0; ll < L; ++ll) {
+    cpx<double> e = mk(0.0, 0.0);
+    for (int f = 0; f < F; ++f) {
+      const cpx<double> *p = Ez + ll * es + (long long)slot * nr * F + nr * f + idxR;
+      e = e + p[0] * wr0;
+      e = e + p[1] * wr1;
+    }
+    Em[ll] = e;
+  }
+  return true;
+}
+
 // particlePush.cpp:198-386 for one particle (FP64). Returns false when the
 // particle is outside the push grid (it is then left untouched).
 template <typename T>
@@ -76,7 +102,7 @@ __device__ inline bool push_tg_particle(const GridArgs &g, const LatArgs &l, int
                                         const cpx<double> *__restrict__ Ez,
                                         const double *__restrict__ bes, double x0, double y0,
                                         double px, double py, double &th, double &gam,
-                                        unsigned *err) {
+                                        const cpx<double> *__restrict__ emc, long long n, unsigned *err) {
   const double ku = l.ku, ks = g.ks, K0 = l.K0, ue = l.ue, ka = l.K0_alpha;
   const double dz = g.dz;
   double r2 = x0 * x0 + y0 * y0;
@@ -90,36 +116,18 @@ __device__ inline bool push_tg_particle(const GridArgs &g, const LatArgs &l, int
   double wy1 = (y0 - g.minY) / g.dy - idxJ, wy0 = 1.0 - wy1;
   if (!(idxI >= 0 && idxI < g.nx - 1 && idxJ >= 0 && idxJ < g.ny - 1)) return false;
 
-  // longitudinal space-charge field (one mode l for the usual L=F=1)
+  // longitudinal space-charge field (one mode l for the usual L=F=1); an
+  // in-grid particle beyond the radial grid takes the Em the reference
+  // carries over from the particle it pushed before (emc, filled by
+  // k_em_fill; null: refused)
   cpx<double> Em[4];
-  const int L = g.L, F = g.F, nr = g.nr;
-  double r = sqrt(r2);
-  int idxR = (int)(r / g.dr + 0.5);
-  const long long es = (long long)g.S * F * nr;
-  if (idxR > 0 && idxR < nr) {
-    double wr1 = (r * r / (g.dr * g.dr) - idxR * idxR) / (2.0 * idxR), wr0 = 1.0 - wr1;
-    for (int ll = 0; ll < L; ++ll) {
-      cpx<double> e = mk(0.0, 0.0);
-      for (int f = 0; f < F; ++f) {
-        const cpx<double> *p = Ez + ll * es + (long long)slot * nr * F + nr * f + idxR;
-        e = e + p[0] * wr0;
-        e = e + p[1] * wr1;
-      }
-      Em[ll] = e;
+  if (!em_gather_f64(g, Ez, slot, r2, Em)) {
+    if (emc) {
+      for (int ll = 0; ll < g.L; ++ll) Em[ll] = emc[n * g.L + ll];
+    } else {
+      atomicOr(err, EB_EMCARRY);
+      for (int ll = 0; ll < g.L; ++ll) Em[ll] = mk(0.0, 0.0);
     }
-  } else if (idxR == 0) {
-    for (int ll = 0; ll < L; ++ll) {
-      cpx<double> e = mk(0.0, 0.0);
-      for (int f = 0; f < F; ++f) {
-        const cpx<double> *p = Ez + ll * es + (long long)slot * nr * F + nr * f + idxR;
-        e = e + p[0];
-        e = e + p[1];
-      }
-      Em[ll] = e;
-    }
-  } else {
-    atomicOr(err, EB_EMCARRY);
-    for (int ll = 0; ll < L; ++ll) Em[ll] = mk(0.0, 0.0);
   }
 
   // bilinear field gather per harmonic
@@ -193,7 +201,7 @@ __device__ inline bool push_tg_particle(const GridArgs &g, const LatArgs &l, int
     double thm = th0 + dz * kth * cm[m];
     double gm = gam0 + dz * kg * cm[m];
     double sumTh = 0.0, sumG = 0.0, sumEz = 0.0;
-    for (int ll = 0; ll < L; ++ll) {
+    for (int ll = 0; ll < g.L; ++ll) {
       double s, c;
       sincos((ll + 1.0) * thm, &s, &c);
       sumEz += 2.0 * (Em[ll].re * c - Em[ll].im * s);
@@ -227,7 +235,8 @@ __global__ void __launch_bounds__(kTile)
               const R *__restrict__ x, const R *__restrict__ y, const R *__restrict__ px,
               const R *__restrict__ py, const cpx<T> *__restrict__ Ux,
               const cpx<T> *__restrict__ Uy, const cpx<double> *__restrict__ Ez,
-              const double *__restrict__ bes, const double *__restrict__ wakeE, unsigned *err) {
+              const double *__restrict__ bes, const double *__restrict__ wakeE,
+              const cpx<double> *__restrict__ emc, unsigned *err) {
   int t = blockIdx.x;
   int k = threadIdx.x;
   if (k >= tl.cnt[t]) return;
@@ -237,11 +246,35 @@ __global__ void __launch_bounds__(kTile)
   double th = theta[n];
   double gm = (double)gamma[n] + g.gref;
   if (push_tg_particle<T>(g, l, sl + 1, wk, Ux, Uy, Ez, bes, (double)x[n], (double)y[n],
-                          (double)px[n], (double)py[n], th, gm, err)) {
+                          (double)px[n], (double)py[n], th, gm, emc, n, err)) {
     theta[n] = th;
     gamma[n] = (R)(gm - g.gref);
   }
 }
+
+// Em carry of the per-phase push: the positions are those the transverse
+// kernels left, Em is formed in FP64 (see felis_emcarry.h)
+template <typename R>
+struct EmCarryPlain {
+  using Pos = double2;
+  using E = cpx<double>;
+  GridArgs g;
+  const R *x, *y;
+  const cpx<double> *Ez;
+  __device__ int where(long long n, Pos &q) const {
+    const double x0 = (double)x[n], y0 = (double)y[n];
+    const int idxI = (int)((x0 - g.minX) / g.dx), idxJ = (int)((y0 - g.minY) / g.dy);
+    q = make_double2(x0, y0);
+    if (!(idxI >= 0 && idxI < g.nx - 1 && idxJ >= 0 && idxJ < g.ny - 1)) return 0;
+    return (int)(sqrt(x0 * x0 + y0 * y0) / g.dr + 0.5) < g.nr ? 1 : 2;
+  }
+  __device__ int modes() const { return g.L; }
+  __device__ void em(int seg, const Pos &q, E *out) const {
+    cpx<double> Em[4];
+    em_gather_f64(g, Ez, seg % g.slice_n + 1, q.x * q.x + q.y * q.y, Em);
+    for (int l = 0; l < g.L; ++l) out[l] = Em[l];
+  }
+};
 
 // particlePush.cpp:119-195: gaps; wake loss is applied to theta.
 template <typename R, typename T>

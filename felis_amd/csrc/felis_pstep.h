@@ -193,6 +193,35 @@ __device__ inline void transverse_f32(const TransArgs &t, float dz, float gam, f
   py += dz6 * spy;
 }
 
+// Em of the longitudinal space-charge modes (the first L <= ML) for an
+// in-grid particle at radius^2 r2 in slot (particlePush.cpp:283-304): the two
+// radial points weighted by wr, summed unweighted on the axis. Zero and false
+// when idxR >= nr, where the reference keeps the Em of the particle it pushed
+// before.
+template <int ML>
+__device__ inline bool em_gather_f32(const GridArgs &g, double inv_dr, int L, const cpx<double> *__restrict__ Ez,
+                                     int slot, float r2, cpx<float> (&Em)[ML]) {
+  const int F = g.F, nr = g.nr;
+  const long long es = (long long)g.S * F * nr;
+  const double r = sqrt((double)r2);
+  const double rr = r * inv_dr;
+  const int idxR = (int)(rr + 0.5);
+#pragma unroll
+  for (int l = 0; l < ML; ++l) Em[l] = mk(0.f, 0.f);
+  if (!(idxR >= 0 && idxR < nr)) return false;
+  const float wr1 = idxR > 0 ? __fdividef((float)(rr * rr - (double)idxR * idxR), 2.f * idxR) : 1.f;
+  const float wr0 = idxR > 0 ? 1.f - wr1 : 1.f;
+#pragma unroll
+  for (int l = 0; l < ML; ++l) {
+    if (l >= L) break;
+    for (int f = 0; f < F; ++f) {
+      const cpx<double> *p = Ez + l * es + (long long)slot * nr * F + nr * f + idxR;
+      Em[l] = Em[l] + cvt<float>(p[0]) * wr0 + cvt<float>(p[1]) * wr1;
+    }
+  }
+  return true;
+}
+
 // push_theta_gamma_3D of one particle (particlePush.cpp:260-379): the field
 // coupling in FP32, the slippage rate ku - ks/(2 gamma0^2) K2 in FP64 with the
 // RK stages' changes of it in FP32, gamma and theta in FP64. One sincos per particle: the RK stages rotate it by their
@@ -203,7 +232,7 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
                                    const cpx<float> *__restrict__ Ux, const cpx<float> *__restrict__ Uy,
                                    const cpx<double> *__restrict__ Ez, const float *__restrict__ bes, float x0,
                                    float y0, float px, float py, double &th, double &gam, cpx<float> &eth_out,
-                                   unsigned *err) {
+                                   const cpx<float> *__restrict__ emc, long long n, unsigned *err) {
   const double ku = l.ku, ks = g.ks;
   const float ka = a.kaf;
   const double dz = g.dz;
@@ -221,28 +250,17 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
   // longitudinal space charge, modes l = 0 .. L-1 (particlePush.cpp:287-304);
   // on the axis the two radial points are summed unweighted. LM: more than
   // one mode (otherwise the first only, in fewer registers)
+  // an in-grid particle beyond the radial grid takes the Em the reference
+  // carries over from the particle it pushed before (emc, filled by
+  // k_em_fill; null: refused)
   constexpr int ML = LM ? kMaxModes : 1;
   cpx<float> Em[ML];
   const int L = LM ? g.L : 1;
-  {
-    const int F = g.F, nr = g.nr;
-    const long long es = (long long)g.S * F * nr;
-    const double r = sqrt((double)r2);
-    const double rr = r * a.inv_dr;
-    const int idxR = (int)(rr + 0.5);
+  if (!em_gather_f32<ML>(g, a.inv_dr, L, Ez, slot, r2, Em)) {
+    if (emc) {
 #pragma unroll
-    for (int l = 0; l < ML; ++l) Em[l] = mk(0.f, 0.f);
-    if (idxR >= 0 && idxR < nr) {
-      const float wr1 = idxR > 0 ? __fdividef((float)(rr * rr - (double)idxR * idxR), 2.f * idxR) : 1.f;
-      const float wr0 = idxR > 0 ? 1.f - wr1 : 1.f;
-#pragma unroll
-      for (int l = 0; l < ML; ++l) {
-        if (l >= L) break;
-        for (int f = 0; f < F; ++f) {
-          const cpx<double> *p = Ez + l * es + (long long)slot * nr * F + nr * f + idxR;
-          Em[l] = Em[l] + cvt<float>(p[0]) * wr0 + cvt<float>(p[1]) * wr1;
-        }
-      }
+      for (int l = 0; l < ML; ++l)
+        if (l < L) Em[l] = emc[n * L + l];
     } else {
       atomicOr(err, EB_EMCARRY);
     }
@@ -370,6 +388,46 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
   return true;
 }
 
+// the step's two transversePush half steps at the particle's gamma
+__device__ inline void transverse_step(const GridArgs &g, const PStepArgs &a, double gm, float &xx, float &yy,
+                                       float &ppx, float &ppy) {
+  const float dzh = (float)(g.dz * 0.5);
+  const float gf = (float)gm;
+  transverse_f32(a.t0, dzh, gf, xx, yy, ppx, ppy);
+  transverse_f32(a.t1, dzh, gf, xx, yy, ppx, ppy);
+}
+
+// Em carry of the batched step (felis_emcarry.h): the positions are those
+// k_pstep reaches after its transverse half steps, Em is formed as k_pstep
+// forms it
+template <bool LM>
+struct EmCarryMixed {
+  using Pos = float2;
+  using E = cpx<float>;
+  GridArgs g;
+  PStepArgs a;
+  const float *x, *y, *px, *py, *gamma;
+  const cpx<double> *Ez;
+  __device__ int where(long long n, Pos &q) const {
+    float xx = x[n], yy = y[n], ppx = px[n], ppy = py[n];
+    transverse_step(g, a, g.gref + (double)gamma[n], xx, yy, ppx, ppy);
+    q = make_float2(xx, yy);
+    const double fx_ = ((double)xx - g.minX) * a.inv_dx, fy_ = ((double)yy - g.minY) * a.inv_dy;
+    const int idxI = (int)fx_, idxJ = (int)fy_;
+    if (!(idxI >= 0 && idxI < g.nx - 1 && idxJ >= 0 && idxJ < g.ny - 1)) return 0;
+    const float r2 = xx * xx + yy * yy;
+    const int idxR = (int)(sqrt((double)r2) * a.inv_dr + 0.5);
+    return idxR >= 0 && idxR < g.nr ? 1 : 2;
+  }
+  __device__ int modes() const { return LM ? g.L : 1; }
+  __device__ void em(int seg, const Pos &q, E *out) const {
+    constexpr int ML = LM ? kMaxModes : 1;
+    cpx<float> Em[ML];
+    em_gather_f32<ML>(g, a.inv_dr, modes(), Ez, seg % g.slice_n + 1, q.x * q.x + q.y * q.y, Em);
+    for (int l = 0; l < modes(); ++l) out[l] = Em[l];
+  }
+};
+
 // One workgroup of kTile threads per tile, one particle per thread. UY =
 // false: Uy is exactly zero (see k_deposit_slot) and is not read. LM: more
 // than one longitudinal space-charge mode.
@@ -379,7 +437,7 @@ __global__ void __launch_bounds__(kTile)
             float *__restrict__ x, float *__restrict__ y, float *__restrict__ px, float *__restrict__ py,
             const double *__restrict__ seg_w, const cpx<float> *__restrict__ Ux, const cpx<float> *__restrict__ Uy,
             const cpx<double> *__restrict__ Ez, const float *__restrict__ bes, const double *__restrict__ wakeE,
-            double *__restrict__ tile_red, unsigned *err) {
+            const cpx<float> *__restrict__ emc, double *__restrict__ tile_red, unsigned *err) {
   __shared__ double red[(2 * NH + 1 + 7) * kTile];
   GridArgs g = g0;
   g.ring_base = ring_of(g0);
@@ -398,14 +456,12 @@ __global__ void __launch_bounds__(kTile)
     double th = theta[n];
     double gm = g.gref + (double)gamma[n];
     float xx = x[n], yy = y[n], ppx = px[n], ppy = py[n];
-    const float dzh = (float)(g.dz * 0.5);
-    const float gf = (float)gm;
-    transverse_f32(a.t0, dzh, gf, xx, yy, ppx, ppy);
-    transverse_f32(a.t1, dzh, gf, xx, yy, ppx, ppy);
+    transverse_step(g, a, gm, xx, yy, ppx, ppy);
     cpx<float> eth = mk(1.f, 0.f);
     bool have_eth = false;
     if (!a.drift) {
-      have_eth = push_tg_f32<NH, UY, LM>(g, a, a.l1, slot, wk, Ux, Uy, Ez, bes, xx, yy, ppx, ppy, th, gm, eth, err);
+      have_eth =
+          push_tg_f32<NH, UY, LM>(g, a, a.l1, slot, wk, Ux, Uy, Ez, bes, xx, yy, ppx, ppy, th, gm, eth, emc, n, err);
     } else {
       const double fx_ = ((double)xx - g.minX) * a.inv_dx, fy_ = ((double)yy - g.minY) * a.inv_dy;
       const int idxI = (int)fx_, idxJ = (int)fy_;

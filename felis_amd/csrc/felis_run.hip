@@ -130,17 +130,41 @@ bool adi_scan_fits(felis_ctx *ctx) {
          (!fn[1] || hipFuncSetAttribute(fn[1], hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess);
 }
 
+// the Em carry's three launches ahead of a push (felis_emcarry.h)
+template <bool LM>
+void em_carry_mixed(felis_ctx *ctx, const GridArgs &g, const PStepArgs &a) {
+  const EmCarryMixed<LM> pol{g,
+                             a,
+                             (const float *)ctx->x,
+                             (const float *)ctx->y,
+                             (const float *)ctx->px,
+                             (const float *)ctx->py,
+                             (const float *)ctx->gamma,
+                             (const cpx<double> *)ctx->Ez};
+  hipLaunchKernelGGL((k_em_last<EmCarryMixed<LM>>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, pol,
+                     ctx->tiles(), ctx->em_last);
+  hipLaunchKernelGGL(k_em_scan, dim3(1), dim3(1024), 0, ctx->stream, ctx->ntiles, ctx->em_last, ctx->em_src);
+  hipLaunchKernelGGL((k_em_fill<EmCarryMixed<LM>>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, pol,
+                     ctx->tiles(), ctx->em_last, ctx->em_src, (cpx<float> *)ctx->emc);
+}
+
 template <int NH>
 int launch_particles(felis_ctx *ctx, const PStepArgs &a, double *tile_red, void *Ufield) {
   GridArgs g = ctx->grid();
   KTimer kt(ctx, FELIS_K_PARTICLE_STEP);
   const cpx<float> *Ux = (const cpx<float> *)Ufield, *Uy = (const cpx<float> *)((char *)Ufield + ctx->pol_bytes());
+  const bool lm = ctx->d.sc_l > 1;
+  const bool carry = ctx->em_carry && !a.drift;
+  if (carry) {
+    if (lm) em_carry_mixed<true>(ctx, g, a);
+    else em_carry_mixed<false>(ctx, g, a);
+  }
+  const cpx<float> *emc = carry ? (const cpx<float> *)ctx->emc : nullptr;
 #define PSTEP_LAUNCH(UY, LM)                                                                                    \
   hipLaunchKernelGGL((k_pstep<NH, UY, LM>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, g, a, ctx->tiles(),     \
                      ctx->theta, (float *)ctx->gamma, (float *)ctx->x, (float *)ctx->y, (float *)ctx->px,          \
                      (float *)ctx->py, ctx->seg_w, Ux, Uy, (const cpx<double> *)ctx->Ez, ctx->bessel_f, ctx->wakeE, \
-                     tile_red, ctx->errw)
-  const bool lm = ctx->d.sc_l > 1;
+                     emc, tile_red, ctx->errw)
   if (ctx->uy_live) {
     if (lm) PSTEP_LAUNCH(true, true);
     else PSTEP_LAUNCH(true, false);

@@ -28,6 +28,9 @@ CASES = {
     "helical_h12": dict(harmonics=(1, 2), ue=0.6, und_type=2),
     "ka_minus1_h123": dict(harmonics=(1, 2, 3), ue=0.3, K0_alpha=-1),
     "two_species_uneven": dict(n_species=2, uneven=True, ppslice=96),
+    # radial_grids 3 and a wide beam: in-grid particles beyond the radial grid carry Em
+    # (particlePush.cpp:248, 283-304), the radial source spills (A.5)
+    "em_carry_nr3": dict(nr=3, sig=60e-6, harmonics=(1, 3), sc_l=2),
 }
 
 
@@ -183,6 +186,53 @@ def test_run_mixed_matches_oracle(harm, first_cf, sc_l, sc_f):
     assert fc.rel_err(t_g, t_o) < 1e-4
     p = ctx.download_particles()
     assert fc.rel_err(p["gamma"] - case.state["gamma"], o.gamma - case.state["gamma"]) < 1e-3
+
+
+def _carried(o):
+    """In-grid particles at idxR >= nr, from the oracle state's positions."""
+    p = o.params
+    ix = np.trunc((o.x - p.min_x) / p.dx)
+    iy = np.trunc((o.y - p.min_y) / p.dy)
+    inside = (ix >= 0) & (ix < p.nx - 1) & (iy >= 0) & (iy < p.ny - 1)
+    return inside & (np.trunc(np.sqrt(o.x ** 2 + o.y ** 2) / p.dr + 0.5) >= p.nr)
+
+
+def test_run_mixed_carries_em():
+    """particlePush.cpp:248, 283-304 in the batched step: with radial_grids 2
+    about a tenth of the in-grid particles lie at idxR >= nr and take the Em of
+    the particle pushed before them (flat order: slice, then particle). One
+    period from a prebunched beam, so that Ez is large: every particle's gamma
+    change agrees with the oracle's to 2e-4 of the largest, and the carried
+    particles' Em terms (the oracle's push with and without Ez) are far larger
+    than the differences, so a carried particle with a wrong or zero Em fails."""
+    case = fc.make_case(slice_n=6, harmonics=(1,), ppslice=256, nx=33, ny=31, nr=3, sc_l=2)
+    th = case.state["theta"]
+    th[:] = th + 0.8 * np.sin(th)
+    tab = [case.lattice(current_flag=1)] + [case.lattice() for _ in range(2)]
+    g0 = case.state["gamma"].copy()
+    o = case.oracle_state()
+    rc, _, _, _ = o.run(tab, 0, 1, True)
+    assert rc == 0
+    ctx = fc.gpu_context(case, precision=fl.MIXED_FP32)
+    ctx.run([fc.to_lib_lattice(l) for l in tab], 0, 1)
+    dg_g = ctx.download_particles()["gamma"] - g0
+    dg_o = o.gamma - g0
+    diff = np.max(np.abs(dg_g - dg_o))
+    assert diff < 2e-4 * np.max(np.abs(dg_o))
+    # the Em terms: the same period up to the push, with and without Ez
+    pushed = []
+    for zero_ez in (False, True):
+        q = case.oracle_state()
+        assert q.solve_field(tab[0]) == 0
+        assert q.transverse_push(tab[1]) == 0 and q.transverse_push(tab[2]) == 0
+        carried = _carried(q)
+        if zero_ez:
+            q.Ez[:] = 0
+        assert q.push_theta_gamma(tab[2]) == 0
+        pushed.append(q.gamma.copy())
+    em_term = np.abs(pushed[0] - pushed[1])
+    assert carried.mean() > 0.05
+    assert np.min(np.sort(em_term[carried])[-20:]) > 30 * diff
 
 
 @pytest.mark.parametrize("n", [33, 151, 171], ids=["fused_adi", "split_lds_adi", "two_pass_adi"])

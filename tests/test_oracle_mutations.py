@@ -77,6 +77,10 @@ MUTATIONS = {
     "abc_profile_shift": ("sx[i] = abcSig * (i - nx + L) * (i - nx + L) / (1.0 * L * L);",
                           "sx[i] = abcSig * (i - nx + L + 1) * (i - nx + L + 1) / (1.0 * L * L);",
                           "test_absorbing_layer_profile_closed_form"),
+    "em_reset_per_slice": ("if (seg == 0 || nt > 1)", "if (1)", "test_push_carries_em_past_the_radial_grid"),
+    "em_not_carried": ("ny - 1)) continue;\n        double r = sqrt(r2);",
+                       "ny - 1)) continue;\n        for (int l = 0; l < L; ++l) Em[l] = 0;\n        double r = sqrt(r2);",
+                       "test_push_carries_em_past_the_radial_grid"),
     "abc_amplifying": ("cx[i] = 1.0 + I * sx[i];", "cx[i] = 1.0 - I * sx[i];",
                        "test_absorbing_layer_absorbs_a_beam_and_never_amplifies"),
 }

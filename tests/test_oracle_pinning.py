@@ -764,3 +764,52 @@ def test_guard_and_nan_in_the_oracle():
     o = case.oracle_state()
     assert o.push_theta_gamma(case.lattice()) == 0
     assert o.periodic() == 0 and np.isnan(o.theta[17])
+
+
+def test_push_carries_em_past_the_radial_grid():
+    """particlePush.cpp:248, 283-304: Em is one vector per push call. An
+    in-grid particle with idxR < nr sets it from Ez; one with idxR >= nr uses
+    what it holds, i.e. the Em of the last particle before it (slices in
+    order) that set it, zero when none did; a particle outside the grid leaves
+    it alone. With K0 = 0 and U = 0 a particle's push depends on its own
+    (theta, gamma, p) and on Em only, so a carried particle must end exactly
+    where a particle with its (theta, gamma, p) placed at its Em source does.
+    Two modes, two slices with different Ez; dr = dx, nr = 5: radii from
+    4.5 dr on are carried."""
+    rng = np.random.default_rng(17)
+    dx = 10e-6
+    mom = dict(theta=[0.3, 1.1, 2.9, 4.0, 5.2, 0.7], gamma=[8001.0, 7999.0, 8003.0, 7998.5, 8002.0, 8000.5],
+               px=[0.01, -0.02, 0.03, 0.0, -0.01, 0.02], py=[0.0, 0.01, -0.02, 0.02, 0.01, -0.01])
+    pos = dict(x=[60e-6, 15e-6, 50e-6, 200e-6, 0.0, 20e-6], y=[0.0, 0.0, 50e-6, 0.0, 60e-6, 15e-6])
+    # 0 carried, no source yet | 1 sets Em (slot 1) | 2 carried: source 1
+    # 3 outside the grid (slot 2) | 4 carried: source 1 in the previous slice | 5 sets Em
+    kw = dict(nx=21, ny=21, dx=dx, K0=0.0, nr=5, dr=dx, sc_l=2, slice_n=2)
+
+    def state(parts, seg):
+        st, lat = make_state(parts, **kw)
+        st.seg_off[:] = seg
+        st.Ez[:] = ez
+        return st, lat
+
+    S = 4
+    ez = (rng.normal(size=(2, S * 5)) + 1j * rng.normal(size=(2, S * 5))) * 1e4
+    st, lat = state(particles(6, **pos, **mom), [0, 3, 6])
+    th0, g0 = st.theta.copy(), st.gamma.copy()
+    assert st.push_theta_gamma(lat) == 0
+    assert st.theta[3] == th0[3] and st.gamma[3] == g0[3]
+    # particle 0: no Em yet -> the zero-field closed form, gamma unchanged
+    dth0 = st.params.dz * (lat.ku - st.params.ks * (1 + 0.01 ** 2) / (2 * 8001.0 ** 2))
+    assert st.gamma[0] == g0[0]
+    np.testing.assert_allclose(st.theta[0] - th0[0], dth0, rtol=0, atol=4 * np.spacing(th0[0]))
+    for n in (2, 4):
+        one = {k: [mom[k][n]] for k in mom}
+        ref, _ = state(particles(1, x=pos["x"][1], y=pos["y"][1], **one), [0, 1, 1])
+        assert ref.push_theta_gamma(lat) == 0
+        assert st.theta[n] == ref.theta[0] and st.gamma[n] == ref.gamma[0], n
+        assert abs(st.gamma[n] - g0[n]) > 1e-6  # the carried Em moves gamma
+    # particles 1 and 5 set Em from their own slots
+    for n, seg in ((1, [0, 1, 1]), (5, [0, 0, 1])):
+        one = {k: [mom[k][n]] for k in mom}
+        ref, _ = state(particles(1, x=pos["x"][n], y=pos["y"][n], **one), seg)
+        assert ref.push_theta_gamma(lat) == 0
+        assert st.theta[n] == ref.theta[0] and st.gamma[n] == ref.gamma[0], n
