@@ -18,6 +18,7 @@ MAX_HARM = 8
 OK = 0
 ERR_ARG, ERR_HIP, ERR_GUARD, ERR_RANGE, ERR_NAN, ERR_UNSUPPORTED, ERR_STATE, ERR_IO = -1, -2, -3, -4, -5, -6, -7, -8
 PLAIN_FP64, MIXED_FP32 = 0, 1
+DEPOSIT_ATOMIC, DEPOSIT_FIXED = 0, 1
 UX, UY, SCUX, SCUY, EZ = 0, 1, 2, 3, 4
 
 # every entry point include/felis.h declares
@@ -36,7 +37,7 @@ EXPORTS = [
     "felis_run_surrogate", "felis_state_save", "felis_state_restore", "felis_slice_moments",
     "felis_jump_residual", "felis_slice_features", "felis_mlp_shape", "felis_mlp_set_input_affine",
     "felis_mlp_get_input_affine", "felis_jump_emission_residual", "felis_live_polarisations",
-    "felis_set_tuning", "felis_get_tuning", "felis_get_desc", "felis_particle_layout", "felis_deck_save_settings",
+    "felis_set_tuning", "felis_get_tuning", "felis_set_deposit_mode", "felis_get_deposit_mode", "felis_get_desc", "felis_particle_layout", "felis_deck_save_settings",
     "felis_field_norms", "felis_field_diag", "felis_wake_function", "felis_deck_wake_function_device",
 ]
 # every entry point include/felis_hdf.h declares (libfelis_hdf.so)
@@ -149,6 +150,8 @@ def load() -> C.CDLL:
         L.felis_deck_wake_function_device.argtypes = [C.c_void_p, _dp, _dp]
         L.felis_field_diag.argtypes = [C.c_void_p, C.c_double, _dp, _dp]
         L.felis_get_tuning.argtypes = [C.c_void_p, C.POINTER(Tuning)]
+        L.felis_set_deposit_mode.argtypes = [C.c_void_p, C.c_int]
+        L.felis_get_deposit_mode.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.felis_slice_features.argtypes = [C.c_void_p, C.POINTER(C.c_float)]
         L.felis_mlp_shape.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.felis_mlp_set_input_affine.argtypes = [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]
@@ -417,6 +420,15 @@ class Context:
         """Launch shapes of the batched step (felis_tuning)."""
         t = Tuning(int(step_graph), int(deposit_lds), int(deposit_threads), int(adi_threads))
         self._chk(self.L.felis_set_tuning(self.h, C.byref(t)))
+
+    def set_deposit_mode(self, mode: int):
+        """DEPOSIT_ATOMIC or DEPOSIT_FIXED (felis_set_deposit_mode)."""
+        self._chk(self.L.felis_set_deposit_mode(self.h, int(mode)))
+
+    def deposit_mode(self) -> int:
+        m = C.c_int()
+        self._chk(self.L.felis_get_deposit_mode(self.h, C.byref(m)))
+        return m.value
 
     def tuning(self) -> dict:
         t = Tuning()

@@ -114,6 +114,7 @@ struct felis_ctx {
   } snap[2];
 
   felis_tuning tune{};  // launch shapes (felis_set_tuning); zero: the defaults
+  int dep_mode = FELIS_DEPOSIT_ATOMIC;  // batched deposit accumulation (felis_set_deposit_mode)
 
   // while a step is captured as a graph: the device word holding the ring
   // position, which grid() hands to the kernels (null otherwise)

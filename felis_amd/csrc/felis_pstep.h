@@ -552,7 +552,32 @@ __global__ void __launch_bounds__(kTile)
 // that the loop does not take.
 // NPOL = 1 deposits the x polarisation only: the y source is exactly zero
 // (eta_Y = 0: a planar undulator with K0_alpha = 1) and Uy is zero.
-template <int NH, int NPOL, bool LM>
+// DET (felis_set_deposit_mode FELIS_DEPOSIT_FIXED): the window and the radial
+// row accumulate in 64-bit fixed point instead of float and double atomics.
+// Integer addition is exact and commutative, so the result no longer depends
+// on the order in which particles reach a cell: the deposit is bitwise the
+// same from run to run and for every thread count and LDS budget. The
+// scales are powers of two from a bound on the slice's total deposit (the
+// particle count times the largest single contribution, computed from the
+// slice's largest 1/gamma and xi2 / gamma), which keeps every sum below 2^61;
+// the resolution is 2^-61 of that bound, far below the FP32 rounding of the
+// source and the FP64 rounding of the radial row.
+template <bool DET>
+__device__ inline void lds_add(void *base, int k, float v, double sc) {
+  if (DET) atomicAdd(reinterpret_cast<unsigned long long *>(base) + k, (unsigned long long)__double2ll_rn((double)v * sc));
+  else atomicAdd(reinterpret_cast<float *>(base) + k, v);
+}
+template <bool DET>
+__device__ inline void lds_add(cpx<double> *base, int k, double v, double sc) {
+  if (DET) atomicAdd(reinterpret_cast<unsigned long long *>(base) + k, (unsigned long long)__double2ll_rn(v * sc));
+  else atomicAdd(reinterpret_cast<double *>(base) + k, v);
+}
+// power-of-two scale with bound * scale < 2^61 (1 for a zero bound)
+__device__ inline double fixed_scale(double bound) {
+  return bound > 0.0 ? ldexp(1.0, 60 - ilogb(bound)) : 1.0;
+}
+
+template <int NH, int NPOL, bool LM, bool DET>
 __global__ void __launch_bounds__(1024)
     k_deposit_slot(GridArgs g, DepArgs d, int n_species, const long long *__restrict__ seg_off,
                    const double *__restrict__ seg_w, const double *__restrict__ theta,
@@ -598,26 +623,62 @@ __global__ void __launch_bounds__(1024)
     atomicMin(&bb[0], mi); atomicMax(&bb[1], Mi); atomicMin(&bb[2], mj); atomicMax(&bb[3], Mj);
   }
   __syncthreads();
+  bool any_even = false;
+#pragma unroll
+  for (int h = 0; h < NH; ++h) any_even |= (g.harm[h] % 2 == 0);
+  const float ka = (float)d.ka;
+  // DET: the slice's largest |macro_u| / gamma and |macro_u| xi2 / gamma (as
+  // the deposit below forms them) and its particle count
+  __shared__ unsigned dmax[2];
+  long long cnt_all = 0;
+  if (DET) {
+    if (tid < 2) dmax[tid] = 0u;
+    __syncthreads();
+    float am = 0.f, xm = 0.f;
+    for (int sp = 0; sp < n_species; ++sp) {
+      const int seg = sp * g.slice_n + sl;
+      const long long b = seg_off[seg], e = seg_off[seg + 1];
+      cnt_all += e - b;
+      const float macro_u = fabsf((float)(seg_w[seg] * d.coef_U));
+      for (long long n = b + tid; n < e; n += nt) {
+        const float a0 = __fdividef(macro_u, (float)(g.gref + (double)gamma[n]));
+        am = fmaxf(am, a0);
+        if (any_even) {
+          const float ppx = px[n], ppy = py[n], ue = (float)d.ue;
+          const float xi2 = (float)d.xi2c * sqrtf((1 + ka) * (ppx * ppx + ue * ue * ppy * ppy) +
+                                                  (1 - ka) * (ppx * ppx * ue * ue + ppy * ppy));
+          xm = fmaxf(xm, a0 * xi2);
+        }
+      }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      am = fmaxf(am, __shfl_xor(am, o));
+      xm = fmaxf(xm, __shfl_xor(xm, o));
+    }
+    // non-negative floats order as their bit patterns
+    if ((tid & 63) == 0) {
+      atomicMax(&dmax[0], __float_as_uint(am));
+      atomicMax(&dmax[1], __float_as_uint(xm));
+    }
+  }
   const int wi0 = bb[0], wj0 = bb[2];
   const bool empty = bb[1] < 0;
   const int ww = empty ? 0 : bb[1] + 2 - wi0, wh = empty ? 0 : bb[3] + 2 - wj0;
   const int W = ww * wh;
-  int hp = W > 0 ? lds_bytes / (W * NPOL * (int)sizeof(cpx<float>)) : NH;
+  constexpr int CB = DET ? 2 * (int)sizeof(long long) : (int)sizeof(cpx<float>);  // bytes per window cell
+  int hp = W > 0 ? lds_bytes / (W * NPOL * CB) : NH;
   if (hp > NH) hp = NH;
   const int step = hp > 0 ? hp : 1;
   // A window larger than the budget for one harmonic goes through LDS in
   // strips of node rows that fit, one harmonic per pass over the particles;
   // only when not even one row fits are the window's interior cells cleared
   // in Sc and added to with global atomics.
-  const int rows_fit = ww > 0 ? lds_bytes / (ww * NPOL * (int)sizeof(cpx<float>)) : 0;
+  const int rows_fit = ww > 0 ? lds_bytes / (ww * NPOL * CB) : 0;
   const bool in_lds = hp > 0 || rows_fit > 0;
   const int sh = hp == 0 && rows_fit > 0 ? rows_fit : max(wh, 1);  // node rows per strip
   // the only interior Sc cells this deposit can make nonzero
   if (tid == 0) win_out[slot] = empty ? make_int4(1, 0, 1, 0) : make_int4(wi0, wi0 + ww - 1, wj0, wj0 + wh - 1);
-  bool any_even = false;
-#pragma unroll
-  for (int h = 0; h < NH; ++h) any_even |= (g.harm[h] % 2 == 0);
-  const float ka = (float)d.ka;
   // per harmonic, from the lattice only: odd H keeps eta (J1 -+ ka J2) H as
   // two complex factors; even H keeps J1, J2, J3 (solveField.cpp:270-300)
   __shared__ float hfac[FELIS_MAX_HARM][4];
@@ -652,6 +713,29 @@ __global__ void __launch_bounds__(1024)
     if (tid == 0) atomicOr(err, EB_BESSEL);
     return;
   }
+  // DET: fixed-point scales of the window (from the largest contribution of
+  // any harmonic: |eta f_H| |macro_u| / gamma, bilinear weights <= 1) and of
+  // the radial row (|coef_Ez macro| (l+1), radial weights below 1.5)
+  __shared__ double dsc[2];
+  if (DET && tid == 0) {
+    const double am = __uint_as_float(dmax[0]), xm = __uint_as_float(dmax[1]);
+    const double eta = fmax(hypot(d.etaXr, d.etaXi), hypot(d.etaYr, d.etaYi)), kb = fabs(d.ka);
+    double fmaxc = 0.0;
+    for (int h = 0; h < NH; ++h) {
+      const int H = g.harm[h];
+      const double f = H % 2 == 1 ? am * fmax(hypot((double)hfac[h][0], (double)hfac[h][1]),
+                                              hypot((double)hfac[h][2], (double)hfac[h][3]))
+                                  : xm * eta * 0.5 * H * H *
+                                        (fabs(hfac[h][0]) + (1.0 + kb) * fabs(hfac[h][1]) + kb * fabs(hfac[h][2]));
+      fmaxc = fmax(fmaxc, f);
+    }
+    dsc[0] = fixed_scale(2.0 * (double)cnt_all * fmaxc);
+    const int segl = (n_species - 1) * g.slice_n + sl;
+    const double cl = (double)(seg_off[segl + 1] - seg_off[segl]);
+    dsc[1] = fixed_scale(4.0 * cl * fabs(d.coef_Ez * seg_w[segl]) * (LM ? g.L : 1));
+  }
+  __syncthreads();
+  const double fsc = DET ? dsc[0] : 1.0, rsc = DET ? dsc[1] : 1.0;
   const cpx<float> etaX = mk((float)d.etaXr, (float)d.etaXi), etaY = mk((float)d.etaYr, (float)d.etaYi);
   cpx<float> *win = reinterpret_cast<cpx<float> *>(lds_d);
   if (!in_lds && W > 0) {
@@ -672,7 +756,10 @@ __global__ void __launch_bounds__(1024)
     // when it fits); cells outside the window take global atomics, once
     const int srh = min(sh, wh - sr0), Ws = ww * max(srh, 0);
     if (in_lds) {
-      for (int q = tid; q < hc * NPOL * Ws; q += nt) win[q] = mk(0.f, 0.f);
+      for (int q = tid; q < hc * NPOL * Ws; q += nt) {
+        if (DET) reinterpret_cast<double2 *>(lds_d)[q] = make_double2(0.0, 0.0);
+        else win[q] = mk(0.f, 0.f);
+      }
       __syncthreads();
     }
     for (int sp = 0; sp < n_species; ++sp) {
@@ -697,13 +784,13 @@ __global__ void __launch_bounds__(1024)
         const int row = kJ + jj - wj0 - sr0;                                                                    \
         if (kin && row >= 0 && row < srh) {                                                                     \
           const int q = row * ww + (kI + ii - wi0);                                                             \
-          float *p0 = reinterpret_cast<float *>(lds_d) + 2 * ((hh * NPOL + 0) * Ws + q);                        \
-          atomicAdd(p0, vx.re);                                                                                 \
-          atomicAdd(p0 + 1, vx.im);                                                                             \
+          const int k0 = 2 * ((hh * NPOL + 0) * Ws + q);                                                        \
+          lds_add<DET>((void *)lds_d, k0, vx.re, fsc);                                                          \
+          lds_add<DET>((void *)lds_d, k0 + 1, vx.im, fsc);                                                      \
           if (NPOL == 2) {                                                                                      \
-            float *p1 = reinterpret_cast<float *>(lds_d) + 2 * ((hh * NPOL + 1) * Ws + q);                      \
-            atomicAdd(p1, vy.re);                                                                               \
-            atomicAdd(p1 + 1, vy.im);                                                                           \
+            const int k1 = 2 * ((hh * NPOL + 1) * Ws + q);                                                      \
+            lds_add<DET>((void *)lds_d, k1, vy.re, fsc);                                                        \
+            lds_add<DET>((void *)lds_d, k1 + 1, vy.im, fsc);                                                    \
           }                                                                                                     \
         } else if (!kin && sr0 == 0) {                                                                          \
           const long long o =                                                                                   \
@@ -764,7 +851,14 @@ __global__ void __launch_bounds__(1024)
         }
         const cpx<float> em1 = conjg(expi_f32(th + d.Phi * 0.5));  // e^{-i psi}
         const cpx<float> em2 = em1 * em1;
-        cpx<float> pw = cpow_int(em1, g.harm[h0]);
+        // e^{-iH psi} by the same chain of products from the first harmonic
+        // whichever harmonics this pass takes, so that a pass's values do not
+        // depend on how the harmonics are split over passes
+        cpx<float> pw = cpow_int(em1, g.harm[0]);
+        for (int h = 1; h <= h0; ++h) {
+          const int dH = g.harm[h] - g.harm[h - 1];
+          pw = pw * (dH == 1 ? em1 : dH == 2 ? em2 : cpow_int(em1, dH));
+        }
 #pragma unroll
         for (int hh = 0; hh < NH; ++hh) {
           if (hh >= hc) break;
@@ -802,7 +896,13 @@ __global__ void __launch_bounds__(1024)
     if (in_lds) {
       __syncthreads();
       for (int q = tid; q < hc * NPOL * Ws; q += nt) {
-        const cpx<float> v = win[q];
+        cpx<float> v;
+        if (DET) {
+          const long long *wi = reinterpret_cast<const long long *>(lds_d) + 2 * q;
+          v = mk((float)((double)wi[0] / fsc), (float)((double)wi[1] / fsc));
+        } else {
+          v = win[q];
+        }
         const int hh = q / (NPOL * Ws), pol = (q / Ws) % NPOL, r = q % Ws;
         const int jy = wj0 + sr0 + r / ww, ix = wi0 + r % ww;
         cpx<float> *dst = (pol ? Scy : Scx) + (long long)(h0 + hh) * g.S * g.N + (long long)slot * g.N +
@@ -830,7 +930,7 @@ __global__ void __launch_bounds__(1024)
   cpx<double> *hist = reinterpret_cast<cpx<double> *>(lds_d);
   const bool priv = (long long)nwv * rows * (long long)sizeof(cpx<double>) <= lds_bytes;
   cpx<double> *mine = priv ? hist + (long long)wid * rows : hist;
-  for (int q = tid; q < (priv ? nwv : 1) * rows; q += nt) hist[q] = mk(0.0, 0.0);
+  for (int q = tid; q < (priv ? nwv : 1) * rows; q += nt) hist[q] = mk(0.0, 0.0);  // zero bits in DET too
   __syncthreads();
   {
     const int seg = (n_species - 1) * g.slice_n + sl;
@@ -844,14 +944,13 @@ __global__ void __launch_bounds__(1024)
       cpx<double> sum[2] = {mk(0.0, 0.0), mk(0.0, 0.0)};  // F = 1 and F > 1 use f = 0 here
 #define FELIS_RADIAL_FLUSH()                                                                 \
   if (key > 0) {                                                                             \
-    cpx<double> *hp2 = mine + key;                                                           \
-    atomicAdd(&hp2[0].re, sum[0].re * wr0);                                                  \
-    atomicAdd(&hp2[0].im, sum[0].im * wr0);                                                  \
-    atomicAdd(&hp2[1].re, sum[0].re * wr1);                                                  \
-    atomicAdd(&hp2[1].im, sum[0].im * wr1);                                                  \
+    lds_add<DET>(mine, 2 * key, sum[0].re * wr0, rsc);                                       \
+    lds_add<DET>(mine, 2 * key + 1, sum[0].im * wr0, rsc);                                   \
+    lds_add<DET>(mine, 2 * key + 2, sum[0].re * wr1, rsc);                                   \
+    lds_add<DET>(mine, 2 * key + 3, sum[0].im * wr1, rsc);                                   \
   } else if (key == 0) {                                                                     \
-    atomicAdd(&mine[0].re, sum[1].re);                                                       \
-    atomicAdd(&mine[0].im, sum[1].im);                                                       \
+    lds_add<DET>(mine, 0, sum[1].re, rsc);                                                   \
+    lds_add<DET>(mine, 1, sum[1].im, rsc);                                                   \
   }
       for (long long n = r0; n < r1; ++n) {
         const double xx = x[n], yy = y[n];
@@ -874,11 +973,11 @@ __global__ void __launch_bounds__(1024)
                 double sn, cs;
                 sincos(-((l + 1.0) * theta[n] + f * ph), &sn, &cs);
                 const double a = amp * (l + 1.0);
-                cpx<double> *hp2 = mine + l * row + f * nr + idxR;
-                atomicAdd(&hp2[0].re, -sn * a * w0);
-                atomicAdd(&hp2[0].im, cs * a * w0);
-                atomicAdd(&hp2[1].re, -sn * a * w1);
-                atomicAdd(&hp2[1].im, cs * a * w1);
+                const int k0 = 2 * (l * row + f * nr + idxR);
+                lds_add<DET>(mine, k0, -sn * a * w0, rsc);
+                lds_add<DET>(mine, k0 + 1, cs * a * w0, rsc);
+                lds_add<DET>(mine, k0 + 2, -sn * a * w1, rsc);
+                lds_add<DET>(mine, k0 + 3, cs * a * w1, rsc);
               }
             continue;
           }
@@ -904,8 +1003,8 @@ __global__ void __launch_bounds__(1024)
               double sn, cs;
               sincos(-(l + 1.0) * theta[n], &sn, &cs);
               const double a = d.coef_Ez * macro * (l + 1.0);
-              atomicAdd(&mine[l * row].re, -sn * a);
-              atomicAdd(&mine[l * row].im, cs * a);
+              lds_add<DET>(mine, 2 * l * row, -sn * a, rsc);
+              lds_add<DET>(mine, 2 * l * row + 1, cs * a, rsc);
             }
             continue;
           }
@@ -923,9 +1022,21 @@ __global__ void __launch_bounds__(1024)
   }
   __syncthreads();
   for (int q = tid; q < rows; q += nt) {
-    cpx<double> v = hist[q];
-    if (priv)
-      for (int w = 1; w < nwv; ++w) v = v + hist[(long long)w * rows + q];
+    cpx<double> v;
+    if (DET) {
+      const long long *hi = reinterpret_cast<const long long *>(hist);
+      long long re = hi[2 * q], im = hi[2 * q + 1];
+      if (priv)
+        for (int w = 1; w < nwv; ++w) {
+          re += hi[2 * ((long long)w * rows + q)];
+          im += hi[2 * ((long long)w * rows + q) + 1];
+        }
+      v = mk((double)re / rsc, (double)im / rsc);
+    } else {
+      v = hist[q];
+      if (priv)
+        for (int w = 1; w < nwv; ++w) v = v + hist[(long long)w * rows + q];
+    }
     ScR[(long long)slot * rows + q] = v;
   }
 }

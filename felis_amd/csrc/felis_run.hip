@@ -187,19 +187,32 @@ int launch_deposit_slot(felis_ctx *ctx, const DepArgs &dep, int lds) {
   KTimer kt(ctx, FELIS_K_DEPOSIT);
   cpx<float> *Scx = (cpx<float> *)ctx->Sc, *Scy = (cpx<float> *)((char *)ctx->Sc + ctx->pol_bytes());
   const int nt = ctx->tune.deposit_threads ? ctx->tune.deposit_threads : 1024;
-#define DEPOSIT_LAUNCH(NPOL, LM)                                                                             \
-  hipLaunchKernelGGL((k_deposit_slot<NH, NPOL, LM>), dim3(ctx->d.slice_n), dim3(nt), lds, ctx->stream, g, dep,   \
-                     ctx->d.n_species, ctx->seg_off, ctx->seg_w, ctx->theta, (const float *)ctx->gamma,          \
+#define DEPOSIT_LAUNCH(NPOL, LM, DET)                                                                        \
+  hipLaunchKernelGGL((k_deposit_slot<NH, NPOL, LM, DET>), dim3(ctx->d.slice_n), dim3(nt), lds, ctx->stream, g,   \
+                     dep, ctx->d.n_species, ctx->seg_off, ctx->seg_w, ctx->theta, (const float *)ctx->gamma,     \
                      (const float *)ctx->x, (const float *)ctx->y, (const float *)ctx->px, (const float *)ctx->py, \
                      ctx->bessel, Scx, Scy, ctx->ScR, ctx->win, lds, ctx->errw)
+#define DEPOSIT_LAUNCH_DET(NPOL, LM)                 \
+  if (ctx->dep_mode == FELIS_DEPOSIT_FIXED) {        \
+    DEPOSIT_LAUNCH(NPOL, LM, true);                  \
+  } else {                                           \
+    DEPOSIT_LAUNCH(NPOL, LM, false);                 \
+  }
   const bool lm = ctx->d.sc_l > 1;
   if (ctx->uy_live) {
-    if (lm) DEPOSIT_LAUNCH(2, true);
-    else DEPOSIT_LAUNCH(2, false);
+    if (lm) {
+      DEPOSIT_LAUNCH_DET(2, true);
+    } else {
+      DEPOSIT_LAUNCH_DET(2, false);
+    }
   } else {
-    if (lm) DEPOSIT_LAUNCH(1, true);
-    else DEPOSIT_LAUNCH(1, false);
+    if (lm) {
+      DEPOSIT_LAUNCH_DET(1, true);
+    } else {
+      DEPOSIT_LAUNCH_DET(1, false);
+    }
   }
+#undef DEPOSIT_LAUNCH_DET
 #undef DEPOSIT_LAUNCH
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
@@ -249,8 +262,14 @@ void reduce_diag(felis_ctx *ctx, int skip, const double *tile_red, const int *sp
 
 template <int NH>
 hipError_t set_deposit_lds(int lds) {
-  const void *fn[4] = {(const void *)k_deposit_slot<NH, 1, false>, (const void *)k_deposit_slot<NH, 2, false>,
-                       (const void *)k_deposit_slot<NH, 1, true>, (const void *)k_deposit_slot<NH, 2, true>};
+  const void *fn[8] = {(const void *)k_deposit_slot<NH, 1, false, false>,
+                       (const void *)k_deposit_slot<NH, 2, false, false>,
+                       (const void *)k_deposit_slot<NH, 1, true, false>,
+                       (const void *)k_deposit_slot<NH, 2, true, false>,
+                       (const void *)k_deposit_slot<NH, 1, false, true>,
+                       (const void *)k_deposit_slot<NH, 2, false, true>,
+                       (const void *)k_deposit_slot<NH, 1, true, true>,
+                       (const void *)k_deposit_slot<NH, 2, true, true>};
   for (const void *f : fn) {
     const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     if (e != hipSuccess) return e;
@@ -463,9 +482,11 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
   // strips of node rows, and then builds the radial source there; the tuning
   // may give it less LDS than the device allows. Its budget never goes below
   // one node row of the widest window (nx cells of both polarisations) and
-  // one radial row per mode (sc_l (sc_f nr + 1) complex doubles), so both always fit.
+  // one radial row per mode (sc_l (sc_f nr + 1) complex doubles), so both always fit
+  // (a fixed-point deposit keeps 16 bytes per window cell instead of 8).
+  const int cell_bytes = ctx->dep_mode == FELIS_DEPOSIT_FIXED ? 16 : 8;
   const int dep_floor =
-      std::max(d.nx * 2 * (int)sizeof(float) * 2, d.sc_l * (d.sc_f * d.nr + 1) * 2 * (int)sizeof(double));
+      std::max(d.nx * 2 * cell_bytes, d.sc_l * (d.sc_f * d.nr + 1) * 2 * (int)sizeof(double));
   if (dep_floor > ctx->lds_max) {
     ctx->err = "the radial source row (number_azimuthal_mode x radial_grids) exceeds the LDS of one workgroup";
     return FELIS_ERR_UNSUPPORTED;
@@ -957,6 +978,22 @@ extern "C" int felis_set_tuning(felis_ctx *ctx, const felis_tuning *t) {
 extern "C" int felis_get_tuning(felis_ctx *ctx, felis_tuning *t) {
   if (!ctx || !t) return FELIS_ERR_ARG;
   *t = ctx->tune;
+  return FELIS_OK;
+}
+
+extern "C" int felis_set_deposit_mode(felis_ctx *ctx, int mode) {
+  if (!ctx) return FELIS_ERR_ARG;
+  if (mode != FELIS_DEPOSIT_ATOMIC && mode != FELIS_DEPOSIT_FIXED) {
+    ctx->err = "felis_set_deposit_mode: FELIS_DEPOSIT_ATOMIC or FELIS_DEPOSIT_FIXED";
+    return FELIS_ERR_ARG;
+  }
+  ctx->dep_mode = mode;
+  return FELIS_OK;
+}
+
+extern "C" int felis_get_deposit_mode(felis_ctx *ctx, int *mode) {
+  if (!ctx || !mode) return FELIS_ERR_ARG;
+  *mode = ctx->dep_mode;
   return FELIS_OK;
 }
 

@@ -250,6 +250,18 @@ typedef struct {
 int felis_set_tuning(felis_ctx *ctx, const felis_tuning *t);
 int felis_get_tuning(felis_ctx *ctx, felis_tuning *t);
 
+/* How the batched step's deposit (solveField.cpp:246-359, whose serial loops
+ * add in particle order) accumulates: FELIS_DEPOSIT_ATOMIC adds with float
+ * and double atomics, whose order varies from run to run (results agree to
+ * rounding); FELIS_DEPOSIT_FIXED adds in 64-bit fixed point, which is exact
+ * and order-free, so the step is bitwise reproducible from run to run and
+ * across deposit launch shapes (for diagnosing against the atomic mode; it
+ * reads the particles once more and keeps 16 bytes per window cell). The
+ * per-phase FP64 deposit is not affected. FELIS_ERR_ARG for another mode. */
+enum { FELIS_DEPOSIT_ATOMIC = 0, FELIS_DEPOSIT_FIXED = 1 };
+int felis_set_deposit_mode(felis_ctx *ctx, int mode);
+int felis_get_deposit_mode(felis_ctx *ctx, int *mode);
+
 /* Polarisations the batched step solves (1: the y field and its source are
  * exactly zero so far and are skipped; 2: both). No counterpart in the
  * reference, which always solves both. */
