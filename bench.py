@@ -34,6 +34,8 @@ WORKLOADS = {
     "cfg1": "time-dependent 256 slices x 4096 particles, 101x101 grid, harmonic 1, flat 3 kA, 10 m (385 periods)",
     "cfg2": "time-dependent 2048 slices, 256-16384 particles/slice (11,670,128), 129x129, harmonics 1,3,5, "
             "Gaussian 4 kA, 20 m (770 periods)",
+    "cfg2e": "cfg2 in an elliptical undulator (polarity 0.5, K 1.6726 for the same resonance): both polarisations "
+             "coupled and solved",
     "cfg3": "time-dependent 4096 slices x 8192 particles (33,554,432), 151x151, harmonic 1, Gaussian 3 kA, six 5 m "
             "units with 1 m drifts, -1%/unit taper from the fourth, FODO quads, circular AC wake, 29 m "
             "(1116 periods)",
@@ -69,8 +71,11 @@ def kernel_bytes(name: str, d, np_: int, mixed: bool, npol: int = 2) -> float:
 # kernel-timer name -> rocprofv3 kernel name, for the measured HBM traffic
 # (the field solve's name carries its workgroup size, a tuned launch shape)
 PROFILED = {("cfg2", "mixed"): {"adi_fused": "felis::k_adi_scan<8, {adi_threads}>",
-                                "particle_step": "felis::k_pstep<3, false>",
-                                "deposit": "felis::k_deposit_slot<3, 1>"}}
+                                "particle_step": "felis::k_pstep<3, false, false>",
+                                "deposit": "felis::k_deposit_slot<3, 1, false, false>"},
+            ("cfg2e", "mixed"): {"adi_fused": "felis::k_adi_scan<8, {adi_threads}>",
+                                 "particle_step": "felis::k_pstep<3, true, false>",
+                                 "deposit": "felis::k_deposit_slot<3, 2, false, false>"}}
 
 
 def measured_traffic(config: str, precision: str, kernel: str, tuning=None):
@@ -128,6 +133,37 @@ def run_gpu(args, precision=None, steps=None, warmup=None, keep_init=None):
     return dict(deck=deck, desc=desc, ctx=ctx, seconds=t1 - t0, prof=prof, energy=energy, init=init, tab=tab,
                 tuning=tuning,
                 wakeF=wakeF, np=ctx.np, npol=ctx.live_polarisations() if precision == "mixed" else 2)
+
+
+def roofline(prof, d, np_, mixed, npol, config, precision, tuning):
+    """The dominant kernel of a profiled run against HBM peak."""
+    dom = max(prof, key=lambda k: prof[k][0])
+    ms, n = prof[dom]
+    avg_s = ms / n * 1e-3
+    bpl = kernel_bytes(dom, d, np_, mixed, npol)
+    achieved = bpl / avg_s / 1e9 if bpl else 0.0
+    return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS, "traffic": measured_traffic(config, precision, dom, tuning),
+            "bytes_per_launch": bpl, "avg_launch_us": avg_s * 1e6}
+
+
+def two_polarisation_leg(args, steps: int = 20):
+    """The same beam in an elliptical undulator (cfg2e): both polarisations
+    are coupled, so the batched step solves, deposits and gathers Ux and Uy.
+    Timed like the main workload, over fewer steps."""
+    import copy
+    a = copy.copy(args)
+    a.config = "cfg2e"
+    g = run_gpu(a, steps=steps, warmup=3, keep_init=False)
+    d = g["desc"]
+    out = {"workload": f"cfg2e: {WORKLOADS['cfg2e']}", "steps": steps,
+           "ms_per_step": g["seconds"] / steps * 1e3, "value": d.slice_n * steps / g["seconds"],
+           "polarisations_solved": g["npol"],
+           "roofline": roofline(g["prof"], d, g["np"], True, g["npol"], "cfg2e", "mixed", g["tuning"]),
+           "kernels_ms_per_step": {k: v[0] / args.prof_steps
+                                   for k, v in sorted(g["prof"].items(), key=lambda kv: -kv[1][0])}}
+    g["ctx"].close()
+    return out
 
 
 def host_cores():
@@ -327,6 +363,8 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the whole-run comparison of multi-period jumps (with and without the learned "
                          "correction) against the batched step")
+    ap.add_argument("--no-two-pol", action="store_true",
+                    help="skip the elliptical-undulator leg (both polarisations solved) of the 2048-slice workload")
     ap.add_argument("--tuning", default=None, metavar="JSON",
                     help="autotuner cache the batched step's launch shapes come from (default: "
                          "felis_amd/tuning.json)")
@@ -351,13 +389,7 @@ def main():
     K = args.steps
     value = d.slice_n * K / g["seconds"]
     mixed = args.precision == "mixed"
-    # dominant kernel and its roofline
     prof = g["prof"]
-    dom = max(prof, key=lambda k: prof[k][0])
-    ms, n = prof[dom]
-    avg_s = ms / n * 1e-3
-    bpl = kernel_bytes(dom, d, g["np"], mixed, g["npol"])
-    achieved = bpl / avg_s / 1e9 if bpl else 0.0
     out = {
         "metric": METRIC,
         "value": value,
@@ -375,10 +407,7 @@ def main():
                    "slices": d.slice_n, "particles": g["np"], "grid": f"{d.nx}x{d.ny}",
                    "harmonics": [d.harmony[i] for i in range(d.n_harm)], "polarisations_solved": g["npol"],
                    "tuning": g["tuning"]},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": measured_traffic(args.config, args.precision, dom, g["tuning"]), "bytes_per_launch": bpl,
-                     "avg_launch_us": avg_s * 1e6},
+        "roofline": roofline(prof, d, g["np"], mixed, g["npol"], args.config, args.precision, g["tuning"]),
         "kernels_ms_per_step": {k: v[0] / args.prof_steps for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])},
     }
     if mixed and args.plain_steps > 0:
@@ -388,6 +417,8 @@ def main():
         out["plain_fp64"] = {"ms_per_step": plain_ms, "steps": args.plain_steps,
                              "value": d.slice_n * args.plain_steps / gp["seconds"],
                              "speedup_of_batched": plain_ms / out["ms_per_step"]}
+    if mixed and args.config == "cfg2" and not args.no_two_pol:
+        out["two_polarisation"] = two_polarisation_leg(args)
     if mixed and not args.no_surrogate and g["deck"].mode == 2:
         out["surrogate"] = surrogate_runs(args)
     if not args.no_cpu:

@@ -14,6 +14,9 @@ gamma0; the slice window is (N + 1/2) wavelengths so int() gives N slices.
   cfg2  Time_Dependent, 2048 slices, 129x129, harmonics 1,3,5, Gaussian
         current (sigma 400 slices, 4 kA), max(32, int(2048 n0)) x 8, 20 m
   cfg2s the first 256 slices' worth of cfg2 physics (parity-size cut)
+  cfg2e cfg2 in an elliptical undulator (polarity 0.5, K lowered to keep the
+        resonance): both polarisations couple, so the batched step solves,
+        deposits and reads Ux and Uy (particlePush.cpp:227-238)
   cfg3  Time_Dependent, 4096 slices, 151x151, harmonic 1, 8192 particles per
         slice under a Gaussian current (3 kA), six 5 m units (4 m undulator,
         1 m drift) with a -1%/unit taper from the fourth, FODO quads, circular
@@ -150,6 +153,23 @@ def gauss_beam(current: float, beamlets: int, phases: int, center: int, sigma: i
 """
 
 
+def elliptical_undulator(length: float, ue: float) -> str:
+    """one block with polarity ue and the K of the same resonance:
+    (1 + ue^2) K^2 = K0^2 (parameterSetting.cpp:175)"""
+    return f"""[Undulator]
+ undulator_type = BiPolar
+ polarity = {ue}
+ K0_alpha = 1
+ numbers = 1
+ lambdaU = 2.6        // [cm]
+ K0 = {K0 / math.sqrt(1.0 + ue * ue):.12f}
+ unit_start = 0
+ unit_end = {length + 1.0}
+ undulator_start = 0
+ undulator_end = {length + 1.0}
+"""
+
+
 def segmented_undulator(units_flat: int, units_tapered: int, taper: float) -> str:
     """units of 5 m (4 m undulator, 1 m drift in air, updateK_quadG.cpp:47-70);
     the last units_tapered have
@@ -198,6 +218,8 @@ DECKS = {
                      beam=flat_beam(3000, 1024, 4)),
     "cfg2.inp": dict(mode="Time_Dependent", slices=2048, n=129, harmonics=(1, 3, 5), length=20.0,
                      beam=gauss_beam(4000, 2048, 8, 1024, 400, 32)),
+    "cfg2e.inp": dict(mode="Time_Dependent", slices=2048, n=129, harmonics=(1, 3, 5), length=20.0,
+                      beam=gauss_beam(4000, 2048, 8, 1024, 400, 32), undulator=elliptical_undulator(20.0, 0.5)),
     "cfg2s.inp": dict(mode="Time_Dependent", slices=64, n=129, harmonics=(1, 3, 5), length=20.0,
                       beam=gauss_beam(4000, 256, 8, 32, 12, 8)),
     # six units, the last three tapered, FODO quads, circular AC resistive

@@ -341,7 +341,7 @@ def main():
                     sc_f=int(atof(f("Domain", "number_azimuthal_mode", "1"))))
 
     decks_d = {"deck_lattice.inp": deck}
-    for name in ("cfg0.inp", "cfg1.inp", "cfg2.inp", "cfg2s.inp", "cfg3.inp"):
+    for name in ("cfg0.inp", "cfg1.inp", "cfg2.inp", "cfg2e.inp", "cfg2s.inp", "cfg3.inp"):
         decks_d[name] = ROOT / "felis_amd" / "decks" / name
     (HERE / "deck_desc.json").write_text(json.dumps({k: derived(v) for k, v in decks_d.items()}, indent=1))
     print("wrote", len(out), "npz fixtures and findparam.json to", HERE)

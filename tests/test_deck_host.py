@@ -8,7 +8,7 @@ libfelis_hip.so; no GPU call).
   (tests/golden/update_k_quadg_deck.npz; tests/golden/make_golden.py).
 * felis_deck_desc / felis_deck_steps: the derived Domain scalars of
   parameterSetting.cpp:71-77, 89-111, 168-175, 186-196, 236-240 for this deck
-  and the four workload decks (tests/golden/deck_desc.json).
+  and the five workload decks (tests/golden/deck_desc.json).
 """
 from __future__ import annotations
 
@@ -53,7 +53,8 @@ def test_deck_lattice_table_matches_the_reference():
     assert (seq[:, 4] > 0).any() and (seq[:, 4] < 0).any() and len(set(seq[:, 0])) > 100
 
 
-@pytest.mark.parametrize("name", ["deck_lattice.inp", "cfg0.inp", "cfg1.inp", "cfg2.inp", "cfg2s.inp", "cfg3.inp"])
+@pytest.mark.parametrize("name", ["deck_lattice.inp", "cfg0.inp", "cfg1.inp", "cfg2.inp", "cfg2e.inp", "cfg2s.inp",
+                                  "cfg3.inp"])
 def test_deck_derived_scalars(name):
     """ks, lambda0, the grid (dx from the deck's nx, then nx + 1), the radial
     grid, gamR, dz, maxStep and sliceN (with the Static mode's window) as
