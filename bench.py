@@ -273,15 +273,21 @@ def whole_run_parity(args, g):
     return out
 
 
-def surrogate_runs(args, periods: int = 10, gate: float = 0.1, net_gate: float = 0.05):
-    """The whole run of the deck (every period of the undulator) three ways
-    from the same initial state: the batched step; jumps of `periods` periods
-    while |b1| < gate; the same with the committed correction network (applied
-    while |b1| < net_gate, the range it was trained on). Wall times of the
-    step loop and the relative error of the total field energy (the
-    integrated FEL power) against the batched run. For the 2048-slice
-    workload also the same beam in a 40 m undulator, batched against jumps
-    with the network, to saturation (the first maximum of the power)."""
+def surrogate_runs(args, periods: int = 10, gate: float = 0.05, wide_gate: float = 0.1):
+    """The surrogate-assisted run against the batched one. Jumps of `periods`
+    periods stand in for the fine steps while |b1| < gate (0.05, the range
+    the surrogate is specified for), with the committed correction network;
+    the batched step runs and scores everything else.
+
+    to_saturation (the figure the 3x target is set on, for the 2048-slice
+    workload): the same beam in a 40 m undulator, batched vs jumps without
+    and with the network, to saturation (the first maximum of the power):
+    wall times of the step loop, saturation length and power errors, and the
+    mean |error| of the field energy along the run to saturation (what the
+    network gains over the jumps alone). wider_gate repeats the run with the
+    network while |b1| < 0.1, outside the specified range.
+    whole_run: every period of the deck itself, batched vs jumps with the
+    network (the run ends before saturation)."""
     from felis_amd import _lib as fl
     from felis_amd import surrogate as su
     deck = ROOT / "felis_amd" / "decks" / f"{args.config}.inp"
@@ -289,11 +295,37 @@ def surrogate_runs(args, periods: int = 10, gate: float = 0.1, net_gate: float =
     slices = dk.slice_n
     dk.close()
     model = su.load()
-    rf = su.run_curves(deck, seed=args.seed)
-    rj = su.run_curves(deck, seed=args.seed, periods=periods, gate=gate)
     net = model.network(capacity=slices)
-    rn = su.run_curves(deck, seed=args.seed, periods=periods, gate=gate, net=net, net_gate=net_gate)
-    sat = None
+
+    def filled(r):
+        ok = ~np.isnan(r["E"])
+        idx = np.nonzero(ok)[0]
+        e = r["E"].copy()
+        e[~ok] = np.interp(np.nonzero(~ok)[0], idx, r["E"][idx])
+        return e
+
+    def versus(rf, r, z=None, zs=None, ps=None):
+        ok = ~np.isnan(r["E"])
+        rel = r["E"][ok] / rf["E"][ok] - 1
+        o = {"seconds": r["seconds"], "speedup_of_batched": rf["seconds"] / r["seconds"],
+             "jumps": int(np.sum(r["kind"] == 1)), "periods_in_jumps": int(np.sum(r["kind"] > 0)),
+             "max_power_rel_err": float(np.max(np.abs(rel)))}
+        if zs is None:
+            o["final_power_rel_err"] = float(r["E"][-1] / rf["E"][-1] - 1)
+        else:
+            zj, pj, _ = su.saturation(z, filled(r))
+            m = ok & (z <= zs)
+            o.update(z_sat_rel_err=float(zj / zs - 1), power_sat_rel_err=float(pj / ps - 1),
+                     mean_power_rel_err_to_sat=float(np.mean(np.abs(r["E"][m] / rf["E"][m] - 1))))
+        return o
+
+    rf = su.run_curves(deck, seed=args.seed)
+    rn = su.run_curves(deck, seed=args.seed, periods=periods, gate=gate, net=net, net_gate=gate)
+    out = {"periods_per_jump": periods, "bunching_gate": gate, "model": "felis_amd/models/jump_corr_J10.npz",
+           "network": "per-slice 20-64-64-6 perceptron (BF16 MFMA, FP32 accumulate) scaling the jump's emission; "
+                      "not the field-to-field FNO / U-Net of the surrogate's specification",
+           "whole_run": {"periods": int(rf["steps"]), "batched_seconds": rf["seconds"], "jump_learned": versus(rf, rn)},
+           "model_meta": model.meta}
     if args.config == "cfg2":
         import tempfile
         from felis_amd.decks import make_decks as md
@@ -304,30 +336,20 @@ def surrogate_runs(args, periods: int = 10, gate: float = 0.1, net_gate: float =
             dzs = dk40.desc(fl.MIXED_FP32).dz  # m per step
             dk40.close()
             sf = su.run_curves(p40, seed=args.seed)
-            sn = su.run_curves(p40, seed=args.seed, periods=periods, gate=gate, net=net, net_gate=net_gate)
-        z = np.arange(sf["steps"]) * dzs
-        zs, Ps, found = su.saturation(z, sf["E"])
-        ok = ~np.isnan(sn["E"])
-        idx = np.nonzero(ok)[0]
-        En = sn["E"].copy()
-        En[~ok] = np.interp(np.nonzero(~ok)[0], idx, sn["E"][idx])
-        zj, Pj, _ = su.saturation(z, En)
-        sat = {"undulator_m": 40.0, "periods": int(sf["steps"]), "saturated": bool(found),
-               "batched_seconds": sf["seconds"], "jump_learned_seconds": sn["seconds"],
-               "speedup_of_batched": sf["seconds"] / sn["seconds"], "jumps": int(np.sum(sn["kind"] == 1)),
-               "z_sat_m": zs, "z_sat_rel_err": zj / zs - 1, "power_sat_rel_err": Pj / Ps - 1}
+            sj = su.run_curves(p40, seed=args.seed, periods=periods, gate=gate)
+            sn = su.run_curves(p40, seed=args.seed, periods=periods, gate=gate, net=net, net_gate=gate)
+            sw = su.run_curves(p40, seed=args.seed, periods=periods, gate=wide_gate, net=net, net_gate=gate)
+        z = np.arange(sf["steps"], dtype=float)
+        zs, ps, found = su.saturation(z, sf["E"])
+        learned = versus(sf, sn, z, zs, ps)
+        out["to_saturation"] = {
+            "undulator_m": 40.0, "periods": int(sf["steps"]), "saturated": bool(found),
+            "z_sat_m": float(zs * dzs), "batched_seconds": sf["seconds"],
+            "speedup_of_batched": learned["speedup_of_batched"], "target_speedup": 3.0,
+            "jump": versus(sf, sj, z, zs, ps), "jump_learned": learned,
+            "wider_gate": dict(versus(sf, sw, z, zs, ps), bunching_gate=wide_gate)}
     net.close()
-
-    def err(r):
-        ok = ~np.isnan(r["E"])
-        rel = r["E"][ok] / rf["E"][ok] - 1
-        return {"seconds": r["seconds"], "speedup_of_batched": rf["seconds"] / r["seconds"],
-                "jumps": int(np.sum(r["kind"] == 1)), "periods_in_jumps": int(np.sum(r["kind"] > 0)),
-                "final_power_rel_err": float(r["E"][-1] / rf["E"][-1] - 1), "max_power_rel_err": float(
-                    np.max(np.abs(rel)))}
-    return {"periods_per_jump": periods, "bunching_gate": gate, "network_bunching_gate": net_gate,
-            "periods": int(rf["steps"]), "batched_seconds": rf["seconds"], "jump": err(rj), "jump_learned": err(rn),
-            "saturation": sat, "model": "felis_amd/models/jump_corr_J10.npz", "model_meta": model.meta}
+    return out
 
 
 def calibrate_copy(mib: int):

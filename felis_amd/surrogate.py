@@ -387,52 +387,6 @@ def run_curves(deck_path, seed: int, periods: int | None = None, gate: float = 0
     return dict(E=E, b1=b1, kind=kind, seconds=dt, final=float(fin), steps=N)
 
 
-def single_jump_errors(model: "Model | None", deck_path, seed: int, at_steps, periods: int = 10):
-    """One jump from states of the fine run at each step in at_steps, with and
-    without the network, against the fine steps from the same state: per
-    state the weighted mean |U_jump - U_fine|^2 / |U_fine|^2 of the
-    fundamental (weights: fine field energy per slice)."""
-    deck = fl.Deck(deck_path)
-    d = deck.desc(fl.MIXED_FP32)
-    nh = d.n_harm
-    ctx = fl.Context(d)
-    deck.load(ctx, seed=seed)
-    N = max(at_steps) + periods
-    tab = deck.lattice(0, N)
-    net = model.network(capacity=d.slice_n) if model is not None else None
-    out = []
-    k = 0
-    J = periods
-    for ks in sorted(at_steps):
-        if ks > k:
-            ctx.run(tab[2 * k:2 * ks + 1], k, ks - k, outputs=False)
-            k = ks
-        ctx.save_state(0)
-        ctx.run(tab[2 * k:2 * (k + J) + 1], k, J, outputs=False)
-        ctx.save_state(1)
-        row = dict(step=ks)
-        for tag, nn in (("jump", None), ("jump_net", net)):
-            if tag == "jump_net" and nn is None:
-                continue
-            ctx.restore_state(0)
-            ctx.run_surrogate(tab[2 * k:2 * (k + J) + 1], k, J, periods=J, bunching_max=math.inf, max_jumps=1,
-                              net=nn)
-            f, _ = ctx.jump_residual(1)
-            a2 = f[0, :, 2] + f[nh, :, 2]
-            b2 = f[0, :, 3] + f[nh, :, 3]
-            ab = f[0, :, 0] + f[nh, :, 0]
-            diff = a2 + b2 - 2 * ab          # |a - b|^2 per slice
-            row[tag] = float(np.sum(diff) / np.sum(b2))
-        ctx.restore_state(1)
-        k += J
-        out.append(row)
-    if net is not None:
-        net.close()
-    ctx.close()
-    deck.close()
-    return out
-
-
 def _main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(description="train the jump correction network on the GPU")

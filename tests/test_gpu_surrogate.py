@@ -1,8 +1,9 @@
 """The matrix-core perceptron of the surrogate against its CPU reference
 (oracle/mlp_ref.py): the BF16 product's lane layout, forward outputs and
 gradients, and that Adam training reduces the loss; the committed
-correction network on single jumps, along a run, and to saturation against
-the batched run (2% in saturation length and power)."""
+correction network along a run and to saturation against the batched run
+(2% in saturation length and power), and what it gains over the jumps
+without it."""
 from pathlib import Path
 
 import numpy as np
@@ -102,20 +103,6 @@ def test_correction_network_learns_the_jump_residual(tmp_path):
     assert su.load(tmp_path / "m.npz").dims == model.dims
 
 
-def test_committed_model_on_single_jumps():
-    """What the committed network gains over the jump alone, one jump from
-    states of the fine run of the 64-slice deck: the field error
-    |U_jump - U_fine|^2 / |U_fine|^2 with and without it (printed). The
-    measured gain is about 1% of that error; the test holds it to no worse
-    than the jump alone (2% slack for the run-to-run rounding)."""
-    from felis_amd import surrogate as su
-    deck = Path(__file__).resolve().parent.parent / "felis_amd" / "decks" / "cfg2s.inp"
-    rows = su.single_jump_errors(su.load(), deck, 20261203, [31, 151, 301])
-    print(rows)
-    for r in rows:
-        assert r["jump_net"] <= 1.02 * r["jump"], r
-
-
 def test_corrected_jumps_follow_the_fine_run():
     """The committed network on 150 periods of the 64-slice deck: jumps of 10
     periods with the correction stay within 25% of the plain batched run's
@@ -131,31 +118,51 @@ def test_corrected_jumps_follow_the_fine_run():
     assert np.max(np.abs(rj["E"][ok] / rf["E"][ok] - 1)) < 0.25
 
 
-def test_jumps_reach_saturation_within_two_percent(tmp_path):
-    """The target of the jumps with the learned correction: the 2048-slice
-    beam in a 40 m undulator, batched run against 10-period jumps while
-    |b1| < 0.1 with the committed network while |b1| < 0.05, to saturation
-    (the first maximum of the total field energy). Saturation length and
-    power agree within 2% (measured: +0.73% and +0.39%), and the run with
-    jumps takes less time than the batched one."""
+def _sat(z, r):
+    """Saturation point of a run with jumps (energies inside jumps filled in
+    linearly)."""
+    from felis_amd import surrogate as su
+    ok = ~np.isnan(r["E"])
+    idx = np.nonzero(ok)[0]
+    e = r["E"].copy()
+    e[~ok] = np.interp(np.nonzero(~ok)[0], idx, r["E"][idx])
+    return su.saturation(z, e)[:2]
+
+
+def test_jumps_to_saturation_and_the_network_gain(tmp_path):
+    """The 2048-slice beam in a 40 m undulator to saturation (the first
+    maximum of the total field energy): the batched run against 10-period
+    jumps while |b1| < 0.05 (the surrogate's range), without and with the
+    committed correction network. With the network the saturation length and
+    power agree within 2% (measured: +0.66% and +0.42%), and the network
+    gains over the jumps alone (measured: saturation power +1.51% without
+    it, +0.42% with it; the mean |error| of the field energy along the run
+    to saturation 4.1% without, 2.9% with): the test asserts the power error
+    at most half and the mean error at most 0.85 of the jumps' own. The
+    runs with jumps take less time than the batched one."""
     from felis_amd import surrogate as su
     from felis_amd.decks import make_decks as md
     deck = tmp_path / "cfg2_40m.inp"
     deck.write_text(md.deck(**dict(md.DECKS["cfg2.inp"], length=40.0)))
     rf = su.run_curves(deck, seed=20261203)
+    rj = su.run_curves(deck, seed=20261203, periods=10, gate=0.05)
     net = su.load().network(capacity=2048)
-    rn = su.run_curves(deck, seed=20261203, periods=10, gate=0.1, net=net, net_gate=0.05)
+    rn = su.run_curves(deck, seed=20261203, periods=10, gate=0.05, net=net, net_gate=0.05)
     net.close()
     z = np.arange(rf["steps"], dtype=float)
     zs, ps, found = su.saturation(z, rf["E"])
     assert found
-    ok = ~np.isnan(rn["E"])
-    idx = np.nonzero(ok)[0]
-    e = rn["E"].copy()
-    e[~ok] = np.interp(np.nonzero(~ok)[0], idx, rn["E"][idx])
-    zj, pj, _ = su.saturation(z, e)
-    print(dict(z_rel=zj / zs - 1, p_rel=pj / ps - 1, speedup=rf["seconds"] / rn["seconds"]))
+    zj, pj = _sat(z, rj)
+    zn, pn = _sat(z, rn)
+
+    def mean_err(r):
+        m = ~np.isnan(r["E"]) & (z <= zs)
+        return np.mean(np.abs(r["E"][m] / rf["E"][m] - 1))
+    res = dict(z_net=zn / zs - 1, p_net=pn / ps - 1, z_jump=zj / zs - 1, p_jump=pj / ps - 1,
+               mean_net=mean_err(rn), mean_jump=mean_err(rj), speedup=rf["seconds"] / rn["seconds"])
+    print(res)
     assert np.sum(rn["kind"] == 1) >= 50
-    assert abs(zj / zs - 1) < 0.02
-    assert abs(pj / ps - 1) < 0.02
-    assert rn["seconds"] < rf["seconds"]
+    assert abs(res["z_net"]) < 0.02 and abs(res["p_net"]) < 0.02
+    assert abs(res["p_net"]) < 0.5 * abs(res["p_jump"])
+    assert res["mean_net"] < 0.85 * res["mean_jump"]
+    assert rn["seconds"] < rf["seconds"] and rj["seconds"] < rf["seconds"]
