@@ -43,6 +43,7 @@ struct PStepArgs {
   int nred;        // partial-sum row length: 2*nh + 1 + 7
   double gamR;
   double inv_dx, inv_dy, inv_dr;  // grid reciprocals
+  float inv_drf;
   // per harmonic H: 1/H, H/dBessel, 1/(2 H^2) and H - H_prev (H_prev of the
   // first harmonic is 0), the step of the power chain e^{iH psi}
   float hinv[FELIS_MAX_HARM], hdb[FELIS_MAX_HARM], hu2[FELIS_MAX_HARM];
@@ -62,6 +63,21 @@ __device__ inline double rcp_f64(double x) {
 // the Bessel table (FP32 copy) at order `order`, linear in the argument
 __device__ inline float bessel_interp_f(const float *__restrict__ tab, int bo, int idx, int order, float w0, float w1) {
   return tab[idx * bo + order] * w0 + tab[(idx + 1) * bo + order] * w1;
+}
+
+// word 3 of a raw buffer resource on gfx9-family GPUs (untyped, range checked)
+constexpr int kBufW3 = 0x00020000;
+
+// a buffer resource over `bytes` bytes from p: loads take a 32-bit lane
+// offset plus a uniform one, with no 64-bit address arithmetic per lane
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, long long bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void *)p, 0, (int)(bytes < 0x7fffffffLL ? bytes : 0x7fffffffLL), kBufW3);
+}
+__device__ inline cpx<float> buf_c(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(cpx<float>, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ inline float buf_f(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
 
 // e^{i H_h psi} for every harmonic from e^{i psi}, by multiplying along the
@@ -198,18 +214,23 @@ __device__ inline void transverse_f32(const TransArgs &t, float dz, float gam, f
 // radial points weighted by wr, summed unweighted on the axis. Zero and false
 // when idxR >= nr, where the reference keeps the Em of the particle it pushed
 // before.
+// the radial bin idxR = (int)(r/dr + 0.5) of radius^2 r2, in FP32; rr = r/dr
+__device__ inline int radial_bin_f32(float r2, float inv_dr, float &rr) {
+  rr = sqrtf(r2) * inv_dr;
+  return (int)(rr + 0.5f);
+}
+
 template <int ML>
-__device__ inline bool em_gather_f32(const GridArgs &g, double inv_dr, int L, const cpx<double> *__restrict__ Ez,
+__device__ inline bool em_gather_f32(const GridArgs &g, float inv_dr, int L, const cpx<double> *__restrict__ Ez,
                                      int slot, float r2, cpx<float> (&Em)[ML]) {
   const int F = g.F, nr = g.nr;
   const long long es = (long long)g.S * F * nr;
-  const double r = sqrt((double)r2);
-  const double rr = r * inv_dr;
-  const int idxR = (int)(rr + 0.5);
+  float rr;
+  const int idxR = radial_bin_f32(r2, inv_dr, rr);
 #pragma unroll
   for (int l = 0; l < ML; ++l) Em[l] = mk(0.f, 0.f);
   if (!(idxR >= 0 && idxR < nr)) return false;
-  const float wr1 = idxR > 0 ? __fdividef((float)(rr * rr - (double)idxR * idxR), 2.f * idxR) : 1.f;
+  const float wr1 = idxR > 0 ? __fdividef(fmaf(rr, rr, -(float)(idxR * idxR)), 2.f * idxR) : 1.f;
   const float wr0 = idxR > 0 ? 1.f - wr1 : 1.f;
 #pragma unroll
   for (int l = 0; l < ML; ++l) {
@@ -256,7 +277,7 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
   constexpr int ML = LM ? kMaxModes : 1;
   cpx<float> Em[ML];
   const int L = LM ? g.L : 1;
-  if (!em_gather_f32<ML>(g, a.inv_dr, L, Ez, slot, r2, Em)) {
+  if (!em_gather_f32<ML>(g, a.inv_drf, L, Ez, slot, r2, Em)) {
     if (emc) {
 #pragma unroll
       for (int l = 0; l < ML; ++l)
@@ -265,7 +286,9 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
       atomicOr(err, EB_EMCARRY);
     }
   }
-  const long long base = (long long)phys_slot(slot, g.ring_base, g.S) * g.N + (long long)idxJ * g.nx + idxI;
+  // this slot's field (the same for the whole tile) and the particle's cell
+  const long long sbase = (long long)phys_slot(slot, g.ring_base, g.S) * g.N;
+  const int cell = idxJ * g.nx + idxI;
   bool any_even = false;
 #pragma unroll
   for (int h = 0; h < NH; ++h) any_even |= (g.harm[h] % 2 == 0);
@@ -282,14 +305,22 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
   const cpx<float> etaX = a.etaX, etaY = a.etaY;
   cpx<float> cu[NH];
   float sumU2 = 0.f;
+  // the four corners of every harmonic's cell through buffer resources: one
+  // lane offset for all, the row step and the harmonic blocks uniform
+  const long long hblk = g.S * g.N;  // cells per harmonic block
+  const long long sbytes = g.N * (long long)sizeof(cpx<float>);
+  const int voff = cell * (int)sizeof(cpx<float>);
+  const int rowb = g.nx * (int)sizeof(cpx<float>);
+  const __amdgpu_buffer_rsrc_t rbes = buf_rsrc(bes, (long long)g.bn * g.bo * (long long)sizeof(float));
+  const int bob = g.bo * (int)sizeof(float);
 #pragma unroll
   for (int h = 0; h < NH; ++h) {
-    const cpx<float> *a0 = Ux + (long long)h * g.S * g.N + base;
-    const cpx<float> *b0 = Uy + (long long)h * g.S * g.N + base;
-    const cpx<float> ux =
-        a0[0] * (wx0 * wy0) + a0[g.nx] * (wx0 * wy1) + a0[1] * (wx1 * wy0) + a0[g.nx + 1] * (wx1 * wy1);
-    const cpx<float> uy = UY ? b0[0] * (wx0 * wy0) + b0[g.nx] * (wx0 * wy1) + b0[1] * (wx1 * wy0) +
-                                   b0[g.nx + 1] * (wx1 * wy1)
+    const __amdgpu_buffer_rsrc_t ra = buf_rsrc(Ux + h * hblk + sbase, sbytes);
+    const __amdgpu_buffer_rsrc_t rb = buf_rsrc(Uy + h * hblk + sbase, sbytes);
+    const cpx<float> ux = buf_c(ra, voff, 0) * (wx0 * wy0) + buf_c(ra, voff, rowb) * (wx0 * wy1) +
+                          buf_c(ra, voff + 8, 0) * (wx1 * wy0) + buf_c(ra, voff + 8, rowb) * (wx1 * wy1);
+    const cpx<float> uy = UY ? buf_c(rb, voff, 0) * (wx0 * wy0) + buf_c(rb, voff, rowb) * (wx0 * wy1) +
+                                   buf_c(rb, voff + 8, 0) * (wx1 * wy0) + buf_c(rb, voff + 8, rowb) * (wx1 * wy1)
                              : mk(0.f, 0.f);
     const int H = g.harm[h];
     sumU2 += (norm2(ux) + norm2(uy)) * a.hu2[h];
@@ -297,19 +328,24 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
     const int idx = (int)xb;
     const float w1 = xb - idx, w0 = 1.f - w1;
     cpx<float> fx = mk(0.f, 0.f), fy = mk(0.f, 0.f);
+    // table rows idx and idx + 1 at an order: lane offset idx * bo, uniform order
+    const int boff = idx * bob;
+    auto J = [&](int order) {
+      return buf_f(rbes, boff, order * (int)sizeof(float)) * w0 + buf_f(rbes, boff, order * (int)sizeof(float) + bob) * w1;
+    };
     if (idx < 0 || idx + 1 >= g.bn) {
       atomicOr(err, EB_BESSEL);
     } else if (H % 2 == 1) {
       const float sign = ((H - 1) / 2 % 2 == 0) ? 1.f : -1.f;
-      const float J1 = bessel_interp_f(bes, g.bo, idx, (H - 1) / 2, w0, w1);
-      const float J2 = bessel_interp_f(bes, g.bo, idx, (H + 1) / 2, w0, w1);
+      const float J1 = J((H - 1) / 2);
+      const float J2 = J((H + 1) / 2);
       fx = mk(sign * (ka * J1 - J2), 0.f);
       fy = mk(sign * (ka * J1 + J2), 0.f);
     } else {
       const float sign = (H / 2 % 2 == 0) ? 1.f : -1.f;
-      const float J1 = bessel_interp_f(bes, g.bo, idx, (H - 2) / 2, w0, w1);
-      const float J2 = bessel_interp_f(bes, g.bo, idx, H / 2, w0, w1);
-      const float J3 = bessel_interp_f(bes, g.bo, idx, (H + 2) / 2, w0, w1);
+      const float J1 = J((H - 2) / 2);
+      const float J2 = J(H / 2);
+      const float J3 = J((H + 2) / 2);
       const float s = sign * xi2 * H * 0.5f;
       const cpx<float> expM = conjg(expP);
       fx = (expP * (ka * J1 - J2) + expM * (ka * J2 - J3)) * s;
@@ -388,6 +424,12 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
   return true;
 }
 
+// lane l of each 16-lane row reads lane l - D (lanes without a source read 0)
+template <int D>
+__device__ inline float row_shr_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x110 + D, 0xF, 0xF, true));
+}
+
 // the step's two transversePush half steps at the particle's gamma
 __device__ inline void transverse_step(const GridArgs &g, const PStepArgs &a, double gm, float &xx, float &yy,
                                        float &ppx, float &ppy) {
@@ -415,15 +457,15 @@ struct EmCarryMixed {
     const double fx_ = ((double)xx - g.minX) * a.inv_dx, fy_ = ((double)yy - g.minY) * a.inv_dy;
     const int idxI = (int)fx_, idxJ = (int)fy_;
     if (!(idxI >= 0 && idxI < g.nx - 1 && idxJ >= 0 && idxJ < g.ny - 1)) return 0;
-    const float r2 = xx * xx + yy * yy;
-    const int idxR = (int)(sqrt((double)r2) * a.inv_dr + 0.5);
+    float rr;
+    const int idxR = radial_bin_f32(xx * xx + yy * yy, a.inv_drf, rr);
     return idxR >= 0 && idxR < g.nr ? 1 : 2;
   }
   __device__ int modes() const { return LM ? g.L : 1; }
   __device__ void em(int seg, const Pos &q, E *out) const {
     constexpr int ML = LM ? kMaxModes : 1;
     cpx<float> Em[ML];
-    em_gather_f32<ML>(g, a.inv_dr, modes(), Ez, seg % g.slice_n + 1, q.x * q.x + q.y * q.y, Em);
+    em_gather_f32<ML>(g, a.inv_drf, modes(), Ez, seg % g.slice_n + 1, q.x * q.x + q.y * q.y, Em);
     for (int l = 0; l < modes(); ++l) out[l] = Em[l];
   }
 };
@@ -438,7 +480,10 @@ __global__ void __launch_bounds__(kTile)
             const double *__restrict__ seg_w, const cpx<float> *__restrict__ Ux, const cpx<float> *__restrict__ Uy,
             const cpx<double> *__restrict__ Ez, const float *__restrict__ bes, const double *__restrict__ wakeE,
             const cpx<float> *__restrict__ emc, double *__restrict__ tile_red, unsigned *err) {
-  __shared__ double red[(2 * NH + 1 + 7) * kTile];
+  // per-particle moments (FP32): e^{iH theta} re, im per harmonic, the six
+  // twiss products and 1; their row sums, one set per 16-lane row
+  constexpr int NF = 2 * NH + 7;
+  __shared__ float rows[kTile / 16][NF];
   GridArgs g = g0;
   g.ring_base = ring_of(g0);
   const int t = blockIdx.x, k = threadIdx.x;
@@ -446,10 +491,9 @@ __global__ void __launch_bounds__(kTile)
   const int sl = seg % g.slice_n, slot = sl + 1;
   const bool on = k < tl.cnt[t];
   const double w = seg_w[seg];
-  constexpr int NV = 2 * NH + 1 + 7;
-  double acc[NV];
+  float acc[NF];
 #pragma unroll
-  for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+  for (int q = 0; q < NF; ++q) acc[q] = 0.f;
   if (on) {
     const long long n = tl.start[t] + k;
     const double wk = g.wake_on ? wakeE[sl] / (kMc2 * 1.0e6) : 0.0;
@@ -502,39 +546,50 @@ __global__ void __launch_bounds__(kTile)
     y[n] = yy;
     px[n] = ppx;
     py[n] = ppy;
-    // moments of the next step's bFactor and twiss, accumulated in FP64
-    // (the wrap moves theta by whole periods of e^{i theta})
+    // moments of the next step's bFactor and twiss (the wrap moves theta
+    // by whole periods of e^{i theta})
     cpx<float> eh[NH];
     harm_powers<NH>(have_eth ? eth : expi_f32(th), a.hstep, eh);
 #pragma unroll
     for (int h = 0; h < NH; ++h) {
-      acc[2 * h] = (double)eh[h].re * w;
-      acc[2 * h + 1] = (double)eh[h].im * w;
+      acc[2 * h] = eh[h].re;
+      acc[2 * h + 1] = eh[h].im;
     }
-    acc[2 * NH] = w;
-    const double inv = rcp_f64(gm);
-    const double xp = (double)ppx * inv, yp = (double)ppy * inv, xv = xx, yv = yy;
-    acc[2 * NH + 1] = xv * xv;
-    acc[2 * NH + 2] = yv * yv;
-    acc[2 * NH + 3] = xp * xp;
-    acc[2 * NH + 4] = yp * yp;
-    acc[2 * NH + 5] = xv * xp;
-    acc[2 * NH + 6] = yv * yp;
-    acc[2 * NH + 7] = 1.0;
+    const float inv = __builtin_amdgcn_rcpf((float)gm);
+    const float xp = ppx * inv, yp = ppy * inv;
+    acc[2 * NH] = xx * xx;
+    acc[2 * NH + 1] = yy * yy;
+    acc[2 * NH + 2] = xp * xp;
+    acc[2 * NH + 3] = yp * yp;
+    acc[2 * NH + 4] = xx * xp;
+    acc[2 * NH + 5] = yy * yp;
+    acc[2 * NH + 6] = 1.f;
   }
-  // tile sums in a fixed order: the moments go through LDS transposed, 16
-  // threads add 16 particles each per moment, then a 16-lane tree
+  // tile sums in a fixed order: each 16-lane row adds its particles by a
+  // DPP tree in FP32 (16 terms), the 16 row sums are added in FP64; the
+  // bunching sums take the segment weight, the tile's weight is w times its
+  // particle count
 #pragma unroll
-  for (int q = 0; q < NV; ++q) red[q * kTile + k] = acc[q];
+  for (int q = 0; q < NF; ++q) {
+    float v = acc[q];
+    v += row_shr_f<1>(v);
+    v += row_shr_f<2>(v);
+    v += row_shr_f<4>(v);
+    v += row_shr_f<8>(v);
+    if ((k & 15) == 15) rows[k >> 4][q] = v;
+  }
   __syncthreads();
-  const int part = k & 15;
-  for (int q = k >> 4; q < NV; q += kTile / 16) {
+  for (int q = k; q < NF; q += kTile) {
     double v = 0.0;
 #pragma unroll
-    for (int u = 0; u < kTile / 16; ++u) v += red[q * kTile + part + 16 * u];
-#pragma unroll
-    for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 16);
-    if (part == 0) tile_red[(long long)q * gridDim.x + t] = v;
+    for (int r = 0; r < kTile / 16; ++r) v += (double)rows[r][q];
+    // output order: bunching re, im per harmonic, weight, six twiss sums, count
+    if (q < 2 * NH) {
+      tile_red[(long long)q * gridDim.x + t] = v * w;
+    } else {
+      tile_red[(long long)(q + 1) * gridDim.x + t] = v;
+      if (q == NF - 1) tile_red[(long long)(2 * NH) * gridDim.x + t] = v * w;
+    }
   }
 }
 

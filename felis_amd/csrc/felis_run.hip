@@ -416,6 +416,7 @@ int pstep_args(felis_ctx *ctx, const felis_lattice *l0, const felis_lattice *l1,
   a.inv_dx = 1.0 / d.dx;
   a.inv_dy = 1.0 / d.dy;
   a.inv_dr = 1.0 / d.dr;
+  a.inv_drf = (float)a.inv_dr;
   for (int h = 0; h < d.n_harm; ++h) {
     const int H = d.harmony[h];
     a.hinv[h] = (float)(1.0 / H);
