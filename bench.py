@@ -106,6 +106,8 @@ def run_gpu(args, precision=None, steps=None, warmup=None, keep_init=None):
     if prec == fl.MIXED_FP32:
         from felis_amd import tune
         tuning = tune.apply(ctx, args.tuning)
+        if getattr(args, "deposit_mode", "atomic") == "fixed":
+            ctx.set_deposit_mode(fl.DEPOSIT_FIXED)
     init = None
     if (not args.no_cpu or not args.no_whole_run) if keep_init is None else keep_init:
         init = dict(particles=ctx.download_particles(), Ux=ctx.download_field(fl.UX), Uy=ctx.download_field(fl.UY))
@@ -385,6 +387,9 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the whole-run comparison of multi-period jumps (with and without the learned "
                          "correction) against the batched step")
+    ap.add_argument("--deposit-mode", default="atomic", choices=["atomic", "fixed"],
+                    help="the batched deposit's accumulation (felis_set_deposit_mode): float atomics, or 64-bit fixed "
+                         "point (bitwise reproducible)")
     ap.add_argument("--no-two-pol", action="store_true",
                     help="skip the elliptical-undulator leg (both polarisations solved) of the 2048-slice workload")
     ap.add_argument("--tuning", default=None, metavar="JSON",
@@ -428,7 +433,7 @@ def main():
         "config": {"workload": f"{args.config}: {WORKLOADS[args.config]}", "precision": args.precision,
                    "slices": d.slice_n, "particles": g["np"], "grid": f"{d.nx}x{d.ny}",
                    "harmonics": [d.harmony[i] for i in range(d.n_harm)], "polarisations_solved": g["npol"],
-                   "tuning": g["tuning"]},
+                   "tuning": g["tuning"], "deposit_mode": args.deposit_mode},
         "roofline": roofline(prof, d, g["np"], mixed, g["npol"], args.config, args.precision, g["tuning"]),
         "kernels_ms_per_step": {k: v[0] / args.prof_steps for k, v in sorted(prof.items(), key=lambda kv: -kv[1][0])},
     }
