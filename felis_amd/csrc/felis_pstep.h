@@ -50,6 +50,11 @@ struct PStepArgs {
   int hstep[FELIS_MAX_HARM];
 };
 
+// a / b and sqrt(a) in FP32 from the hardware reciprocal and square root
+// (1 ulp), where the IEEE sequences would cost a dozen instructions each
+__device__ inline float fdiv_fast(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ inline float fsqrt_fast(float a) { return __builtin_amdgcn_sqrtf(a); }
+
 // 1/x in FP64 for a positive normal x: the hardware reciprocal and two
 // Newton steps (each squares the relative error), half the instructions of
 // the IEEE division sequence
@@ -165,7 +170,7 @@ __device__ inline void transverse_f32(const TransArgs &t, float dz, float gam, f
                                       float &py) {
   const float aa = t.aa, bb = t.bb, cc = t.cc;
   const int ka = t.ka;
-  const float ig = 1.0f / gam, ig2 = ig * ig;
+  const float ig = __builtin_amdgcn_rcpf(gam), ig2 = ig * ig;
   const float dd = t.ddc * ig2 * t.ue;
   const float fx = bb * ig * t.xCoef + cc, fy = bb * ig * t.yCoef - cc;
   if (aa == 0.f && dd == 0.f) {
@@ -216,7 +221,7 @@ __device__ inline void transverse_f32(const TransArgs &t, float dz, float gam, f
 // before.
 // the radial bin idxR = (int)(r/dr + 0.5) of radius^2 r2, in FP32; rr = r/dr
 __device__ inline int radial_bin_f32(float r2, float inv_dr, float &rr) {
-  rr = sqrtf(r2) * inv_dr;
+  rr = fsqrt_fast(r2) * inv_dr;
   return (int)(rr + 0.5f);
 }
 
@@ -230,7 +235,7 @@ __device__ inline bool em_gather_f32(const GridArgs &g, float inv_dr, int L, con
 #pragma unroll
   for (int l = 0; l < ML; ++l) Em[l] = mk(0.f, 0.f);
   if (!(idxR >= 0 && idxR < nr)) return false;
-  const float wr1 = idxR > 0 ? __fdividef(fmaf(rr, rr, -(float)(idxR * idxR)), 2.f * idxR) : 1.f;
+  const float wr1 = idxR > 0 ? fdiv_fast(fmaf(rr, rr, -(float)(idxR * idxR)), 2.f * idxR) : 1.f;
   const float wr0 = idxR > 0 ? 1.f - wr1 : 1.f;
 #pragma unroll
   for (int l = 0; l < ML; ++l) {
@@ -696,7 +701,7 @@ __global__ void __launch_bounds__(1024)
       cnt_all += e - b;
       const float macro_u = fabsf((float)(seg_w[seg] * d.coef_U));
       for (long long n = b + tid; n < e; n += nt) {
-        const float a0 = __fdividef(macro_u, (float)(g.gref + (double)gamma[n]));
+        const float a0 = fdiv_fast(macro_u, (float)(g.gref + (double)gamma[n]));
         am = fmaxf(am, a0);
         if (any_even) {
           const float ppx = px[n], ppy = py[n], ue = (float)d.ue;
@@ -893,7 +898,7 @@ __global__ void __launch_bounds__(1024)
         wx[1] = (float)(fx_ - idxI); wx[0] = 1.f - wx[1];
         wy[1] = (float)(fy_ - idxJ); wy[0] = 1.f - wy[1];
         const double th = thn;
-        const float amp0 = __fdividef(macro_u, (float)(g.gref + (double)gn));
+        const float amp0 = fdiv_fast(macro_u, (float)(g.gref + (double)gn));
         float xi2 = 0.f;
         cpx<float> expP = mk(1.f, 0.f);
         if (any_even) {
@@ -991,43 +996,48 @@ __global__ void __launch_bounds__(1024)
     const int seg = (n_species - 1) * g.slice_n + sl;
     const long long b = seg_off[seg], e = seg_off[seg + 1];
     const double macro = seg_w[seg];
-    const double inv_dr = 1.0 / g.dr;
+    const float inv_drf = (float)(1.0 / g.dr);
     for (long long r0 = b + (long long)tid * kRun; r0 < e; r0 += (long long)nt * kRun) {
       const long long r1 = min(r0 + (long long)kRun, e);
       int key = -1;
-      double wr0 = 0.0, wr1 = 0.0;
-      cpx<double> sum[2] = {mk(0.0, 0.0), mk(0.0, 0.0)};  // F = 1 and F > 1 use f = 0 here
+      double amp = 0.0;
+      // a run's sums of i e^{-i theta} weighted by each particle's own radial
+      // weights (w0 into sum[0], w1 into sum[2]), and the axis sum (sum[1])
+      cpx<double> sum[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
 #define FELIS_RADIAL_FLUSH()                                                                 \
   if (key > 0) {                                                                             \
-    lds_add<DET>(mine, 2 * key, sum[0].re * wr0, rsc);                                       \
-    lds_add<DET>(mine, 2 * key + 1, sum[0].im * wr0, rsc);                                   \
-    lds_add<DET>(mine, 2 * key + 2, sum[0].re * wr1, rsc);                                   \
-    lds_add<DET>(mine, 2 * key + 3, sum[0].im * wr1, rsc);                                   \
+    lds_add<DET>(mine, 2 * key, sum[0].re * amp, rsc);                                       \
+    lds_add<DET>(mine, 2 * key + 1, sum[0].im * amp, rsc);                                   \
+    lds_add<DET>(mine, 2 * key + 2, sum[2].re * amp, rsc);                                   \
+    lds_add<DET>(mine, 2 * key + 3, sum[2].im * amp, rsc);                                   \
   } else if (key == 0) {                                                                     \
     lds_add<DET>(mine, 0, sum[1].re, rsc);                                                   \
     lds_add<DET>(mine, 1, sum[1].im, rsc);                                                   \
   }
       for (long long n = r0; n < r1; ++n) {
-        const double xx = x[n], yy = y[n];
-        const double rr = sqrt(xx * xx + yy * yy);
-        const int idxR = (int)(rr * inv_dr + 0.5);
+        const float xf = x[n], yf = y[n];
+        // the radial bin and weights in FP32 (1e-7 of a bin), as the push
+        // forms them; the sums in FP64
+        float rrf;
+        const int idxR = radial_bin_f32(xf * xf + yf * yf, inv_drf, rrf);
         const cpx<float> em = conjg(expi_f32(theta[n]));  // e^{-i theta}
         if (idxR > 0 && idxR < nr) {
+          const double w1 = (double)fdiv_fast(fmaf(rrf, rrf, -(float)(idxR * idxR)), 2.f * idxR), w0 = 1.0 - w1;
           if (F > 1 || (LM && L > 1)) {
             // azimuthal modes need the angle of every particle, longitudinal
             // modes (l+1) theta: no combining. Mode l adds
             // i (l+1) coef_Ez macro / idxR e^{-i((l+1) theta + f phi)}
             // (solveField.cpp:340-347)
-            const double amp = d.coef_Ez * macro / (1.0 * idxR);
             FELIS_RADIAL_FLUSH();
             key = -1;
+            const double ampl = d.coef_Ez * macro / (1.0 * idxR);
+            const double xx = xf, yy = yf;
             const double ph = (xx == 0.0) ? 0.0 : atan2(yy, xx);
-            const double w1 = (rr * rr * inv_dr * inv_dr - idxR * idxR) / (2.0 * idxR), w0 = 1.0 - w1;
             for (int l = 0; l < L; ++l)
               for (int f = 0; f < F; ++f) {
                 double sn, cs;
                 sincos(-((l + 1.0) * theta[n] + f * ph), &sn, &cs);
-                const double a = amp * (l + 1.0);
+                const double a = ampl * (l + 1.0);
                 const int k0 = 2 * (l * row + f * nr + idxR);
                 lds_add<DET>(mine, k0, -sn * a * w0, rsc);
                 lds_add<DET>(mine, k0 + 1, cs * a * w0, rsc);
@@ -1039,16 +1049,14 @@ __global__ void __launch_bounds__(1024)
           if (idxR != key) {
             FELIS_RADIAL_FLUSH();
             key = idxR;
-            sum[0] = mk(0.0, 0.0);
-            // the bin's amplitude coef_Ez macro / idxR, the same for the whole
-            // run, goes into the run's radial weights
-            const double amp = d.coef_Ez * macro / (1.0 * idxR);
-            const double w1 = (rr * rr * inv_dr * inv_dr - idxR * idxR) / (2.0 * idxR);
-            wr1 = amp * w1;
-            wr0 = amp * (1.0 - w1);
+            sum[0] = sum[2] = mk(0.0, 0.0);
+            // the bin's amplitude coef_Ez macro / idxR, the same for the run
+            amp = d.coef_Ez * macro / (1.0 * idxR);
           }
           // i e^{-i theta} = (sin theta, cos theta) = (-em.im, em.re)
-          sum[0] = sum[0] + mk(-(double)em.im, (double)em.re);
+          const cpx<double> ie = mk(-(double)em.im, (double)em.re);
+          sum[0] = sum[0] + ie * w0;
+          sum[2] = sum[2] + ie * w1;
         } else if (idxR == 0) {
           if (LM && L > 1) {
             // the axis row of every mode, without 1/idxR (solveField.cpp:348-356)
