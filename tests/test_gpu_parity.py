@@ -235,6 +235,27 @@ def test_run_mixed_carries_em():
     assert np.min(np.sort(em_term[carried])[-20:]) > 30 * diff
 
 
+def test_run_mixed_radial_source_takes_each_particles_weights():
+    """solveField.cpp:322-359 in the batched deposit: every particle adds to
+    bins idxR and idxR+1 with its own radial weights. Here every particle has
+    its own position (no beamlets), so consecutive particles in one bin have
+    different weights; after one period with the deposit on, Ez (solved from
+    that source) agrees with the oracle's to 1e-5 of its largest value (the
+    positions are FP32 in the batched path). Two longitudinal modes."""
+    case = fc.make_case(slice_n=4, harmonics=(1,), ppslice=512, nx=33, ny=31, in_phases=1, sc_l=2)
+    tab = [case.lattice(current_flag=1)] + [case.lattice() for _ in range(2)]
+    o = case.oracle_state()
+    rc, _, _, _ = o.run(tab, 0, 1, True)
+    assert rc == 0
+    ctx = fc.gpu_context(case, precision=fl.MIXED_FP32)
+    ctx.run([fc.to_lib_lattice(l) for l in tab], 0, 1)
+    ez = ctx.download_field(fl.EZ).reshape(o.Ez.shape)
+    sl = slice(case.desc_kw["nr"] * 1, case.desc_kw["nr"] * 5)  # slots 1..4
+    want = o.Ez[:, sl]
+    assert np.max(np.abs(want)) > 0
+    assert np.max(np.abs(ez[:, sl] - want)) < 1e-5 * np.max(np.abs(want))
+
+
 @pytest.mark.parametrize("n", [33, 151, 171], ids=["fused_adi", "split_lds_adi", "two_pass_adi"])
 def test_run_mixed_lattice_and_wake(n):
     """The batched loop through the features of the multi-undulator workload

@@ -102,6 +102,14 @@ def _run_pinning_test(lib: Path, test: str) -> subprocess.CompletedProcess:
     return subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=str(ROOT), timeout=300)
 
 
+def _build(d: Path) -> Path:
+    """oracle/Makefile's recipe: compile with ORCFLAGS, link without them."""
+    obj, lib = d / "felis_oracle.o", d / "liboracle.so"
+    subprocess.run(["gcc", *_flags(), "-c", "-o", str(obj), str(d / "felis_oracle.c")], check=True)
+    subprocess.run(["gcc", "-shared", "-fopenmp", "-o", str(lib), str(obj), "-lm"], check=True)
+    return lib
+
+
 @pytest.mark.parametrize("name", sorted(MUTATIONS))
 def test_pinning_catches_planted_mistake(name, tmp_path):
     old, new, test = MUTATIONS[name]
@@ -109,9 +117,7 @@ def test_pinning_catches_planted_mistake(name, tmp_path):
     assert old in src, f"{name}: the text to mutate is gone from felis_oracle.c"
     (tmp_path / "felis_oracle.c").write_text(src.replace(old, new, 1))
     (tmp_path / "felis_oracle.h").write_text((ORACLE / "felis_oracle.h").read_text())
-    lib = tmp_path / "liboracle.so"
-    subprocess.run(["gcc", *_flags(), "-shared", "-o", str(lib), str(tmp_path / "felis_oracle.c"), "-lm"],
-                   check=True)
+    lib = _build(tmp_path)
     r = _run_pinning_test(lib, test)
     assert r.returncode == 1, f"{name}: {test} did not fail\n{r.stdout[-2000:]}"
 
@@ -121,9 +127,7 @@ def test_unmutated_copy_passes(tmp_path):
     mutations are run against."""
     (tmp_path / "felis_oracle.c").write_text((ORACLE / "felis_oracle.c").read_text())
     (tmp_path / "felis_oracle.h").write_text((ORACLE / "felis_oracle.h").read_text())
-    lib = tmp_path / "liboracle.so"
-    subprocess.run(["gcc", *_flags(), "-shared", "-o", str(lib), str(tmp_path / "felis_oracle.c"), "-lm"],
-                   check=True)
+    lib = _build(tmp_path)
     for test in sorted({m[2] for m in MUTATIONS.values()}):
         r = _run_pinning_test(lib, test)
         assert r.returncode == 0, f"{test} fails on the unmutated oracle\n{r.stdout[-2000:]}"
