@@ -554,11 +554,12 @@ int felis_ctx::alloc_particles(const int64_t *seg_off, const double *seg_weight)
   ctx->h_seg_off.assign(seg_off, seg_off + nseg + 1);
   ctx->h_seg_w.assign(seg_weight, seg_weight + nseg);
   // tiles of <= kTile particles, never straddling a segment
-  std::vector<int> tseg, tcnt;
+  std::vector<int> tseg, tcnt, segt(nseg + 1, 0);
   std::vector<long long> tstart;
   ctx->species_tile.assign(d.n_species + 1, 0);
   for (int s = 0; s < nseg; ++s) {
     if (s % d.slice_n == 0) ctx->species_tile[s / d.slice_n] = (int)tseg.size();
+    segt[s] = (int)tseg.size();
     for (long long b = seg_off[s]; b < seg_off[s + 1]; b += kTile) {
       tseg.push_back(s);
       tstart.push_back(b);
@@ -566,6 +567,7 @@ int felis_ctx::alloc_particles(const int64_t *seg_off, const double *seg_weight)
     }
   }
   ctx->species_tile[d.n_species] = (int)tseg.size();
+  segt[nseg] = (int)tseg.size();
   ctx->ntiles = (int)tseg.size();
   std::vector<int> cseg, ccnt;
   std::vector<long long> cstart;
@@ -595,6 +597,8 @@ int felis_ctx::alloc_particles(const int64_t *seg_off, const double *seg_weight)
       (e = pal((void **)&ctx->tile_seg, tseg.size() * sizeof(int))) ||
       (e = pal((void **)&ctx->tile_start, tstart.size() * sizeof(long long))) ||
       (e = pal((void **)&ctx->tile_cnt, tcnt.size() * sizeof(int))) ||
+      (e = pal((void **)&ctx->seg_tile, segt.size() * sizeof(int))) ||
+      (e = pal((void **)&ctx->tile_bb, tseg.size() * sizeof(int4))) ||
       (e = pal((void **)&ctx->tile_red, std::max<size_t>(tseg.size(), 1) * (2 * FELIS_MAX_HARM + 1) * sizeof(double))) ||
       (e = pal((void **)&ctx->chunk_seg, cseg.size() * sizeof(int))) ||
       (e = pal((void **)&ctx->chunk_start, cstart.size() * sizeof(long long))) ||
@@ -615,6 +619,7 @@ int felis_ctx::alloc_particles(const int64_t *seg_off, const double *seg_weight)
     HIPCHK(hipMemcpy(ctx->tile_seg, tseg.data(), tseg.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->tile_start, tstart.data(), tstart.size() * sizeof(long long), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->tile_cnt, tcnt.data(), tcnt.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->seg_tile, segt.data(), segt.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->chunk_seg, cseg.data(), cseg.size() * sizeof(int), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->chunk_start, cstart.data(), cstart.size() * sizeof(long long), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->chunk_cnt, ccnt.data(), ccnt.size() * sizeof(int), hipMemcpyHostToDevice));
@@ -912,8 +917,7 @@ static int launch_deposit(felis_ctx *ctx, const DepArgs &da, double A) {
   return FELIS_OK;
 }
 
-// solveField.cpp:361-397 for every slice from the radial source in ScR,
-// which is cleared afterwards for the next deposit.
+// solveField.cpp:361-397 for every slice from the radial source in ScR.
 int felis_ctx::ez_solve(const felis_lattice *lat) {
   felis_ctx *ctx = this;
   GridArgs g = grid();

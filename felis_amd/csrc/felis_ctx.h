@@ -59,6 +59,10 @@ struct felis_ctx {
   int *tile_seg = nullptr, *tile_cnt = nullptr;
   long long *tile_start = nullptr;
   double *tile_red = nullptr;
+  // first tile of each segment (nseg + 1 entries) and, per tile, the box of
+  // in-grid cells the batched push left (min i, max i, min j, max j)
+  int *seg_tile = nullptr;
+  int4 *tile_bb = nullptr;
   // deposit chunks: <= kChunk particles of one segment per workgroup
   int nchunks = 0;
   int *chunk_seg = nullptr, *chunk_cnt = nullptr;

@@ -253,19 +253,19 @@ __device__ inline bool em_gather_f32(const GridArgs &g, float inv_dr, int L, con
 // RK stages' changes of it in FP32, gamma and theta in FP64. One sincos per particle: the RK stages rotate it by their
 // small phase increments; harmonics by complex powers. On return eth_out
 // holds e^{i theta} of the new theta (when the particle was pushed: true).
+// (fx_, fy_): the position in cells, (idxI, idxJ) its cell, in_grid: the cell
+// is inside the field grid
 template <int NH, bool UY, bool LM>
 __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const LatArgs &l, int slot, double wakeE,
                                    const cpx<float> *__restrict__ Ux, const cpx<float> *__restrict__ Uy,
                                    const cpx<double> *__restrict__ Ez, const float *__restrict__ bes, float x0,
-                                   float y0, float px, float py, double &th, double &gam, cpx<float> &eth_out,
+                                   float y0, double fx_, double fy_, int idxI, int idxJ, bool in_grid, float px,
+                                   float py, double &th, double &gam, cpx<float> &eth_out,
                                    const cpx<float> *__restrict__ emc, long long n, unsigned *err) {
   const double ku = l.ku, ks = g.ks;
   const float ka = a.kaf;
   const double dz = g.dz;
-  const double fx_ = ((double)x0 - g.minX) * a.inv_dx, fy_ = ((double)y0 - g.minY) * a.inv_dy;
-  const int idxI = (int)fx_;
-  const int idxJ = (int)fy_;
-  if (!(idxI >= 0 && idxI < g.nx - 1 && idxJ >= 0 && idxJ < g.ny - 1)) return false;
+  if (!in_grid) return false;
   const float wx1 = (float)(fx_ - idxI), wx0 = 1.f - wx1;
   const float wy1 = (float)(fy_ - idxJ), wy0 = 1.f - wy1;
   const float r2 = x0 * x0 + y0 * y0, pr2 = px * px + py * py;
@@ -434,6 +434,12 @@ template <int D>
 __device__ inline float row_shr_f(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x110 + D, 0xF, 0xF, true));
 }
+// the same shift, lanes without a source reading their own value (for
+// minima and maxima)
+template <int D>
+__device__ inline int row_shr_keep(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, 0x110 + D, 0xF, 0xF, false);
+}
 
 // the step's two transversePush half steps at the particle's gamma
 __device__ inline void transverse_step(const GridArgs &g, const PStepArgs &a, double gm, float &xx, float &yy,
@@ -478,20 +484,27 @@ struct EmCarryMixed {
 // One workgroup of kTile threads per tile, one particle per thread. UY =
 // false: Uy is exactly zero (see k_deposit_slot) and is not read. LM: more
 // than one longitudinal space-charge mode.
+// With tile_bb the tile's box of in-grid cells (min i, max i, min j, max j of
+// the pushed positions) goes to tile_bb[tile], from which k_deposit_slot takes
+// its window without reading the particles' positions for it. The tile is
+// blockIdx.x + t0 (a launch may cover a range of tiles).
 template <int NH, bool UY, bool LM>
 __global__ void __launch_bounds__(kTile)
     k_pstep(GridArgs g0, PStepArgs a, Tiles tl, double *__restrict__ theta, float *__restrict__ gamma,
             float *__restrict__ x, float *__restrict__ y, float *__restrict__ px, float *__restrict__ py,
             const double *__restrict__ seg_w, const cpx<float> *__restrict__ Ux, const cpx<float> *__restrict__ Uy,
             const cpx<double> *__restrict__ Ez, const float *__restrict__ bes, const double *__restrict__ wakeE,
-            const cpx<float> *__restrict__ emc, double *__restrict__ tile_red, unsigned *err) {
+            const cpx<float> *__restrict__ emc, double *__restrict__ tile_red, int4 *__restrict__ tile_bb, int t0,
+            unsigned *err) {
   // per-particle moments (FP32): e^{iH theta} re, im per harmonic, the six
   // twiss products and 1; their row sums, one set per 16-lane row
   constexpr int NF = 2 * NH + 7;
   __shared__ float rows[kTile / 16][NF];
+  __shared__ int rbb[kTile / 16][4];
+  __shared__ int scell[kTile];  // each particle's cell (j << 16 | i), -1 off the grid
   GridArgs g = g0;
   g.ring_base = ring_of(g0);
-  const int t = blockIdx.x, k = threadIdx.x;
+  const int t = blockIdx.x + t0, k = threadIdx.x;
   const int seg = tl.seg[t];
   const int sl = seg % g.slice_n, slot = sl + 1;
   const bool on = k < tl.cnt[t];
@@ -506,15 +519,19 @@ __global__ void __launch_bounds__(kTile)
     double gm = g.gref + (double)gamma[n];
     float xx = x[n], yy = y[n], ppx = px[n], ppy = py[n];
     transverse_step(g, a, gm, xx, yy, ppx, ppy);
+    // the pushed position's cell (the deposit's index: truncation toward zero)
+    const double fx_ = ((double)xx - g.minX) * a.inv_dx, fy_ = ((double)yy - g.minY) * a.inv_dy;
+    const int idxI = (int)fx_, idxJ = (int)fy_;
+    const bool in_grid = idxI >= 0 && idxI < g.nx - 1 && idxJ >= 0 && idxJ < g.ny - 1;
+    // kept in LDS for the tile's box, which is formed after the push
+    if (tile_bb) scell[k] = in_grid ? (idxJ << 16) | idxI : -1;
     cpx<float> eth = mk(1.f, 0.f);
     bool have_eth = false;
     if (!a.drift) {
-      have_eth =
-          push_tg_f32<NH, UY, LM>(g, a, a.l1, slot, wk, Ux, Uy, Ez, bes, xx, yy, ppx, ppy, th, gm, eth, emc, n, err);
+      have_eth = push_tg_f32<NH, UY, LM>(g, a, a.l1, slot, wk, Ux, Uy, Ez, bes, xx, yy, fx_, fy_, idxI, idxJ,
+                                         in_grid, ppx, ppy, th, gm, eth, emc, n, err);
     } else {
-      const double fx_ = ((double)xx - g.minX) * a.inv_dx, fy_ = ((double)yy - g.minY) * a.inv_dy;
-      const int idxI = (int)fx_, idxJ = (int)fy_;
-      if (idxI >= 0 && idxI < g.nx - 1 && idxJ >= 0 && idxJ < g.ny - 1) {
+      if (in_grid) {
         const float wx1 = (float)(fx_ - idxI), wx0 = 1.f - wx1;
         const float wy1 = (float)(fy_ - idxJ), wy0 = 1.f - wy1;
         const long long base =
@@ -583,7 +600,41 @@ __global__ void __launch_bounds__(kTile)
     v += row_shr_f<8>(v);
     if ((k & 15) == 15) rows[k >> 4][q] = v;
   }
+  if (tile_bb) {
+    // the cell box by the same 16-lane trees (minima and maxima); the cell is
+    // read back from LDS (volatile: not held in a register through the push)
+    int bi0 = 1 << 30, bi1 = -1, bj0 = 1 << 30, bj1 = -1;
+    const int c = on ? reinterpret_cast<volatile int *>(scell)[k] : -1;
+    if (c >= 0) {
+      bi0 = bi1 = c & 0xffff;
+      bj0 = bj1 = c >> 16;
+    }
+    bi0 = min(bi0, row_shr_keep<1>(bi0)); bi1 = max(bi1, row_shr_keep<1>(bi1));
+    bj0 = min(bj0, row_shr_keep<1>(bj0)); bj1 = max(bj1, row_shr_keep<1>(bj1));
+    bi0 = min(bi0, row_shr_keep<2>(bi0)); bi1 = max(bi1, row_shr_keep<2>(bi1));
+    bj0 = min(bj0, row_shr_keep<2>(bj0)); bj1 = max(bj1, row_shr_keep<2>(bj1));
+    bi0 = min(bi0, row_shr_keep<4>(bi0)); bi1 = max(bi1, row_shr_keep<4>(bi1));
+    bj0 = min(bj0, row_shr_keep<4>(bj0)); bj1 = max(bj1, row_shr_keep<4>(bj1));
+    bi0 = min(bi0, row_shr_keep<8>(bi0)); bi1 = max(bi1, row_shr_keep<8>(bi1));
+    bj0 = min(bj0, row_shr_keep<8>(bj0)); bj1 = max(bj1, row_shr_keep<8>(bj1));
+    if ((k & 15) == 15) {
+      rbb[k >> 4][0] = bi0;
+      rbb[k >> 4][1] = bi1;
+      rbb[k >> 4][2] = bj0;
+      rbb[k >> 4][3] = bj1;
+    }
+  }
   __syncthreads();
+  if (tile_bb && k == 0) {
+    int4 b = make_int4(rbb[0][0], rbb[0][1], rbb[0][2], rbb[0][3]);
+    for (int r = 1; r < kTile / 16; ++r) {
+      b.x = min(b.x, rbb[r][0]);
+      b.y = max(b.y, rbb[r][1]);
+      b.z = min(b.z, rbb[r][2]);
+      b.w = max(b.w, rbb[r][3]);
+    }
+    tile_bb[t] = b;
+  }
   for (int q = k; q < NF; q += kTile) {
     double v = 0.0;
 #pragma unroll
@@ -644,30 +695,42 @@ __global__ void __launch_bounds__(1024)
                    const float *__restrict__ gamma, const float *__restrict__ x, const float *__restrict__ y,
                    const float *__restrict__ px, const float *__restrict__ py, const double *__restrict__ bes,
                    cpx<float> *__restrict__ Scx, cpx<float> *__restrict__ Scy, cpx<double> *__restrict__ ScR,
-                   int4 *__restrict__ win_out, int lds_bytes, unsigned *err) {
+                   int4 *__restrict__ win_out, int lds_bytes, const int4 *__restrict__ tile_bb,
+                   const int *__restrict__ seg_tile, int sl0, unsigned *err) {
   extern __shared__ double lds_d[];
   __shared__ int bb[4];
-  const int sl = blockIdx.x, slot = sl + 1, tid = threadIdx.x, nt = blockDim.x;
+  const int sl = blockIdx.x + sl0, slot = sl + 1, tid = threadIdx.x, nt = blockDim.x;
   if (tid == 0) {
     bb[0] = 1 << 30; bb[1] = -1; bb[2] = 1 << 30; bb[3] = -1;
   }
   __syncthreads();
   int mi = 1 << 30, Mi = -1, mj = 1 << 30, Mj = -1;
-  for (int sp = 0; sp < n_species; ++sp) {
-    const int seg = sp * g.slice_n + sl;
-    const long long b = seg_off[seg], e = seg_off[seg + 1];
-    // the next iteration's loads in flight while this one is binned
-    float xq = b + tid < e ? x[b + tid] : 0.f, yq = b + tid < e ? y[b + tid] : 0.f;
-    for (long long n = b + tid; n < e; n += nt) {
-      const float xn = xq, yn = yq;
-      if (n + nt < e) {
-        xq = x[n + nt];
-        yq = y[n + nt];
+  if (tile_bb) {
+    // the box of in-grid cells from the boxes the push left per tile
+    for (int sp = 0; sp < n_species; ++sp) {
+      const int seg = sp * g.slice_n + sl;
+      for (int tt = seg_tile[seg] + tid; tt < seg_tile[seg + 1]; tt += nt) {
+        const int4 q = tile_bb[tt];
+        mi = min(mi, q.x); Mi = max(Mi, q.y); mj = min(mj, q.z); Mj = max(Mj, q.w);
       }
-      // the same cell index as the deposit below
-      const int idxI = (int)(((double)xn - g.minX) * d.inv_dx), idxJ = (int)(((double)yn - g.minY) * d.inv_dy);
-      if (idxI >= 0 && idxI < g.nx - 1 && idxJ >= 0 && idxJ < g.ny - 1) {
-        mi = min(mi, idxI); Mi = max(Mi, idxI); mj = min(mj, idxJ); Mj = max(Mj, idxJ);
+    }
+  } else {
+    for (int sp = 0; sp < n_species; ++sp) {
+      const int seg = sp * g.slice_n + sl;
+      const long long b = seg_off[seg], e = seg_off[seg + 1];
+      // the next iteration's loads in flight while this one is binned
+      float xq = b + tid < e ? x[b + tid] : 0.f, yq = b + tid < e ? y[b + tid] : 0.f;
+      for (long long n = b + tid; n < e; n += nt) {
+        const float xn = xq, yn = yq;
+        if (n + nt < e) {
+          xq = x[n + nt];
+          yq = y[n + nt];
+        }
+        // the same cell index as the deposit below
+        const int idxI = (int)(((double)xn - g.minX) * d.inv_dx), idxJ = (int)(((double)yn - g.minY) * d.inv_dy);
+        if (idxI >= 0 && idxI < g.nx - 1 && idxJ >= 0 && idxJ < g.ny - 1) {
+          mi = min(mi, idxI); Mi = max(Mi, idxI); mj = min(mj, idxJ); Mj = max(Mj, idxJ);
+        }
       }
     }
   }

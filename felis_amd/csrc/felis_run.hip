@@ -148,24 +148,23 @@ void em_carry_mixed(felis_ctx *ctx, const GridArgs &g, const PStepArgs &a) {
                      ctx->tiles(), ctx->em_last, ctx->em_src, (cpx<float> *)ctx->emc);
 }
 
+// k_pstep over tiles [t0, t1) on stream st; uy: the Uy-live form
 template <int NH>
-int launch_particles(felis_ctx *ctx, const PStepArgs &a, double *tile_red, void *Ufield) {
+int launch_particles(felis_ctx *ctx, const PStepArgs &a, double *tile_red, void *Ufield, int4 *tile_bb, int t0,
+                     int t1, bool uy, hipStream_t st) {
+  if (t1 <= t0) return FELIS_OK;
   GridArgs g = ctx->grid();
   KTimer kt(ctx, FELIS_K_PARTICLE_STEP);
   const cpx<float> *Ux = (const cpx<float> *)Ufield, *Uy = (const cpx<float> *)((char *)Ufield + ctx->pol_bytes());
   const bool lm = ctx->d.sc_l > 1;
   const bool carry = ctx->em_carry && !a.drift;
-  if (carry) {
-    if (lm) em_carry_mixed<true>(ctx, g, a);
-    else em_carry_mixed<false>(ctx, g, a);
-  }
   const cpx<float> *emc = carry ? (const cpx<float> *)ctx->emc : nullptr;
 #define PSTEP_LAUNCH(UY, LM)                                                                                    \
-  hipLaunchKernelGGL((k_pstep<NH, UY, LM>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, g, a, ctx->tiles(),     \
-                     ctx->theta, (float *)ctx->gamma, (float *)ctx->x, (float *)ctx->y, (float *)ctx->px,          \
-                     (float *)ctx->py, ctx->seg_w, Ux, Uy, (const cpx<double> *)ctx->Ez, ctx->bessel_f, ctx->wakeE, \
-                     emc, tile_red, ctx->errw)
-  if (ctx->uy_live) {
+  hipLaunchKernelGGL((k_pstep<NH, UY, LM>), dim3(t1 - t0), dim3(kTile), 0, st, g, a, ctx->tiles(), ctx->theta,     \
+                     (float *)ctx->gamma, (float *)ctx->x, (float *)ctx->y, (float *)ctx->px, (float *)ctx->py,    \
+                     ctx->seg_w, Ux, Uy, (const cpx<double> *)ctx->Ez, ctx->bessel_f, ctx->wakeE, emc, tile_red,   \
+                     tile_bb, t0, ctx->errw)
+  if (uy) {
     if (lm) PSTEP_LAUNCH(true, true);
     else PSTEP_LAUNCH(true, false);
   } else {
@@ -181,17 +180,22 @@ int launch_particles(felis_ctx *ctx, const PStepArgs &a, double *tile_red, void 
   return FELIS_OK;
 }
 
+// k_deposit_slot over slices [s0, s1) on stream st; uy: both polarisations;
+// tile_boxes: the window from the boxes the push left per tile
 template <int NH>
-int launch_deposit_slot(felis_ctx *ctx, const DepArgs &dep, int lds) {
+int launch_deposit_slot(felis_ctx *ctx, const DepArgs &dep, int lds, bool tile_boxes, int s0, int s1, bool uy,
+                        hipStream_t st) {
+  if (s1 <= s0) return FELIS_OK;
   GridArgs g = ctx->grid();
   KTimer kt(ctx, FELIS_K_DEPOSIT);
   cpx<float> *Scx = (cpx<float> *)ctx->Sc, *Scy = (cpx<float> *)((char *)ctx->Sc + ctx->pol_bytes());
   const int nt = ctx->tune.deposit_threads ? ctx->tune.deposit_threads : 1024;
 #define DEPOSIT_LAUNCH(NPOL, LM, DET)                                                                        \
-  hipLaunchKernelGGL((k_deposit_slot<NH, NPOL, LM, DET>), dim3(ctx->d.slice_n), dim3(nt), lds, ctx->stream, g,   \
-                     dep, ctx->d.n_species, ctx->seg_off, ctx->seg_w, ctx->theta, (const float *)ctx->gamma,     \
+  hipLaunchKernelGGL((k_deposit_slot<NH, NPOL, LM, DET>), dim3(s1 - s0), dim3(nt), lds, st, g, dep,              \
+                     ctx->d.n_species, ctx->seg_off, ctx->seg_w, ctx->theta, (const float *)ctx->gamma,           \
                      (const float *)ctx->x, (const float *)ctx->y, (const float *)ctx->px, (const float *)ctx->py, \
-                     ctx->bessel, Scx, Scy, ctx->ScR, ctx->win, lds, ctx->errw)
+                     ctx->bessel, Scx, Scy, ctx->ScR, ctx->win, lds, tile_boxes ? ctx->tile_bb : nullptr,          \
+                     ctx->seg_tile, s0, ctx->errw)
 #define DEPOSIT_LAUNCH_DET(NPOL, LM)                 \
   if (ctx->dep_mode == FELIS_DEPOSIT_FIXED) {        \
     DEPOSIT_LAUNCH(NPOL, LM, true);                  \
@@ -199,7 +203,7 @@ int launch_deposit_slot(felis_ctx *ctx, const DepArgs &dep, int lds) {
     DEPOSIT_LAUNCH(NPOL, LM, false);                 \
   }
   const bool lm = ctx->d.sc_l > 1;
-  if (ctx->uy_live) {
+  if (uy) {
     if (lm) {
       DEPOSIT_LAUNCH_DET(2, true);
     } else {
@@ -235,11 +239,41 @@ int launch_deposit_slot(felis_ctx *ctx, const DepArgs &dep, int lds) {
     default: return FELIS_ERR_UNSUPPORTED;       \
   }
 
-int particles(felis_ctx *ctx, const PStepArgs &a, double *tile_red, void *Ufield) {
-  NH_DISPATCH(launch_particles, ctx->d.n_harm, ctx, a, tile_red, Ufield)
+int particles_range(felis_ctx *ctx, const PStepArgs &a, double *tile_red, void *Ufield, int4 *tile_bb, int t0,
+                    int t1, bool uy, hipStream_t st) {
+  NH_DISPATCH(launch_particles, ctx->d.n_harm, ctx, a, tile_red, Ufield, tile_bb, t0, t1, uy, st)
 }
+int deposit_range(felis_ctx *ctx, const DepArgs &dep, int lds, bool tile_boxes, int s0, int s1, bool uy,
+                  hipStream_t st) {
+  NH_DISPATCH(launch_deposit_slot, ctx->d.n_harm, ctx, dep, lds, tile_boxes, s0, s1, uy, st)
+}
+// the deposit of every slice on the context's stream (the window from the
+// particles' positions)
 int deposit_slot(felis_ctx *ctx, const DepArgs &dep, int lds) {
-  NH_DISPATCH(launch_deposit_slot, ctx->d.n_harm, ctx, dep, lds)
+  return deposit_range(ctx, dep, lds, false, 0, ctx->d.slice_n, ctx->uy_live, ctx->stream);
+}
+
+// The particle step of every tile and, when dep is given, the next deposit,
+// which takes its window from the boxes the push leaves per tile. The push's
+// variant is chosen before lat_uy(ln) (the Uy state it runs under), the
+// deposit's after it. dzm: the push's step in periods (a jump's J; the
+// deposit's is one).
+int particles_and_deposit(felis_ctx *ctx, const PStepArgs &a, double *tile_red, void *Ufield, double dzm,
+                          const felis_lattice *ln, const DepArgs *dep, int lds) {
+  const felis_desc &d = ctx->d;
+  const bool uy_push = ctx->uy_live;
+  if (dep) ctx->lat_uy(ln);
+  const bool uy_dep = ctx->uy_live;
+  ctx->dz_mul = dzm;
+  if (ctx->em_carry && !a.drift) {
+    GridArgs g = ctx->grid();
+    if (d.sc_l > 1) em_carry_mixed<true>(ctx, g, a);
+    else em_carry_mixed<false>(ctx, g, a);
+  }
+  int e = particles_range(ctx, a, tile_red, Ufield, ctx->tile_bb, 0, ctx->ntiles, uy_push, ctx->stream);
+  ctx->dz_mul = 1.0;
+  if (e) return e;
+  return dep ? deposit_range(ctx, *dep, lds, true, 0, d.slice_n, uy_dep, ctx->stream) : FELIS_OK;
 }
 
 // the step's diagnostics row (energy, bFactor, twiss) from the slice
@@ -649,8 +683,7 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
         reduce_diag(ctx, 1, tile_red, species_tile, red_parts, dev_out, step_dev, (int)per);
         int ce = ctx->ez_solve(ls);
         if (!ce) ce = launch_adi_scan(ctx, adi_tab, 1, (cpx<float> *)ctx->Sc, ctx->win);
-        if (!ce) ce = particles(ctx, a, tile_red, ctx->U);
-        if (!ce) ce = deposit_slot(ctx, dep, dep_lds);
+        if (!ce) ce = particles_and_deposit(ctx, a, tile_red, ctx->U, 1.0, l1, &dep, dep_lds);
         if (!ce) {
           hipLaunchKernelGGL(k_zero_next, dim3(ncb_all, 2 * nh), dim3(256), 0, ctx->stream, (cpx<float> *)ctx->U,
                              (const int *)ring_dev, ctx->S, (long long)ctx->N, 2 * nh);
@@ -807,19 +840,15 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
     // particles: transverse x2, theta/gamma or drift, wrap, diagnostic moments
     PStepArgs a;
     if ((err = pstep_args(ctx, l0, l1, nred, &a))) break;
-    if (jump) ctx->dz_mul = J;
-    err = particles(ctx, a, tile_red, jump ? ctx->Ub : ctx->U);
-    ctx->dz_mul = 1.0;
-    if (err) break;
     const int kn = k + (jump ? J : 1);  // the next iteration
     const felis_lattice *ln = jump ? &lat_tab[2 * kn] : l1;
-    // the next solveField's deposit, with the lattice it will run under
-    if (ln->current_flag) {
-      ctx->lat_uy(ln);
-      DepArgs dep;
-      if ((err = dep_args(ctx, ln, &dep))) break;
-      if ((err = deposit_slot(ctx, dep, dep_lds))) break;
-    }
+    // the next solveField's deposit, with the lattice it will run under,
+    // after the push (pipelined with it over slice chunks when tuned so)
+    DepArgs dep{};
+    if (ln->current_flag && (err = dep_args(ctx, ln, &dep))) break;
+    if ((err = particles_and_deposit(ctx, a, tile_red, jump ? ctx->Ub : ctx->U, jump ? J : 1.0, ln,
+                                     ln->current_flag ? &dep : nullptr, dep_lds)))
+      break;
     if (jump) {
       // the learned emission correction, before the field slips
       if (jc->use_net) {
