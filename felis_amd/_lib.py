@@ -39,6 +39,9 @@ EXPORTS = [
     "felis_mlp_get_input_affine", "felis_jump_emission_residual", "felis_live_polarisations",
     "felis_set_tuning", "felis_get_tuning", "felis_set_deposit_mode", "felis_get_deposit_mode", "felis_get_desc", "felis_particle_layout", "felis_deck_save_settings",
     "felis_field_norms", "felis_field_diag", "felis_wake_function", "felis_deck_wake_function_device",
+    "felis_fno_create", "felis_fno_destroy", "felis_fno_last_error", "felis_fno_shape", "felis_fno_param_count",
+    "felis_fno_get_params", "felis_fno_set_params", "felis_fno_get_grad", "felis_fno_set_input_affine",
+    "felis_fno_forward", "felis_fno_train_step",
 ]
 # every entry point include/felis_hdf.h declares (libfelis_hdf.so)
 HDF_EXPORTS = ["felis_save_field_hdf", "felis_restore_field_hdf", "felis_save_particles_hdf",
@@ -176,6 +179,21 @@ def load() -> C.CDLL:
         L.felis_mlp_set_layer.argtypes = [C.c_void_p, C.c_int, _fp, _fp]
         L.felis_mlp_forward.argtypes = [C.c_void_p, C.c_int, _fp, _fp]
         L.felis_mlp_train_step.argtypes = [C.c_void_p, C.c_int, _fp, _fp, _fp, C.c_float, _dp]
+        L.felis_fno_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
+                                       C.POINTER(C.c_void_p)]
+        L.felis_fno_destroy.argtypes = [C.c_void_p]
+        L.felis_fno_last_error.argtypes = [C.c_void_p]
+        L.felis_fno_last_error.restype = C.c_char_p
+        L.felis_fno_shape.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.felis_fno_param_count.argtypes = [C.c_void_p]
+        L.felis_fno_param_count.restype = C.c_int64
+        L.felis_fno_get_params.argtypes = [C.c_void_p, _fp]
+        L.felis_fno_set_params.argtypes = [C.c_void_p, _fp]
+        L.felis_fno_get_grad.argtypes = [C.c_void_p, _fp]
+        L.felis_fno_set_input_affine.argtypes = [C.c_void_p, _fp, _fp]
+        L.felis_fno_forward.argtypes = [C.c_void_p, C.c_int, _fp, _fp]
+        L.felis_fno_train_step.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                           C.c_float, _dp]
         L.felis_mlp_get_grad.argtypes = [C.c_void_p, C.c_int, _fp, _fp]
         L.felis_gemm_bf16.argtypes = [C.c_int, C.c_int, C.c_int, _fp, _fp, _fp]
         L.felis_deck_beam_sigma.argtypes = [C.c_void_p, _dp]
@@ -686,3 +704,81 @@ class MLP:
         gb = np.zeros(self.dims[l + 1], np.float32)
         self._chk(self.L.felis_mlp_get_grad(self.h, l, _f32(gW)[1], _f32(gb)[1]))
         return gW, gb
+
+
+class FNO:
+    """The Fourier neural operator of felis_fno.hip on 64 x 64 channel-last
+    maps (BF16 operands on the matrix cores, FP32 accumulation and Adam)."""
+
+    def __init__(self, cin: int, cout: int, width: int = 32, modes: int = 8, layers: int = 4, proj: int = 64,
+                 capacity: int = 16, seed: int = 1):
+        self.L = load()
+        h = C.c_void_p()
+        rc = self.L.felis_fno_create(int(cin), int(cout), int(width), int(modes), int(layers), int(proj),
+                                     int(capacity), int(seed), C.byref(h))
+        if rc != OK:
+            raise FelisError(rc, "felis_fno_create")
+        self.h = h
+        self.cin, self.cout, self.width, self.modes, self.layers, self.proj = cin, cout, width, modes, layers, proj
+        self.capacity = capacity
+        self.n_params = int(self.L.felis_fno_param_count(h))
+
+    def close(self):
+        if self.h:
+            self.L.felis_fno_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != OK:
+            raise FelisError(rc, self.L.felis_fno_last_error(self.h).decode())
+
+    def params(self):
+        P = np.zeros(self.n_params, np.float32)
+        self._chk(self.L.felis_fno_get_params(self.h, _f32(P)[1]))
+        return P
+
+    def set_params(self, P):
+        P, pp = _f32(P)
+        assert P.size == self.n_params
+        self._chk(self.L.felis_fno_set_params(self.h, pp))
+
+    def grad(self):
+        G = np.zeros(self.n_params, np.float32)
+        self._chk(self.L.felis_fno_get_grad(self.h, _f32(G)[1]))
+        return G
+
+    def set_input_affine(self, mu, inv_sd):
+        mu, pm = _f32(mu)
+        inv_sd, ps = _f32(inv_sd)
+        self._chk(self.L.felis_fno_set_input_affine(self.h, pm, ps))
+
+    def forward(self, X):
+        X, px = _f32(X)
+        Y = np.zeros((X.shape[0], X.shape[1], self.cout), np.float32)
+        self._chk(self.L.felis_fno_forward(self.h, X.shape[0], px, _f32(Y)[1]))
+        return Y
+
+    def train_step(self, X, E, R, wb, lr: float) -> float:
+        """Host arrays X [B][4096][cin], E, R [B][4096][cout], wb [B]."""
+        X, px = _f32(X)
+        E, pe = _f32(E)
+        R, pr = _f32(R)
+        wb, pw = _f32(wb)
+        loss = C.c_double()
+        self._chk(self.L.felis_fno_train_step(self.h, X.shape[0], C.cast(px, C.c_void_p), C.cast(pe, C.c_void_p),
+                                              C.cast(pr, C.c_void_p), C.cast(pw, C.c_void_p), 0, float(lr),
+                                              C.byref(loss)))
+        return float(loss.value)
+
+    def train_step_dev(self, B: int, X, E, R, wb, lr: float) -> float:
+        """Device pointers (ints) X, E, R, wb as in train_step."""
+        loss = C.c_double()
+        self._chk(self.L.felis_fno_train_step(self.h, int(B), C.c_void_p(X), C.c_void_p(E), C.c_void_p(R),
+                                              C.c_void_p(wb), 1, float(lr), C.byref(loss)))
+        return float(loss.value)

@@ -15,7 +15,7 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libfelis_hip.so"
 
 HIP_SOURCES = ["felis_api.hip", "felis_run.hip", "felis_load.hip", "felis_deck.cpp", "felis_mlp.hip", "felis_sur.hip",
-               "felis_diag.hip", "felis_wake.hip"]
+               "felis_diag.hip", "felis_wake.hip", "felis_fno.hip"]
 
 # -packed-fp32-ops off: packed FP32 forms are kept out of generated code.
 HIPFLAGS = [

@@ -352,6 +352,40 @@ int felis_mlp_get_grad(felis_mlp *net, int layer, float *gW, float *gb);
 /* C[M][N] = A[M][K] B[K][N] with BF16-rounded operands on the matrix cores */
 int felis_gemm_bf16(int M, int N, int K, const float *A, const float *B, float *C);
 
+/* -------- learned surrogate: a Fourier neural operator on 64 x 64 maps (no
+ * counterpart in the reference; the field-to-field network of the jumps) ----
+ * Channel-last maps [map][y][x][channel], 64 x 64 positions. lift (cin ->
+ * width), `layers` Fourier layers (a = h W + b + spectral part over `modes`
+ * kept x and y modes, GELU after all but the last), projection (width ->
+ * proj -> cout, GELU between). BF16 operands on the matrix cores, FP32
+ * accumulation and storage, FP32 Adam. width and modes powers of two. */
+typedef struct felis_fno felis_fno;
+int felis_fno_create(int cin, int cout, int width, int modes, int layers, int proj, int capacity, uint64_t seed,
+                     felis_fno **out);
+int felis_fno_destroy(felis_fno *net);
+const char *felis_fno_last_error(const felis_fno *net);
+/* dims[7] = cin, cout, width, modes, layers, proj, capacity */
+int felis_fno_shape(const felis_fno *net, int *dims);
+int64_t felis_fno_param_count(const felis_fno *net);
+/* all parameters in one vector: lift W [cin][w], b [w]; per layer W [w][w],
+ * b [w], mode weights [2 modes][modes][2w][2w] (ky in 0..m-1 then -m..-1, kx
+ * in 0..m-1; rows (re, im) x channel in, columns out); Q1 [w][proj], c1,
+ * Q2 [proj][cout], c2 */
+int felis_fno_get_params(felis_fno *net, float *P);
+int felis_fno_set_params(felis_fno *net, const float *P);
+int felis_fno_get_grad(felis_fno *net, float *G);
+/* (x - mu) * inv_sd per input channel, applied before the lift */
+int felis_fno_set_input_affine(felis_fno *net, const float *mu, const float *inv_sd);
+/* X [maps][4096][cin] -> Y [maps][4096][cout], host buffers */
+int felis_fno_forward(felis_fno *net, int maps, const float *X, float *Y);
+/* one step on `maps` maps (dev != 0: device pointers): X [maps][4096][cin];
+ * E, R [maps][4096][cout] as cout/2 complex channels; wb [maps] weights.
+ * Loss sum_b wb sum_{p,h} |o_h E_h - R_h|^2 / (maps 4096) of the output o
+ * (cout/2 complex factors), its gradients (felis_fno_get_grad) and, for lr >
+ * 0, an Adam step. */
+int felis_fno_train_step(felis_fno *net, int maps, const float *X, const float *E, const float *R, const float *wb,
+                         int dev, float lr, double *loss);
+
 #ifdef __cplusplus
 }
 #endif
