@@ -298,6 +298,7 @@ def surrogate_runs(args, periods: int = 10, gate: float = 0.05, wide_gate: float
     dk.close()
     model = su.load()
     net = model.network(capacity=slices)
+    fno_model = su.load_fno()
 
     def filled(r):
         ok = ~np.isnan(r["E"])
@@ -325,7 +326,7 @@ def surrogate_runs(args, periods: int = 10, gate: float = 0.05, wide_gate: float
     rn = su.run_curves(deck, seed=args.seed, periods=periods, gate=gate, net=net, net_gate=gate)
     out = {"periods_per_jump": periods, "bunching_gate": gate, "model": "felis_amd/models/jump_corr_J10.npz",
            "network": "per-slice 20-64-64-6 perceptron (BF16 MFMA, FP32 accumulate) scaling the jump's emission; "
-                      "not the field-to-field FNO / U-Net of the surrogate's specification",
+                      "the specified field-to-field FNO is the to_saturation leg's jump_field_to_field",
            "whole_run": {"periods": int(rf["steps"]), "batched_seconds": rf["seconds"], "jump_learned": versus(rf, rn)},
            "model_meta": model.meta}
     if args.config == "cfg2":
@@ -341,6 +342,9 @@ def surrogate_runs(args, periods: int = 10, gate: float = 0.05, wide_gate: float
             sj = su.run_curves(p40, seed=args.seed, periods=periods, gate=gate)
             sn = su.run_curves(p40, seed=args.seed, periods=periods, gate=gate, net=net, net_gate=gate)
             sw = su.run_curves(p40, seed=args.seed, periods=periods, gate=wide_gate, net=net, net_gate=gate)
+            fno = fno_model.network(capacity=slices)
+            sfno = su.run_curves(p40, seed=args.seed, periods=periods, gate=gate, fno=fno, net_gate=gate)
+            fno.close()
         z = np.arange(sf["steps"], dtype=float)
         zs, ps, found = su.saturation(z, sf["E"])
         learned = versus(sf, sn, z, zs, ps)
@@ -349,6 +353,11 @@ def surrogate_runs(args, periods: int = 10, gate: float = 0.05, wide_gate: float
             "z_sat_m": float(zs * dzs), "batched_seconds": sf["seconds"],
             "speedup_of_batched": learned["speedup_of_batched"], "target_speedup": 3.0,
             "jump": versus(sf, sj, z, zs, ps), "jump_learned": learned,
+            "jump_field_to_field": dict(versus(sf, sfno, z, zs, ps), model="felis_amd/models/jump_fno_J10.npz",
+                                        network="4-layer Fourier neural operator, width 32, 8 modes per axis, on "
+                                                "64x64 maps of each slice's emission and field plus 8 moments "
+                                                "(BF16 MFMA, FP32 accumulate): factor maps on the emission",
+                                        model_meta=fno_model.meta),
             "wider_gate": dict(versus(sf, sw, z, zs, ps), bunching_gate=wide_gate)}
     net.close()
     return out

@@ -41,7 +41,7 @@ EXPORTS = [
     "felis_field_norms", "felis_field_diag", "felis_wake_function", "felis_deck_wake_function_device",
     "felis_fno_create", "felis_fno_destroy", "felis_fno_last_error", "felis_fno_shape", "felis_fno_param_count",
     "felis_fno_get_params", "felis_fno_set_params", "felis_fno_get_grad", "felis_fno_set_input_affine",
-    "felis_fno_forward", "felis_fno_train_step",
+    "felis_fno_forward", "felis_fno_train_step", "felis_fno_jump_maps",
 ]
 # every entry point include/felis_hdf.h declares (libfelis_hdf.so)
 HDF_EXPORTS = ["felis_save_field_hdf", "felis_restore_field_hdf", "felis_save_particles_hdf",
@@ -97,7 +97,7 @@ class Tuning(C.Structure):
 class JumpOpts(C.Structure):
     _fields_ = [("periods", C.c_int), ("max_jumps", C.c_int), ("bunching_max", C.c_double),
                 ("spread_max", C.c_double), ("net", C.c_void_p), ("net_bunching_max", C.c_double),
-                ("pad", C.c_int * 2)]
+                ("fno", C.c_void_p)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -194,6 +194,7 @@ def load() -> C.CDLL:
         L.felis_fno_forward.argtypes = [C.c_void_p, C.c_int, _fp, _fp]
         L.felis_fno_train_step.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                            C.c_float, _dp]
+        L.felis_fno_jump_maps.argtypes = [C.c_void_p, C.c_int, _fp, C.c_int, C.POINTER(C.c_int), _fp, _fp, _fp, _fp]
         L.felis_mlp_get_grad.argtypes = [C.c_void_p, C.c_int, _fp, _fp]
         L.felis_gemm_bf16.argtypes = [C.c_int, C.c_int, C.c_int, _fp, _fp, _fp]
         L.felis_deck_beam_sigma.argtypes = [C.c_void_p, _dp]
@@ -378,11 +379,12 @@ class Context:
 
     def run_surrogate(self, lat_tab, it0: int, n_steps: int, periods: int, bunching_max: float = 0.05,
                       max_jumps: int = -1, wake_step: int = 0, wakeF=None, net=None, spread_max: float = 0.0,
-                      net_bunching_max: float = 0.0):
+                      net_bunching_max: float = 0.0, fno=None):
         """felis_run with multi-period jumps while |b1| < bunching_max (and
         the rms relative energy spread < spread_max when that is > 0); net is
         the learned emission correction (an MLP) or None, applied while
-        |b1| < net_bunching_max (<= 0: always).
+        |b1| < net_bunching_max (<= 0: always); fno (an FNO) the
+        field-to-field correction in its place.
         Returns energy, bfac, twiss (NaN rows inside jumps) and the step kinds."""
         d = self.desc
         arr = (Lattice * len(lat_tab))(*lat_tab)
@@ -391,7 +393,7 @@ class Context:
         t = np.zeros((n_steps, 7 * d.n_species))
         kind = np.zeros(max(1, n_steps), np.int32)
         o = JumpOpts(int(periods), int(max_jumps), float(bunching_max), float(spread_max),
-                     net.h if net is not None else None, float(net_bunching_max), (C.c_int * 2)(0, 0))
+                     net.h if net is not None else None, float(net_bunching_max), fno.h if fno is not None else None)
         wf = None if wakeF is None else np.ascontiguousarray(wakeF, np.float64)
         self._chk(self.L.felis_run_surrogate(self.h, arr, it0, n_steps, wake_step, _p(wf), C.byref(o), _p(e), _p(b),
                                              _p(t), kind.ctypes.data_as(C.POINTER(C.c_int))))
@@ -424,6 +426,24 @@ class Context:
         p = np.zeros((d.slice_n, 12))
         self._chk(self.L.felis_jump_residual(self.h, slot, _p(f), _p(p)))
         return f, p
+
+    def fno_jump_maps(self, slot: int, features, rows):
+        """After a jump (state a) against snapshot `slot` (b): for the slices
+        in rows, the field-to-field network's input maps X [n][4096][cin],
+        the emission E and residual R [n][4096][2 n_harm] (in units of the
+        emission's rms per harmonic) and info [n][3][n_harm] (felis_fno_jump_maps)."""
+        d = self.desc
+        nh = d.n_harm
+        rows = np.ascontiguousarray(rows, np.int32)
+        n = len(rows)
+        X = np.zeros((n, 4096, 4 * nh + 12), np.float32)
+        E = np.zeros((n, 4096, 2 * nh), np.float32)
+        R = np.zeros((n, 4096, 2 * nh), np.float32)
+        info = np.zeros((n, 3, nh), np.float32)
+        f, pf = _f32(features)
+        self._chk(self.L.felis_fno_jump_maps(self.h, slot, pf, n, rows.ctypes.data_as(C.POINTER(C.c_int)),
+                                             _f32(X)[1], _f32(E)[1], _f32(R)[1], _f32(info)[1]))
+        return X, E, R, info
 
     def jump_emission_residual(self, slot: int):
         """After a jump (state a) against snapshot `slot` (b): per (pol x

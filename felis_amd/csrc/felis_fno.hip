@@ -29,6 +29,7 @@
 #include "felis.h"
 #include "felis_ctx.h"
 #include "felis_fno.h"
+#include "felis_sur.h"
 
 namespace {
 
@@ -38,13 +39,9 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 constexpr int kN = 64;        // map side
 constexpr int kP = kN * kN;   // positions per map
 
-// FP32 -> BF16 bits, round to nearest even (NaN kept quiet)
-__device__ inline short f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (short)((u >> 16) | 0x40);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (short)(u >> 16);
-}
+// FP32 -> BF16 bits, round to nearest even: the hardware conversion
+// (v_cvt_pk_bf16_f32 on gfx950)
+__device__ inline short f2bf(float f) { return __builtin_bit_cast(short, (__bf16)f); }
 
 // offset of index i: (i >> sh) * hi + (i & (2^sh - 1)) * lo, or i * lo for sh >= 31
 struct Ix {
@@ -69,6 +66,14 @@ struct OpW {
 // entry (2 x 2 blocks of 16 x 16), four wavefronts per workgroup. In a 16 x
 // 16 x 16 product lane l holds A[l%16][4(l/16) .. +3], B[4(l/16) .. +3][l%16]
 // and returns D[4(l/16) .. +3][l%16].
+// operand element as BF16 bits: FP32 storage is rounded, BF16 storage (the
+// fused inference path's intermediates, already rounded) is taken as it is
+__device__ inline short ld_bf(const float *p, long long o) { return f2bf(p[o]); }
+__device__ inline short ld_bf(const uint16_t *p, long long o) { return (short)p[o]; }
+__device__ inline void st_out(float *p, float v) { *p = v; }
+__device__ inline void st_out(uint16_t *p, float v) { *p = (uint16_t)f2bf(v); }
+
+template <typename TA, typename TB, typename TC>
 __global__ void __launch_bounds__(256) k_bgemm(int batch, int M, int N, int K, Op A, Op B, OpW C, int acc,
                                                const float *__restrict__ bias) {
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -78,7 +83,8 @@ __global__ void __launch_bounds__(256) k_bgemm(int batch, int M, int N, int K, O
   const long long bi = tile / per;
   const int t = (int)(tile % per), m0 = (t / tn_n) * 32, n0 = (t % tn_n) * 32;
   const int r = l & 15, kq = (l >> 4) * 4;
-  const float *pa = A.p + ix(A.b, bi), *pb = B.p + ix(B.b, bi);
+  const TA *pa = reinterpret_cast<const TA *>(A.p) + ix(A.b, bi);
+  const TB *pb = reinterpret_cast<const TB *>(B.p) + ix(B.b, bi);
   long long ra[2], cb[2];
   bool va[2], vb[2];
 #pragma unroll
@@ -99,8 +105,8 @@ __global__ void __launch_bounds__(256) k_bgemm(int batch, int M, int N, int K, O
       const long long ka = vk ? ix(A.c, k) : 0, kb = vk ? ix(B.r, k) : 0;
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
-        a[u][q] = (vk && va[u]) ? f2bf(pa[ra[u] + ka]) : (short)0;
-        b[u][q] = (vk && vb[u]) ? f2bf(pb[kb + cb[u]]) : (short)0;
+        a[u][q] = (vk && va[u]) ? ld_bf(pa, ra[u] + ka) : (short)0;
+        b[u][q] = (vk && vb[u]) ? ld_bf(pb, kb + cb[u]) : (short)0;
       }
     }
 #pragma unroll
@@ -108,7 +114,7 @@ __global__ void __launch_bounds__(256) k_bgemm(int batch, int M, int N, int K, O
 #pragma unroll
       for (int j = 0; j < 2; ++j) d[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], d[i][j], 0, 0, 0);
   }
-  float *pc = C.p + ix(C.b, bi);
+  TC *pc = reinterpret_cast<TC *>(C.p) + ix(C.b, bi);
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -117,22 +123,25 @@ __global__ void __launch_bounds__(256) k_bgemm(int batch, int M, int N, int K, O
       for (int q = 0; q < 4; ++q) {
         const int m = m0 + 16 * i + kq + q, n = n0 + 16 * j + r;
         if (m < M && n < N) {
-          float *o = pc + ix(C.r, m) + ix(C.c, n);
+          TC *o = pc + ix(C.r, m) + ix(C.c, n);
           float v = d[i][j][q];
           if (bias) v += bias[n];
-          if (acc) v += *o;
-          *o = v;
+          if (acc) v += (float)*reinterpret_cast<float *>(o);  // FP32 storage only
+          st_out(o, v);
         }
       }
 }
 
-__device__ inline float gelu(float z) {  // tanh form
+// GELU in its tanh form, with tanh(u) = 1 - 2 / (1 + e^{2u}) from the
+// hardware exponential and reciprocal (saturating to +-1 for large |u|)
+__device__ inline float tanh_fast(float u) { return 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + __expf(2.f * u)); }
+__device__ inline float gelu(float z) {
   const float c = 0.7978845608f;
-  return 0.5f * z * (1.f + tanhf(c * (z + 0.044715f * z * z * z)));
+  return 0.5f * z * (1.f + tanh_fast(c * (z + 0.044715f * z * z * z)));
 }
 __device__ inline float gelu_d(float z) {
   const float c = 0.7978845608f;
-  const float u = c * (z + 0.044715f * z * z * z), t = tanhf(u);
+  const float u = c * (z + 0.044715f * z * z * z), t = tanh_fast(u);
   return 0.5f * (1.f + t) + 0.5f * z * (1.f - t * t) * c * (1.f + 3.f * 0.044715f * z * z);
 }
 
@@ -225,6 +234,168 @@ __global__ void k_adam(long long n, float *__restrict__ w, const float *__restri
 
 inline unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
 
+// The fused forward of inference, one wavefront per map row (64 positions):
+// out_row (64 x W) = in_row (64 x kin) Wt (kin x W) + bias [+ Fxi (64 x M2)
+// Xb_row (M2 x W)], GELU when act, stored; with Xf_next also Xf_next_row (M2
+// x W) = Fx (M2 x 64) out_row, whose B operand is the product's own output
+// (a 16 x 16 block's output layout, D[4(l/16) + q][l%16], is the B operand
+// layout of the K step over those 16 positions). in_row may take the input
+// affine (mu, isd) on load. The same products and BF16 rounding points as
+// the batched-product path of training.
+// Intermediates (the input when TI is uint16_t, Xb, out, Xf_next) are BF16
+// bits: each is only ever a product operand, so storing it rounded equals
+// rounding it where it is loaded.
+template <int W, int M2, typename TI>
+__global__ void __launch_bounds__(256) k_fno_row(long long rows, int kin, const TI *__restrict__ in,
+                                                 const float *__restrict__ mu, const float *__restrict__ isd,
+                                                 const float *__restrict__ Wt, const float *__restrict__ bias,
+                                                 const float *__restrict__ Fxi, const uint16_t *__restrict__ Xb,
+                                                 const float *__restrict__ Fx, int act, uint16_t *__restrict__ out,
+                                                 uint16_t *__restrict__ Xf_next) {
+  static_assert(W % 16 == 0 && M2 == 16, "blocks of 16");
+  constexpr int NB = W / 16;
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const long long rr = (long long)blockIdx.x * 4 + wave;
+  if (rr >= rows) return;
+  const int r = l & 15, kq = (l >> 4) * 4;
+  v4f d[4][NB] = {};
+  const TI *inr = in + rr * kN * kin;
+  for (int k0 = 0; k0 < kin; k0 += 16) {
+    v4s a[4], b[NB];
+    if constexpr (sizeof(TI) == 2) {
+      // four consecutive BF16 channels of one position: one 8-byte load
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        a[i] = k0 + kq < kin ? *reinterpret_cast<const v4s *>(inr + (16 * i + r) * kin + k0 + kq) : v4s{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = k0 + kq + q;
+      const bool vk = k < kin;
+      if constexpr (sizeof(TI) == 4) {
+        const float m0 = (vk && mu) ? mu[k] : 0.f, s0 = (vk && isd) ? isd[k] : 1.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          a[i][q] = vk ? f2bf(((float)inr[(16 * i + r) * kin + k] - m0) * s0) : (short)0;
+      }
+#pragma unroll
+      for (int j = 0; j < NB; ++j) b[j][q] = vk ? f2bf(Wt[k * W + 16 * j + r]) : (short)0;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) d[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], d[i][j], 0, 0, 0);
+  }
+  if (Xb) {
+    v4s a[4], b[NB];
+    const uint16_t *xb = Xb + rr * M2 * W;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = kq + q;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i][q] = f2bf(Fxi[(16 * i + r) * M2 + k]);
+#pragma unroll
+      for (int j = 0; j < NB; ++j) b[j][q] = (short)xb[k * W + 16 * j + r];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) d[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], d[i][j], 0, 0, 0);
+  }
+  uint16_t *o = out + rr * kN * W;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const int c = 16 * j + r;
+      const float bc = bias[c];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        float v = d[i][j][q] + bc;
+        if (act) v = gelu(v);
+        d[i][j][q] = v;
+        o[(16 * i + kq + q) * W + c] = (uint16_t)f2bf(v);
+      }
+    }
+  if (Xf_next) {
+    v4f e[NB] = {};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      v4s a, b[NB];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        a[q] = f2bf(Fx[r * kN + 16 * i + kq + q]);
+#pragma unroll
+        for (int j = 0; j < NB; ++j) b[j][q] = f2bf(d[i][j][q]);
+      }
+#pragma unroll
+      for (int j = 0; j < NB; ++j) e[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[j], e[j], 0, 0, 0);
+    }
+    uint16_t *xf = Xf_next + rr * M2 * W;
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) xf[(kq + q) * W + 16 * j + r] = (uint16_t)f2bf(e[j][q]);
+  }
+}
+
+// The projection of one map row, transposed so that the first product's
+// output is the second's B operand: zq^T (PH x 64) = Q1^T h^T + c1, q =
+// gelu(zq), out^T (CO x 64) = Q2^T q^T + c2 (CO <= 16).
+template <int W, int PH>
+__global__ void __launch_bounds__(256) k_fno_proj_row(long long rows, int co, const uint16_t *__restrict__ h,
+                                                      const float *__restrict__ Q1, const float *__restrict__ c1,
+                                                      const float *__restrict__ Q2, const float *__restrict__ c2,
+                                                      float *__restrict__ out) {
+  static_assert(W % 16 == 0 && PH % 16 == 0, "blocks of 16");
+  constexpr int MB = PH / 16;
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const long long rr = (long long)blockIdx.x * 4 + wave;
+  if (rr >= rows) return;
+  const int r = l & 15, kq = (l >> 4) * 4;
+  const uint16_t *hr = h + rr * kN * W;
+  v4f z[MB][4] = {};
+  for (int k0 = 0; k0 < W; k0 += 16) {
+    v4s a[MB], b[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) b[n] = *reinterpret_cast<const v4s *>(hr + (16 * n + r) * W + k0 + kq);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = k0 + kq + q;
+#pragma unroll
+      for (int m = 0; m < MB; ++m) a[m][q] = f2bf(Q1[k * PH + 16 * m + r]);
+    }
+#pragma unroll
+    for (int m = 0; m < MB; ++m)
+#pragma unroll
+      for (int n = 0; n < 4; ++n) z[m][n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], b[n], z[m][n], 0, 0, 0);
+  }
+  v4f y[4] = {};
+#pragma unroll
+  for (int m = 0; m < MB; ++m) {
+    v4s a, b[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int ph = 16 * m + kq + q;
+      a[q] = r < co ? f2bf(Q2[(16 * m + kq + q) * co + r]) : (short)0;
+      const float cb = c1[ph];
+#pragma unroll
+      for (int n = 0; n < 4; ++n) b[n][q] = f2bf(gelu(z[m][n][q] + cb));
+    }
+#pragma unroll
+    for (int n = 0; n < 4; ++n) y[n] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[n], y[n], 0, 0, 0);
+  }
+  float *o = out + rr * kN * co;
+#pragma unroll
+  for (int n = 0; n < 4; ++n)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = kq + q;
+      if (c < co) o[(16 * n + r) * co + c] = y[n][q] + c2[c];
+    }
+}
+
+
 Ix U1(long long s) { return Ix{31, 0, s}; }
 Ix U2(int sh, long long hi, long long lo) { return Ix{sh, hi, lo}; }
 int lg2(int v) {
@@ -233,10 +404,14 @@ int lg2(int v) {
   return s;
 }
 
+// the batched product; TA, TB, TC the operands' and result's storage
+// (float, or uint16_t BF16 bits; accumulation into FP32 storage only)
+template <typename TA = float, typename TB = float, typename TC = float>
 void bgemm(hipStream_t st, int batch, int M, int N, int K, Op A, Op B, OpW C, int acc, const float *bias) {
   const long long tiles = (long long)batch * ((M + 31) / 32) * ((N + 31) / 32);
   if (tiles <= 0) return;
-  hipLaunchKernelGGL(k_bgemm, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, st, batch, M, N, K, A, B, C, acc, bias);
+  hipLaunchKernelGGL((k_bgemm<TA, TB, TC>), dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, st, batch, M, N, K, A,
+                     B, C, acc, bias);
 }
 
 }  // namespace
@@ -421,6 +596,46 @@ void fno_backward(felis_fno *n, int B) {
   hipLaunchKernelGGL(k_colsum, dim3(w), dim3(256), 0, st, rows, w, n->dh, G + n->o_lb);
 }
 
+// the fused inference forward (k_fno_row, k_fno_proj_row) for the shape it
+// is built for: width 32, 8 modes, projection 64, at most 16 outputs
+bool fno_fusable(const felis_fno *n) { return n->w == 32 && n->m == 8 && n->ph == 64 && n->cout <= 16; }
+
+void fno_forward_fused(felis_fno *n, int B, const float *X) {
+  const hipStream_t st = n->stream;
+  constexpr int w = 32, m = 8;
+  const int lw = lg2(w);
+  const long long rows = (long long)B * kN;
+  const long long sX = 2LL * m * w, sY = 2LL * m * m * 2 * w;
+  const unsigned nb = (unsigned)((rows + 3) / 4);
+  // BF16 intermediates in the training path's FP32 buffers
+  uint16_t *hA = reinterpret_cast<uint16_t *>(n->h[0]), *hB = reinterpret_cast<uint16_t *>(n->a[0]);
+  uint16_t *Xf = reinterpret_cast<uint16_t *>(n->Xf), *Yb = reinterpret_cast<uint16_t *>(n->Y[0]),
+           *Zb = reinterpret_cast<uint16_t *>(n->Zs), *Xb = reinterpret_cast<uint16_t *>(n->Xb);
+  hipLaunchKernelGGL((k_fno_row<32, 16, float>), dim3(nb), dim3(256), 0, st, rows, n->cin, X, n->mu, n->isd,
+                     n->P + n->o_lw, n->P + n->o_lb, nullptr, nullptr, n->Fx, 0, hA, Xf);
+  for (int l = 0; l < n->L; ++l) {
+    bgemm<float, uint16_t, uint16_t>(st, B, 4 * m, m * w, 2 * kN, Op{n->Gy, U1(0), U1(2 * kN), U1(1)},
+                                     Op{(const float *)Xf, U1((long long)kN * sX), U1((long long)m * w), U1(1)},
+                                     OpW{(float *)Yb, U1(sY), U2(1, (long long)m * 2 * w, w), U2(lw, 2LL * w, 1)}, 0,
+                                     nullptr);
+    bgemm<uint16_t, float, uint16_t>(st, 2 * m * m, B, 2 * w, 2 * w,
+                                     Op{(const float *)Yb, U1(2LL * w), U1(sY), U1(1)},
+                                     Op{n->P + n->o_Wm[l], U1(4LL * w * w), U1(2LL * w), U1(1)},
+                                     OpW{(float *)Zb, U1(2LL * w), U1(sY), U1(1)}, 0, nullptr);
+    bgemm<float, uint16_t, uint16_t>(st, B, 2 * kN, m * w, 4 * m, Op{n->Giy, U1(0), U1(4 * m), U1(1)},
+                                     Op{(const float *)Zb, U1(sY), U2(1, (long long)m * 2 * w, w), U2(lw, 2LL * w, 1)},
+                                     OpW{(float *)Xb, U1((long long)kN * sX), U1((long long)m * w), U1(1)}, 0,
+                                     nullptr);
+    const bool last = l == n->L - 1;
+    hipLaunchKernelGGL((k_fno_row<32, 16, uint16_t>), dim3(nb), dim3(256), 0, st, rows, w, hA, nullptr, nullptr,
+                       n->P + n->o_W[l], n->P + n->o_b[l], n->Fxi, Xb, n->Fx, last ? 0 : 1, hB,
+                       last ? nullptr : Xf);
+    std::swap(hA, hB);
+  }
+  hipLaunchKernelGGL((k_fno_proj_row<32, 64>), dim3(nb), dim3(256), 0, st, rows, n->cout, hA, n->P + n->o_q1,
+                     n->P + n->o_c1, n->P + n->o_q2, n->P + n->o_c2, n->out);
+}
+
 int fno_input(felis_fno *n, int B, const float *X) {
   const long long cnt = (long long)B * kP * n->cin;
   hipLaunchKernelGGL(k_affine, dim3(nblk(cnt)), dim3(256), 0, n->stream, cnt, n->cin, X, n->mu, n->isd, n->x);
@@ -438,8 +653,13 @@ int felis_fno_forward_dev(felis_fno *n, int B, const float *X, const float **out
   // the network's stream follows the caller's and the caller's waits for it
   (void)hipEventRecord(ev, st);
   (void)hipStreamWaitEvent(n->stream, ev, 0);
-  int e = fno_input(n, B, X);
-  if (!e) fno_forward(n, B);
+  int e = FELIS_OK;
+  if (fno_fusable(n)) {
+    fno_forward_fused(n, B, X);
+  } else {
+    e = fno_input(n, B, X);
+    if (!e) fno_forward(n, B);
+  }
   (void)hipEventRecord(ev, n->stream);
   (void)hipStreamWaitEvent(st, ev, 0);
   (void)hipEventDestroy(ev);
@@ -628,7 +848,8 @@ int felis_fno_set_input_affine(felis_fno *n, const float *mu, const float *isd) 
   return FELIS_OK;
 }
 
-// X [B][P][cin] host, Y [B][P][cout] host
+// X [B][P][cin] host, Y [B][P][cout] host; the fused forward where the
+// shape allows it (the one the jumps use), else the training path's
 int felis_fno_forward(felis_fno *n, int B, const float *X, float *Y) {
   if (!n || !X || !Y || B < 1 || B > n->cap) return FELIS_ERR_ARG;
   const size_t nx = (size_t)B * kP * n->cin, ny = (size_t)B * kP * n->cout;
@@ -636,9 +857,10 @@ int felis_fno_forward(felis_fno *n, int B, const float *X, float *Y) {
   if (hipMalloc((void **)&dX, nx * sizeof(float)) != hipSuccess) return FELIS_ERR_HIP;
   int e = hipMemcpyAsync(dX, X, nx * sizeof(float), hipMemcpyHostToDevice, n->stream) == hipSuccess ? FELIS_OK
                                                                                                     : FELIS_ERR_HIP;
-  if (!e) e = fno_input(n, B, dX);
+  if (!e && !fno_fusable(n)) e = fno_input(n, B, dX);
   if (!e) {
-    fno_forward(n, B);
+    if (fno_fusable(n)) fno_forward_fused(n, B, dX);
+    else fno_forward(n, B);
     if (hipMemcpyAsync(Y, n->out, ny * sizeof(float), hipMemcpyDeviceToHost, n->stream) != hipSuccess ||
         hipStreamSynchronize(n->stream) != hipSuccess)
       e = FELIS_ERR_HIP;
@@ -698,3 +920,299 @@ int felis_fno_train_step(felis_fno *n, int B, const float *X, const float *E, co
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------
+// The network's maps of a jump's state. The field grid (nx = ny) is cut into
+// 64 x 64 cells of b x b field cells (b = nx / 64), centred (first cell at
+// offset o = (nx - 64 b) / 2). Per slice, from the x polarisation:
+//   E_h  the jump's emission 2 Sw / H at the slot the field was solved at,
+//        inside that slot's source window (zero outside);
+//   U_h  the jump's field there;
+//   R_h  (training) the fine steps' field minus the jump's.
+// The input maps are E_h / s_h and U_h / u_h (s_h, u_h their rms over the
+// 64 x 64 cells), then 8 of the slice's features (felis_slice_features:
+// current, field intensity at the beam, |b_1|, coherent fraction, |b_3|,
+// |b_5|, energy spread, energy modulation), log10 s_1, log10 u_1 and the
+// cell's coordinates in [-1, 1]; the network's output o_h is the factor by
+// which the jump's emission is corrected: U += o_h E_h.
+namespace felis {
+namespace {
+
+constexpr int kFnoScalars = 12;  // 8 features, 2 scales, 2 coordinates
+
+struct FnoGeom {
+  int b, off;  // field cells per map cell, first field cell
+};
+
+// one workgroup per (slice, harmonic): the raw maps and their sums.
+// s_src: slot of the source (Sw, win2), s_fld: slot of the field (< 1 or >
+// slice_n: no field, the maps are zero). scal[sl][3][nh]: sum |E|^2, sum
+// |U|^2, and (training) sum over field cells of the fine field's |B|^2.
+__global__ void __launch_bounds__(256) k_fno_coarse(GridArgs g, FnoGeom fg, int J, const cpx<float> *__restrict__ A,
+                                                    const cpx<float> *__restrict__ Bf,
+                                                    const cpx<float> *__restrict__ Sw, const int4 *__restrict__ win2,
+                                                    float *__restrict__ Ec, float *__restrict__ Uc,
+                                                    float *__restrict__ Rc, float *__restrict__ scal) {
+  __shared__ double red[3][4];
+  const int sl = blockIdx.x, h = blockIdx.y, nh = g.nh;
+  const int s_src = sl + 1, s_fld = sl + 1 + J;
+  const bool live = s_fld <= g.slice_n;
+  const float k2 = 2.f / (float)g.harm[h];
+  const int4 w = win2[s_src];
+  const cpx<float> *src = Sw + ((long long)h * g.S + s_src) * g.N;
+  const long long fo = ((long long)h * g.S + phys_slot(live ? s_fld : 1, g.ring_base, g.S)) * g.N;
+  const float inv = 1.f / (float)(fg.b * fg.b);
+  double se = 0.0, su = 0.0, sb = 0.0;
+  for (int q = threadIdx.x; q < kP; q += 256) {
+    const int I = q % kN, Jc = q / kN;
+    cpx<float> e = mk(0.f, 0.f), u = mk(0.f, 0.f), r = mk(0.f, 0.f);
+    if (live) {
+      for (int dj = 0; dj < fg.b; ++dj)
+        for (int di = 0; di < fg.b; ++di) {
+          const int i = fg.off + fg.b * I + di, j = fg.off + fg.b * Jc + dj;
+          const long long c = (long long)j * g.nx + i;
+          const cpx<float> a = A[fo + c];
+          if (i >= w.x && i <= w.y && j >= w.z && j <= w.w) e = e + src[c] * k2;
+          u = u + a;
+          if (Bf) {
+            const cpx<float> bv = Bf[fo + c];
+            r = r + (bv - a);
+            sb += (double)bv.re * bv.re + (double)bv.im * bv.im;
+          }
+        }
+      e = e * inv;
+      u = u * inv;
+      r = r * inv;
+    }
+    const long long o = ((long long)sl * kP + q) * 2 * nh + 2 * h;
+    Ec[o] = e.re;
+    Ec[o + 1] = e.im;
+    Uc[o] = u.re;
+    Uc[o + 1] = u.im;
+    if (Rc) {
+      Rc[o] = r.re;
+      Rc[o + 1] = r.im;
+    }
+    se += (double)e.re * e.re + (double)e.im * e.im;
+    su += (double)u.re * u.re + (double)u.im * u.im;
+  }
+  double v[3] = {se, su, sb};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off);
+    if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int k = threadIdx.x;
+    scal[((long long)sl * 3 + k) * nh + h] = (float)((red[k][0] + red[k][1]) + (red[k][2] + red[k][3]));
+  }
+}
+
+// the network's inputs [sl][P][cin] and the normalised E (and R) in place
+__global__ void k_fno_inputs(int slice_n, int nh, const float *__restrict__ scal, const float *__restrict__ feat,
+                             int nfeat, float *__restrict__ Ec, const float *__restrict__ Uc, float *__restrict__ Rc,
+                             float *__restrict__ X) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)slice_n * kP) return;
+  const int sl = (int)(i / kP), q = (int)(i % kP);
+  const int cin = 4 * nh + kFnoScalars;
+  float *x = X + i * cin;
+  const float *sc = scal + (long long)sl * 3 * nh;
+  for (int h = 0; h < nh; ++h) {
+    const float s = sqrtf(sc[h] / kP), u = sqrtf(sc[nh + h] / kP);
+    const float is = s > 0.f ? 1.f / s : 0.f, iu = u > 0.f ? 1.f / u : 0.f;
+    float *e = Ec + i * 2 * nh + 2 * h;
+    e[0] *= is;
+    e[1] *= is;
+    if (Rc) {
+      float *r = Rc + i * 2 * nh + 2 * h;
+      r[0] *= is;
+      r[1] *= is;
+    }
+    x[2 * h] = e[0];
+    x[2 * h + 1] = e[1];
+    x[2 * nh + 2 * h] = Uc[i * 2 * nh + 2 * h] * iu;
+    x[2 * nh + 2 * h + 1] = Uc[i * 2 * nh + 2 * h + 1] * iu;
+  }
+  // features: current, intensity h1, |b1|, coherent fraction h1, |b3|, |b5|
+  // (|b1| where there are fewer harmonics), energy spread, modulation
+  const float *f = feat + (long long)sl * nfeat;
+  float *xs = x + 4 * nh;
+  xs[0] = f[0];
+  xs[1] = f[1];
+  xs[2] = f[2];
+  xs[3] = f[3];
+  xs[4] = nh > 1 ? f[2 + 5] : f[2];
+  xs[5] = nh > 2 ? f[2 + 10] : f[2];
+  xs[6] = f[1 + 5 * nh];
+  xs[7] = f[2 + 5 * nh];
+  xs[8] = log10f(sqrtf(sc[0] / kP) + 1e-30f);
+  xs[9] = log10f(sqrtf(sc[nh] / kP) + 1e-30f);
+  xs[10] = (2.f * (q % kN) + 1.f) / kN - 1.f;
+  xs[11] = (2.f * (q / kN) + 1.f) / kN - 1.f;
+}
+
+// the correction of a jump's emission from the network's factor maps, one
+// workgroup per (pol x harmonic, slot): on the slot's source window,
+// U += o_h(x, y) 2 Sw / H with o_h bilinear between the map cells' centres
+// (held at the edge cells beyond them), and the field energy's change added
+// to the slot's slice energy in a fixed order (as k_emit_correct)
+__global__ void __launch_bounds__(256) k_fno_correct(GridArgs g, FnoGeom fg, const int4 *__restrict__ win2,
+                                                     const float *__restrict__ o, const cpx<float> *__restrict__ Sw,
+                                                     cpx<float> *__restrict__ U, double *__restrict__ slice_e) {
+  __shared__ double red[4];
+  const int s = 1 + blockIdx.x, ph = blockIdx.y, h = ph % g.nh, nh = g.nh;
+  const int4 b = win2[s];
+  const int bw = b.y - b.x + 1, bh = b.w - b.z + 1;
+  const int ncell = bw > 0 && bh > 0 ? bw * bh : 0;
+  const float k2 = 2.f / (float)g.harm[h];
+  const float *om = o + (long long)(s - 1) * kP * 2 * nh + 2 * h;
+  const cpx<float> *src = Sw + ((long long)ph * g.S + s) * g.N;
+  cpx<float> *u = U + ((long long)ph * g.S + phys_slot(s, g.ring_base, g.S)) * g.N;
+  const float cen = 0.5f * (fg.b - 1), ib = 1.f / fg.b;
+  double de = 0.0;
+  for (int q = threadIdx.x; q < ncell; q += blockDim.x) {
+    const int j = b.z + q / bw, i = b.x + q % bw;
+    const long long c = (long long)j * g.nx + i;
+    float tx = ((float)(i - fg.off) - cen) * ib, ty = ((float)(j - fg.off) - cen) * ib;
+    tx = fminf(fmaxf(tx, 0.f), (float)(kN - 1));
+    ty = fminf(fmaxf(ty, 0.f), (float)(kN - 1));
+    const int x0 = min((int)tx, kN - 2), y0 = min((int)ty, kN - 2);
+    const float wx = tx - x0, wy = ty - y0;
+    auto at = [&](int xx, int yy) {
+      const float *p = om + (long long)(yy * kN + xx) * 2 * nh;
+      return mk(p[0], p[1]);
+    };
+    const cpx<float> f = (at(x0, y0) * (1.f - wx) + at(x0 + 1, y0) * wx) * (1.f - wy) +
+                         (at(x0, y0 + 1) * (1.f - wx) + at(x0 + 1, y0 + 1) * wx) * wy;
+    const cpx<float> old = u[c];
+    const cpx<float> nw = old + f * (src[c] * k2);
+    u[c] = nw;
+    de += ((double)nw.re * nw.re + (double)nw.im * nw.im) - ((double)old.re * old.re + (double)old.im * old.im);
+  }
+  for (int off = 32; off > 0; off >>= 1) de += __shfl_down(de, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = de;
+  __syncthreads();
+  if (threadIdx.x == 0) slice_e[(long long)ph * g.slice_n + s - 1] += (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+FnoGeom fno_geom(const felis_desc &d) {
+  FnoGeom fg;
+  fg.b = d.nx / kN;
+  fg.off = (d.nx - kN * fg.b) / 2;
+  return fg;
+}
+
+}  // namespace
+}  // namespace felis
+
+using felis::FnoGeom;
+
+int felis_fno_maps_ok(const felis_ctx *ctx) {
+  const felis_desc &d = ctx->d;
+  return d.nx == d.ny && d.nx >= kN && d.dimension == 3 && ctx->mixed();
+}
+
+size_t felis_fno_work_floats(const felis_ctx *ctx) {
+  const size_t sn = ctx->d.slice_n, nh = ctx->d.n_harm;
+  return sn * kP * 2 * nh * 2 + sn * 3 * nh + sn * kP * (4 * nh + felis::kFnoScalars);
+}
+
+// The learned correction of a jump in the run (after its push and deposit,
+// before the slippage): maps of every slot, the network, the correction.
+// work: felis_fno_work_floats(ctx) floats; feat: the slices' features
+// (device, as felis_slice_features) before the jump.
+int felis_fno_correct_dev(felis_ctx *ctx, felis_fno *net, const float *feat, float *work) {
+  const felis_desc &d = ctx->d;
+  const int sn = d.slice_n, nh = d.n_harm;
+  int cin = 0, cout = 0;
+  felis_fno_channels(net, &cin, &cout);
+  if (cin != 4 * nh + felis::kFnoScalars || cout != 2 * nh) {
+    ctx->err = "FNO channels do not match the harmonics";
+    return FELIS_ERR_ARG;
+  }
+  float *Ec = work, *Uc = Ec + (size_t)sn * kP * 2 * nh, *scal = Uc + (size_t)sn * kP * 2 * nh,
+        *X = scal + (size_t)sn * 3 * nh;
+  felis::GridArgs g = ctx->grid();
+  const FnoGeom fg = felis::fno_geom(d);
+  hipLaunchKernelGGL(felis::k_fno_coarse, dim3(sn, nh), dim3(256), 0, ctx->stream, g, fg, 0,
+                     (const felis::cpx<float> *)ctx->U, nullptr, (const felis::cpx<float> *)ctx->Sw, ctx->win2, Ec, Uc,
+                     nullptr, scal);
+  hipLaunchKernelGGL(felis::k_fno_inputs, dim3(nblk((long long)sn * kP)), dim3(256), 0, ctx->stream, sn, nh, scal, feat,
+                     felis::features_per_slice(nh), Ec, Uc, nullptr, X);
+  const float *o = nullptr;
+  int e = felis_fno_forward_dev(net, sn, X, &o, ctx->stream);
+  if (e) {
+    ctx->err = "FNO forward";
+    return e;
+  }
+  hipLaunchKernelGGL(felis::k_fno_correct, dim3(sn, ctx->npol() * nh), dim3(256), 0, ctx->stream, g, fg, ctx->win2, o,
+                     (const felis::cpx<float> *)ctx->Sw, (felis::cpx<float> *)ctx->U, ctx->slice_e);
+  return hipGetLastError() == hipSuccess ? FELIS_OK : FELIS_ERR_HIP;
+}
+
+extern "C" int felis_fno_jump_maps(felis_ctx *ctx, int slot, const float *feat, int nrows, const int *rows, float *X,
+                                   float *E, float *R, float *info) {
+  if (!ctx || slot < 0 || slot > 1 || !feat || nrows < 0 || (nrows && (!rows || !X || !E || !R || !info)))
+    return FELIS_ERR_ARG;
+  if (!felis_fno_maps_ok(ctx)) {
+    ctx->err = "FNO maps: batched 3D path on a square grid of 64 cells or more";
+    return FELIS_ERR_UNSUPPORTED;
+  }
+  felis_ctx::Snap &sn_ = ctx->snap[slot];
+  if (!sn_.valid || sn_.np != ctx->np) {
+    ctx->err = "no snapshot of this state in the slot";
+    return FELIS_ERR_STATE;
+  }
+  if (sn_.ring_base != ctx->ring_base) {
+    ctx->err = "snapshot and state differ in slippage";
+    return FELIS_ERR_STATE;
+  }
+  if (!ctx->last_jump || !ctx->Sw) {
+    ctx->err = "no jump has run on this context";
+    return FELIS_ERR_STATE;
+  }
+  const felis_desc &d = ctx->d;
+  const int sn = d.slice_n, nh = d.n_harm, cin = 4 * nh + felis::kFnoScalars, F = felis::features_per_slice(nh);
+  for (int k = 0; k < nrows; ++k)
+    if (rows[k] < 0 || rows[k] >= sn) return FELIS_ERR_ARG;
+  float *dev = nullptr;
+  const size_t nmap = (size_t)sn * kP * 2 * nh;
+  const size_t total = 3 * nmap + (size_t)sn * 3 * nh + (size_t)sn * kP * cin + (size_t)sn * F;
+  if (hipMalloc((void **)&dev, total * sizeof(float)) != hipSuccess) {
+    ctx->err = "FNO maps: hipMalloc";
+    return FELIS_ERR_HIP;
+  }
+  float *Ec = dev, *Uc = Ec + nmap, *Rc = Uc + nmap, *scal = Rc + nmap, *Xd = scal + (size_t)sn * 3 * nh,
+        *fd = Xd + (size_t)sn * kP * cin;
+  int e = FELIS_OK;
+  felis::GridArgs g = ctx->grid();
+  const FnoGeom fg = felis::fno_geom(d);
+  if (hipMemcpyAsync(fd, feat, (size_t)sn * F * sizeof(float), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+    e = FELIS_ERR_HIP;
+  if (!e) {
+    hipLaunchKernelGGL(felis::k_fno_coarse, dim3(sn, nh), dim3(256), 0, ctx->stream, g, fg, ctx->last_jump,
+                       (const felis::cpx<float> *)ctx->U, (const felis::cpx<float> *)sn_.U,
+                       (const felis::cpx<float> *)ctx->Sw, ctx->win2, Ec, Uc, Rc, scal);
+    hipLaunchKernelGGL(felis::k_fno_inputs, dim3(nblk((long long)sn * kP)), dim3(256), 0, ctx->stream, sn, nh, scal,
+                       fd, F, Ec, Uc, Rc, Xd);
+    if (hipGetLastError() != hipSuccess) e = FELIS_ERR_HIP;
+  }
+  for (int k = 0; k < nrows && !e; ++k) {
+    const size_t r = rows[k];
+    if (hipMemcpyAsync(X + (size_t)k * kP * cin, Xd + r * kP * cin, (size_t)kP * cin * sizeof(float),
+                       hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(E + (size_t)k * kP * 2 * nh, Ec + r * kP * 2 * nh, (size_t)kP * 2 * nh * sizeof(float),
+                       hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(R + (size_t)k * kP * 2 * nh, Rc + r * kP * 2 * nh, (size_t)kP * 2 * nh * sizeof(float),
+                       hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(info + (size_t)k * 3 * nh, scal + r * 3 * nh, (size_t)3 * nh * sizeof(float),
+                       hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+      e = FELIS_ERR_HIP;
+  }
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) e = FELIS_ERR_HIP;
+  (void)hipFree(dev);
+  if (e) ctx->err = "FNO maps";
+  return e;
+}

@@ -18,6 +18,7 @@
 
 #include "felis_ctx.h"
 #include "felis_adi.h"
+#include "felis_fno.h"
 #include "felis_pstep.h"
 #include "felis_jump.h"
 #include "felis_mlp.h"
@@ -335,6 +336,8 @@ struct JumpCtl {
   float *X = nullptr, *Y = nullptr;
   cpx<float> *gain = nullptr;
   bool use_net = false;  // this jump is corrected by the network
+  bool use_fno = false;  // ... by the field-to-field network
+  float *fno_work = nullptr;
 };
 
 // the lattice of steps k..k+J-1 is one undulator section with the field
@@ -553,8 +556,10 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
     if (jc) {
       (void)hipFree(jc->tab);
       (void)hipFree(jc->M);
+      (void)hipFree(jc->fno_work);
       jc->tab = nullptr;
       jc->M = nullptr;
+      jc->fno_work = nullptr;
     }
     ctx->dz_mul = 1.0;
   };
@@ -624,6 +629,10 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
     if (!e && hipMalloc((void **)&jc->M, (nm + 4) * sizeof(double) + ng * sizeof(cpx<float>) +
                                              (nx_ + ny_) * sizeof(float)) != hipSuccess) {
       ctx->err = "jump correction buffers";
+      e = FELIS_ERR_HIP;
+    }
+    if (!e && jc->o.fno && hipMalloc((void **)&jc->fno_work, felis_fno_work_floats(ctx) * sizeof(float)) != hipSuccess) {
+      ctx->err = "field-to-field correction buffers";
       e = FELIS_ERR_HIP;
     }
     if (!e) {
@@ -750,7 +759,7 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
       }
       if (ok) {
         // the network's inputs (and the energy spread) of the state before the jump
-        const bool moments = o.net || o.spread_max > 0;
+        const bool moments = o.net || o.fno || o.spread_max > 0;
         if (moments && ((err = felis_slice_features_dev(ctx, jc->M, jc->X)) ||
                         (err = felis_moment_totals_dev(ctx, jc->M, jc->tot))))
           break;
@@ -768,7 +777,8 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
         }
         const double b1 = b[2] > 0 ? std::sqrt(b[0] * b[0] + b[1] * b[1]) / b[2] : 0.0;
         if (!(b1 < o.bunching_max)) gate = false;  // the linear regime has ended: no more jumps
-        jc->use_net = o.net && (o.net_bunching_max <= 0 || b1 < o.net_bunching_max);
+        jc->use_net = !o.fno && o.net && (o.net_bunching_max <= 0 || b1 < o.net_bunching_max);
+        jc->use_fno = o.fno && (o.net_bunching_max <= 0 || b1 < o.net_bunching_max);
         if (o.spread_max > 0 && b[3] > 0) {
           const double mu = b[4] / b[3], var = std::max(b[5] / b[3] - mu * mu, 0.0);
           if (!(std::sqrt(var) / d.gamma_ref < o.spread_max)) gate = false;
@@ -851,6 +861,10 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
       break;
     if (jump) {
       // the learned emission correction, before the field slips
+      if (jc->use_fno) {
+        KTimer kt(ctx, FELIS_K_JUMP_WINDOW);
+        if ((err = felis_fno_correct_dev(ctx, jc->o.fno, jc->X, jc->fno_work))) break;
+      }
       if (jc->use_net) {
         GridArgs gc = ctx->grid();
         KTimer kt(ctx, FELIS_K_JUMP_WINDOW);
@@ -974,6 +988,10 @@ extern "C" int felis_run_surrogate(felis_ctx *ctx, const felis_lattice *lat_tab,
       ctx->err = "correction network: inputs 5 + 5 n_harm, outputs 2 n_harm, capacity >= slice_n";
       return FELIS_ERR_ARG;
     }
+  }
+  if (opts->fno && !felis_fno_maps_ok(ctx)) {
+    ctx->err = "field-to-field correction: a square field grid of 64 cells or more";
+    return FELIS_ERR_UNSUPPORTED;
   }
   JumpCtl jc;
   jc.o = *opts;

@@ -339,6 +339,166 @@ def fit_quality(model: Model, X, T, W) -> dict:
     return dict(mse=mse, mse_no_correction=base, ratio=mse / base if base > 0 else float("nan"))
 
 
+# ------------------------------------------------- field-to-field network
+
+FNO_MODEL = MODELS / "jump_fno_J10.npz"
+
+
+@dataclass
+class FnoModel:
+    """The field-to-field correction (felis_fno): parameters, input affine
+    and shape."""
+    P: np.ndarray
+    mu: np.ndarray
+    inv_sd: np.ndarray
+    dims: dict
+    periods: int
+    gate: float
+    meta: dict = field(default_factory=dict)
+
+    def network(self, capacity: int) -> fl.FNO:
+        d = self.dims
+        net = fl.FNO(d["cin"], d["cout"], d["width"], d["modes"], d["layers"], d["proj"], capacity=capacity, seed=1)
+        net.set_params(self.P)
+        net.set_input_affine(self.mu, self.inv_sd)
+        return net
+
+
+def save_fno(model: FnoModel, path) -> None:
+    np.savez(path, P=model.P, mu=model.mu, inv_sd=model.inv_sd, dims=np.array(json.dumps(model.dims)),
+             periods=np.int32(model.periods), gate=np.float64(model.gate), meta=np.array(json.dumps(model.meta)))
+
+
+def load_fno(path=FNO_MODEL) -> FnoModel:
+    z = np.load(path, allow_pickle=False)
+    return FnoModel(P=z["P"], mu=z["mu"], inv_sd=z["inv_sd"], dims=json.loads(str(z["dims"])),
+                    periods=int(z["periods"]), gate=float(z["gate"]), meta=json.loads(str(z["meta"])))
+
+
+def fno_pairs(deck_path, periods: int, gate: float, seed: int, rows_per_jump: int, rng):
+    """Training maps of one deck along its fine run: at every jump point
+    while |b1| < gate, J fine steps and one jump from the same state, then
+    felis_fno_jump_maps for `rows_per_jump` random slices whose field is
+    still in the window and took an emission. Returns X, E, R, weights (the
+    fine field energy of the fundamental at the slice, over its mean, at
+    most 10)."""
+    deck = fl.Deck(deck_path)
+    d = deck.desc(fl.MIXED_FP32)
+    ctx = fl.Context(d)
+    deck.load(ctx, seed=seed)
+    n_steps = deck.max_step
+    tab = deck.lattice(0, n_steps)
+    J = periods
+    Xs, Es, Rs, Ws = [], [], [], []
+    ctx.run(tab[0:3], 0, 1, outputs=False)
+    k = 1
+    while k + J <= n_steps:
+        b, tot = ctx.bfactor()
+        if not abs(b[0]) / tot < gate:
+            break
+        F = ctx.slice_features()
+        ctx.save_state(0)
+        ctx.run(tab[2 * k:2 * (k + J) + 1], k, J, outputs=False)
+        ctx.save_state(1)
+        ctx.restore_state(0)
+        _, _, _, kind = ctx.run_surrogate(tab[2 * k:2 * (k + J) + 1], k, J, periods=J, bunching_max=math.inf,
+                                          max_jumps=1)
+        if kind[0] == 1:
+            rows = rng.choice(d.slice_n - J, min(rows_per_jump, d.slice_n - J), replace=False)
+            X, E, R, info = ctx.fno_jump_maps(1, F, rows)
+            ok = info[:, 0, 0] > 0
+            Xs.append(X[ok])
+            Es.append(E[ok])
+            Rs.append(R[ok])
+            Ws.append(info[ok, 2, 0])
+        ctx.restore_state(1)
+        k += J
+    ctx.close()
+    deck.close()
+    if not Xs:
+        return None
+    W = np.concatenate(Ws).astype(np.float64)
+    W = np.clip(W / max(W.mean(), 1e-300), 0.0, 10.0).astype(np.float32)
+    return np.concatenate(Xs), np.concatenate(Es), np.concatenate(Rs), W
+
+
+def fit_fno(n_decks: int = 40, periods: int = 10, gate: float = 0.05, seed: int = 2026, slices: int = 768,
+            rows_per_jump: int = 6, steps: int = 3000, batch: int = 16, lr: float = 1e-3, width: int = 32,
+            modes: int = 8, layers: int = 4, proj: int = 64, log=print) -> FnoModel:
+    """Maps from n_decks random decks (training_decks), then Adam on the
+    GPU (felis_fno_train_step) with a cosine schedule. The input affine
+    z-scores the scalar channels (the maps are already normalised)."""
+    rng = np.random.default_rng(seed)
+    Xs, Es, Rs, Ws = [], [], [], []
+    t0 = time.perf_counter()
+    with tempfile.TemporaryDirectory() as td:
+        for i, (path, p) in enumerate(training_decks(n_decks, seed, td, slices=slices)):
+            out = fno_pairs(path, periods, gate, seed + 17 * i, rows_per_jump, rng)
+            if out is not None:
+                Xs.append(out[0])
+                Es.append(out[1])
+                Rs.append(out[2])
+                Ws.append(out[3])
+    X, E, R, W = (np.concatenate(a) for a in (Xs, Es, Rs, Ws))
+    if log:
+        log(f"fno data: {len(X)} maps from {n_decks} decks in {time.perf_counter() - t0:.1f} s")
+    nh = E.shape[2] // 2
+    # each (map, harmonic) enters the loss at the scale of its own residual:
+    # E and R are both divided by the rms of R (the factor that minimises
+    # the loss does not change), so maps whose residual the emission cannot
+    # explain do not dominate the ones it can
+    for i in range(0, len(R), 256):
+        r2 = R[i:i + 256, :, 0::2] ** 2 + R[i:i + 256, :, 1::2] ** 2
+        c = 1.0 / np.sqrt(r2.mean(1) + 1e-2)  # [maps][nh]
+        cc = np.repeat(c, 2, axis=1)[:, None, :].astype(np.float32)
+        E[i:i + 256] *= cc
+        R[i:i + 256] *= cc
+    cin = X.shape[2]
+    mu = np.zeros(cin, np.float32)
+    sd = np.ones(cin, np.float32)
+    sc = slice(4 * nh, cin - 2)  # the slice's scalars; maps and coordinates stay as they are
+    mu[sc] = X[:, 0, sc].mean(0)
+    sd[sc] = X[:, 0, sc].std(0) + 1e-6
+    net = fl.FNO(cin, 2 * nh, width, modes, layers, proj, capacity=batch, seed=seed + 1)
+    net.set_input_affine(mu, (1.0 / sd).astype(np.float32))
+    # the output layer (Q2 and c2, the last proj * cout + cout parameters)
+    # starts at zero: no correction until the data asks for one
+    P0 = net.params()
+    P0[len(P0) - (proj * 2 * nh + 2 * nh):] = 0.0
+    net.set_params(P0)
+    n = len(X)
+    losses = []
+    t1 = time.perf_counter()
+    for it in range(steps):
+        idx = rng.integers(0, n, batch)
+        lr_t = lr * 0.5 * (1 + math.cos(math.pi * it / steps))
+        losses.append(net.train_step(X[idx], E[idx], R[idx], W[idx], lr_t))
+        if log and (it % max(1, steps // 10) == 0 or it == steps - 1):
+            log(f"fno step {it}: loss {np.mean(losses[-50:]):.4f}")
+    P = net.params()
+    net.close()
+    # the loss of the best constant factor per (map, harmonic), the per-slice
+    # emission factor's model: sum w (|R|^2 - |<E, R>|^2 / |E|^2) / P
+    best = 0.0
+    for i in range(0, n, 256):
+        e = E[i:i + 256, :, 0::2] + 1j * E[i:i + 256, :, 1::2]
+        r = R[i:i + 256, :, 0::2] + 1j * R[i:i + 256, :, 1::2]
+        ee = np.sum(np.abs(e) ** 2, axis=1)
+        er = np.sum(np.conj(e) * r, axis=1)
+        rr = np.sum(np.abs(r) ** 2, axis=1)
+        best += float(np.sum(W[i:i + 256, None] * (rr - np.abs(er) ** 2 / np.maximum(ee, 1e-30))))
+    best /= n * 4096
+    # the loss of no correction (o = 0) on the same maps: sum w |R|^2
+    base = float(sum(np.sum(W[i:i + 256, None, None] * R[i:i + 256].astype(np.float64) ** 2)
+                     for i in range(0, n, 256)) / (n * 4096))
+    dims = dict(cin=cin, cout=2 * nh, width=width, modes=modes, layers=layers, proj=proj)
+    return FnoModel(P=P, mu=mu, inv_sd=(1.0 / sd).astype(np.float32), dims=dims, periods=periods, gate=gate,
+                    meta=dict(maps=int(n), decks=n_decks, slices=slices, steps=steps, batch=batch, lr=lr,
+                              final_loss=float(np.mean(losses[-100:])), first_loss=float(np.mean(losses[:20])),
+                              loss_no_correction=base, loss_best_constant_factor=best, seed=seed,
+                              train_seconds=time.perf_counter() - t1))
+
+
 # ---------------------------------------------------------------- evaluation
 
 def saturation(z, P):
@@ -354,7 +514,7 @@ def saturation(z, P):
 
 
 def run_curves(deck_path, seed: int, periods: int | None = None, gate: float = 0.05, net=None,
-               n_steps: int | None = None, net_gate: float = 0.0):
+               n_steps: int | None = None, net_gate: float = 0.0, fno=None):
     """One run of the deck: plain batched (periods None) or with jumps, with
     the workload's autotuned launch shapes. Returns steps, total field energy
     per step (NaN inside jumps), |b1|, kinds and wall seconds of the step
@@ -375,7 +535,7 @@ def run_curves(deck_path, seed: int, periods: int | None = None, gate: float = 0
         kind = np.zeros(N, np.int32)
     else:
         e, b, _, kind = ctx.run_surrogate(tab, 0, N, periods=periods, bunching_max=gate, net=net,
-                                          net_bunching_max=net_gate)
+                                          net_bunching_max=net_gate, fno=fno)
     ctx.sync()
     dt = time.perf_counter() - t0
     E = e[:, :nh].sum(1) + e[:, nh:].sum(1)

@@ -187,6 +187,7 @@ int felis_run_noalloc(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int
  * bunching |b1| of the state below bunching_max; once it fails on the
  * bunching, the rest of the run takes fine steps (the plain batched step). */
 typedef struct felis_mlp felis_mlp;
+typedef struct felis_fno felis_fno;
 typedef struct {
   int periods;          /* J, 1..min(64, slice_n) */
   int max_jumps;        /* jumps allowed in this call (< 0: no limit) */
@@ -198,7 +199,9 @@ typedef struct {
                            complex factor on the jump's emission E = 2 Sw / H: after the
                            push the field gains (e_h - 1) E */
   double net_bunching_max; /* the network corrects jumps while |b1| < this (<= 0: always) */
-  int pad[2];
+  felis_fno *fno;       /* field-to-field correction (NULL: none; in place of net when set): maps
+                           each slice's emission and field on 64 x 64 cells, with its moments, to
+                           factor maps o_h on the emission: the field gains o_h E_h (felis_fno_jump_maps) */
 } felis_jump_opts;
 /* felis_run with jumps (FELIS_MIXED_FP32, Time_Dependent). kind[n_steps]
  * (may be NULL): 0 a fine step, 1 the first period of a jump, 2 a period
@@ -359,7 +362,6 @@ int felis_gemm_bf16(int M, int N, int K, const float *A, const float *B, float *
  * kept x and y modes, GELU after all but the last), projection (width ->
  * proj -> cout, GELU between). BF16 operands on the matrix cores, FP32
  * accumulation and storage, FP32 Adam. width and modes powers of two. */
-typedef struct felis_fno felis_fno;
 int felis_fno_create(int cin, int cout, int width, int modes, int layers, int proj, int capacity, uint64_t seed,
                      felis_fno **out);
 int felis_fno_destroy(felis_fno *net);
@@ -385,6 +387,17 @@ int felis_fno_forward(felis_fno *net, int maps, const float *X, float *Y);
  * 0, an Adam step. */
 int felis_fno_train_step(felis_fno *net, int maps, const float *X, const float *E, const float *R, const float *wb,
                          int dev, float lr, double *loss);
+/* After a jump (the current state a) against snapshot `slot` (b, the fine
+ * steps from the same state, same slippage): for each slice listed in
+ * rows[nrows], the network's input maps X [4096][4 n_harm + 12] (feat: the
+ * slices' felis_slice_features before the jump, host [slice_n][...]), the
+ * jump's emission E and the residual R = b - a [4096][2 n_harm], both
+ * divided by the rms of E per harmonic, where the field solved at the
+ * slice's slot is now (periods slots on; zero once it has left the slices),
+ * and info [3][n_harm]: sum over the map cells of |E|^2 and |U|^2 (before
+ * the division) and over the field cells of the fine field's |b|^2. */
+int felis_fno_jump_maps(felis_ctx *ctx, int slot, const float *feat, int nrows, const int *rows, float *X, float *E,
+                        float *R, float *info);
 
 #ifdef __cplusplus
 }
