@@ -557,14 +557,25 @@ def _main(argv=None):
     ap.add_argument("--slices", type=int, default=768)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--out", default=str(DEFAULT_MODEL))
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--fno", action="store_true",
+                    help="train the field-to-field network (felis_fno) instead: --decks random decks (40 for the "
+                         "committed model), --steps Adam steps (4000), 6 maps per jump point")
     a = ap.parse_args(argv)
     t0 = time.perf_counter()
+    if a.fno:
+        model = fit_fno(n_decks=a.decks, periods=a.periods, gate=a.gate, seed=2026, slices=a.slices, steps=a.steps,
+                        log=lambda m: print(m, flush=True))
+        out = a.out or str(FNO_MODEL)
+        save_fno(model, out)
+        print(f"saved {out} after {time.perf_counter() - t0:.1f} s", flush=True)
+        return
     model = fit(n_decks=a.decks, rounds=a.rounds, periods=a.periods, gate=a.gate, seed=a.seed, slices=a.slices,
                 steps=a.steps, log=lambda m: print(m, flush=True))
-    Path(a.out).parent.mkdir(parents=True, exist_ok=True)
-    save(model, a.out)
-    print(f"saved {a.out} after {time.perf_counter() - t0:.1f} s", flush=True)
+    out = a.out or str(DEFAULT_MODEL)
+    Path(out).parent.mkdir(parents=True, exist_ok=True)
+    save(model, out)
+    print(f"saved {out} after {time.perf_counter() - t0:.1f} s", flush=True)
 
 
 if __name__ == "__main__":
