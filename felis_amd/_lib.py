@@ -91,7 +91,7 @@ class Lattice(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("step_graph", C.c_int), ("deposit_lds", C.c_int), ("deposit_threads", C.c_int),
-                ("adi_threads", C.c_int)]
+                ("adi_threads", C.c_int), ("deposit_run", C.c_int), ("adi_lanes", C.c_int)]
 
 
 class JumpOpts(C.Structure):
@@ -454,9 +454,11 @@ class Context:
         self._chk(self.L.felis_jump_emission_residual(self.h, slot, _p(out)))
         return out
 
-    def set_tuning(self, step_graph: int = 0, deposit_lds: int = 0, deposit_threads: int = 0, adi_threads: int = 0):
+    def set_tuning(self, step_graph: int = 0, deposit_lds: int = 0, deposit_threads: int = 0, adi_threads: int = 0,
+                   deposit_run: int = 0, adi_lanes: int = 0):
         """Launch shapes of the batched step (felis_tuning)."""
-        t = Tuning(int(step_graph), int(deposit_lds), int(deposit_threads), int(adi_threads))
+        t = Tuning(int(step_graph), int(deposit_lds), int(deposit_threads), int(adi_threads), int(deposit_run),
+                   int(adi_lanes))
         self._chk(self.L.felis_set_tuning(self.h, C.byref(t)))
 
     def set_deposit_mode(self, mode: int):
@@ -472,7 +474,7 @@ class Context:
         t = Tuning()
         self._chk(self.L.felis_get_tuning(self.h, C.byref(t)))
         return dict(step_graph=t.step_graph, deposit_lds=t.deposit_lds, deposit_threads=t.deposit_threads,
-                    adi_threads=t.adi_threads)
+                    adi_threads=t.adi_threads, deposit_run=t.deposit_run, adi_lanes=t.adi_lanes)
 
     def field_norms(self, spot_sig_r: float = 0.0):
         """(powerNorm, intensityNorm) of the field dump."""

@@ -140,6 +140,7 @@ int dep_args(felis_ctx *ctx, const felis_lattice *lat, DepArgs *out) {
   a.inv_dx = 1.0 / d.dx;
   a.inv_dy = 1.0 / d.dy;
   a.inv_dB = 1.0 / ctx->dB;
+  a.run = ctx->tune.deposit_run > 0 ? ctx->tune.deposit_run : 8;  // the default run of the batched deposit
   *out = a;
   return FELIS_OK;
 }

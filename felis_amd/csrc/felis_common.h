@@ -119,6 +119,7 @@ struct DepArgs {
   double ue, ka;
   double inv_dx, inv_dy, inv_dB;  // reciprocals for the batched deposit
   int last_species_first_seg;
+  int run;  // consecutive particles one batched deposit thread combines
 };
 
 struct AdiCoef {

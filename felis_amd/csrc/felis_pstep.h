@@ -20,7 +20,7 @@
 
 namespace felis {
 
-constexpr int kRun = 8;       // consecutive particles one deposit thread combines
+constexpr int kRun = 8;       // consecutive particles one deposit thread combines (DepArgs::run, tuned)
 constexpr int kMaxModes = 4;  // longitudinal space-charge modes of the batched path
 
 // transversePush constants of one lattice row (particlePush.cpp:76-90),
@@ -889,11 +889,11 @@ __global__ void __launch_bounds__(1024)
       const int seg = sp * g.slice_n + sl;
       const long long b = seg_off[seg], e = seg_off[seg + 1];
       const float macro_u = (float)(seg_w[seg] * d.coef_U);
-      // each thread takes kRun consecutive particles; consecutive particles
+      // each thread takes d.run consecutive particles; consecutive particles
       // in the same cell (the phases of a beamlet) are summed in registers
       // and reach the window with one set of atomics
-      for (long long r0 = b + (long long)tid * kRun; r0 < e; r0 += (long long)nt * kRun) {
-      const long long r1 = min(r0 + (long long)kRun, e);
+      for (long long r0 = b + (long long)tid * d.run; r0 < e; r0 += (long long)nt * d.run) {
+      const long long r1 = min(r0 + (long long)d.run, e);
       int key = -1, kI = 0, kJ = 0;
       bool kin = false;
       cpx<float> acc[4][NH][2];
@@ -1060,8 +1060,8 @@ __global__ void __launch_bounds__(1024)
     const long long b = seg_off[seg], e = seg_off[seg + 1];
     const double macro = seg_w[seg];
     const float inv_drf = (float)(1.0 / g.dr);
-    for (long long r0 = b + (long long)tid * kRun; r0 < e; r0 += (long long)nt * kRun) {
-      const long long r1 = min(r0 + (long long)kRun, e);
+    for (long long r0 = b + (long long)tid * d.run; r0 < e; r0 += (long long)nt * d.run) {
+      const long long r1 = min(r0 + (long long)d.run, e);
       int key = -1;
       double amp = 0.0;
       // a run's sums of i e^{-i theta} weighted by each particle's own radial

@@ -111,8 +111,11 @@ int launch_adi_scan(felis_ctx *ctx, const cpx<float> *tab, int cf, const cpx<flo
 // LDS limit
 bool adi_scan_fits(felis_ctx *ctx) {
   const felis_desc &d = ctx->d;
-  const int cols = adi_scan_cols(d.nx, d.ny, (size_t)ctx->lds_max);
+  int cols = adi_scan_cols(d.nx, d.ny, (size_t)ctx->lds_max);
   if (cols == 0) return false;
+  // tuned: fewer lanes' columns in LDS (the rest through scratch)
+  const int Et = adi_scan_E(d.nx, d.ny);
+  if (ctx->tune.adi_lanes > 0 && ctx->tune.adi_lanes * Et < cols) cols = ctx->tune.adi_lanes * Et;
   ctx->adi_cols = cols;
   const size_t lds = adi_scan_lds(d.nx, d.ny, cols);
   const int E = adi_scan_E(d.nx, d.ny);
@@ -1017,6 +1020,15 @@ extern "C" int felis_set_tuning(felis_ctx *ctx, const felis_tuning *t) {
   }
   if (t->adi_threads != 0 && t->adi_threads != 512 && t->adi_threads != 1024) {
     ctx->err = "felis_set_tuning: adi_threads must be 512 or 1024";
+    return FELIS_ERR_ARG;
+  }
+  if (t->deposit_run != 0 && t->deposit_run != 2 && t->deposit_run != 4 && t->deposit_run != 8 &&
+      t->deposit_run != 16) {
+    ctx->err = "felis_set_tuning: deposit_run must be 2, 4, 8 or 16";
+    return FELIS_ERR_ARG;
+  }
+  if (t->adi_lanes < 0 || t->adi_lanes > 15) {
+    ctx->err = "felis_set_tuning: adi_lanes must be 0 .. 15";
     return FELIS_ERR_ARG;
   }
   ctx->tune = *t;

@@ -19,7 +19,14 @@ The tunables:
   adi_threads      threads per field-solve workgroup (one system per
                    workgroup, its LDS allows one per CU: 1024 threads give
                    each SIMD four waves to hide latency with, at 128
-                   registers per lane).
+                   registers per lane);
+  deposit_run      consecutive particles a deposit thread walks (the
+                   phases of a beamlet share a cell and are combined in
+                   registers; longer runs combine more, shorter runs have
+                   more threads in flight);
+  adi_lanes        0: the field solve keeps the whole y right-hand side of
+                   a system in LDS; 8: only the first 8 lanes' columns, the
+                   rest through scratch in HBM, for less LDS per system.
 
     python -m felis_amd.tune felis_amd/decks/cfg2.inp [--out felis_amd/tuning.json]
 """
@@ -37,9 +44,9 @@ from . import _lib as fl
 
 CACHE = Path(__file__).resolve().parent / "tuning.json"
 ARCH = "gfx950"
-DEFAULT = dict(step_graph=0, deposit_threads=1024, deposit_lds=0, adi_threads=512)
+DEFAULT = dict(step_graph=0, deposit_threads=1024, deposit_lds=0, adi_threads=512, deposit_run=8, adi_lanes=0)
 SPACE = dict(step_graph=(0, 1), deposit_threads=(256, 512, 1024), deposit_lds=(0, 65536, 32768),
-             adi_threads=(512, 1024))
+             adi_threads=(512, 1024), deposit_run=(4, 8, 16), adi_lanes=(0, 8))
 
 
 def key(desc, n_particles: int, npol: int) -> str:

@@ -249,6 +249,9 @@ typedef struct {
   int deposit_lds;  /* LDS bytes per deposit workgroup (0: all a workgroup may use) */
   int deposit_threads; /* threads per deposit workgroup: 256, 512 or 1024 (0: 1024) */
   int adi_threads;  /* threads per field-solve workgroup: 512 or 1024 (0: 512) */
+  int deposit_run;  /* consecutive particles one deposit thread combines: 2, 4, 8 or 16 (0: 8) */
+  int adi_lanes;    /* 1..15: the field solve keeps the y right-hand side of that many lanes' columns in LDS
+                       and passes the rest through scratch (less LDS per system); 0: all columns that fit */
 } felis_tuning;
 int felis_set_tuning(felis_ctx *ctx, const felis_tuning *t);
 int felis_get_tuning(felis_ctx *ctx, felis_tuning *t);

@@ -6,7 +6,9 @@ launch shape or order: the field energy of every period and the final field
 agree with the default tuning's to 1e-5 relative (float atomics of the
 deposit summed in another order; the deposit's LDS budget of 32 KiB forces
 its harmonics into chunks, and at 4 KiB the windows go through LDS in strips
-of node rows, one harmonic per pass), and the particles' phases to 1e-4 rad.
+of node rows, one harmonic per pass; other deposit run lengths combine
+other runs of particles; a field solve with 8 lanes' columns in LDS passes
+the rest through scratch), and the particles' phases to 1e-4 rad.
 A 4 KiB budget with 256 radial grids is raised to hold the radial source row
 (257 complex doubles).
 """
@@ -57,9 +59,11 @@ def ref_nr256():
                                dict(step_graph=1, deposit_threads=256, deposit_lds=32768),
                                dict(adi_threads=1024),
                                dict(step_graph=1, deposit_threads=512, deposit_lds=65536, adi_threads=1024),
-                               dict(deposit_threads=512, deposit_lds=4096)],
+                               dict(deposit_threads=512, deposit_lds=4096),
+                               dict(deposit_run=4), dict(deposit_run=16, deposit_threads=256),
+                               dict(adi_lanes=8), dict(adi_lanes=8, adi_threads=1024, step_graph=1)],
                          ids=["graph", "dep256", "dep512_64k", "graph_dep256_32k", "adi1024", "graph_dep512_adi1024",
-                              "dep_window_past_lds"])
+                              "dep_window_past_lds", "run4", "run16_dep256", "adi_split8", "adi_split8_graph"])
 def test_tunings_agree_with_default(ref, t):
     r = _run(t)
     assert tune._rel_energy_err(r["e"], ref["e"]) < 1e-5
@@ -86,8 +90,13 @@ def test_set_tuning_rejects_bad_shapes():
         ctx.set_tuning(deposit_lds=-1)
     with pytest.raises(fl.FelisError):
         ctx.set_tuning(adi_threads=256)
-    ctx.set_tuning(step_graph=1, deposit_threads=512, deposit_lds=40000, adi_threads=1024)
-    assert ctx.tuning() == dict(step_graph=1, deposit_threads=512, deposit_lds=40000, adi_threads=1024)
+    with pytest.raises(fl.FelisError):
+        ctx.set_tuning(deposit_run=3)
+    with pytest.raises(fl.FelisError):
+        ctx.set_tuning(adi_lanes=16)
+    ctx.set_tuning(step_graph=1, deposit_threads=512, deposit_lds=40000, adi_threads=1024, deposit_run=16, adi_lanes=8)
+    assert ctx.tuning() == dict(step_graph=1, deposit_threads=512, deposit_lds=40000, adi_threads=1024, deposit_run=16,
+                                adi_lanes=8)
     ctx.close()
 
 

@@ -16,7 +16,7 @@ def _desc(**kw):
 
 def test_candidates_cover_the_space_once():
     c = list(tune.candidates())
-    assert len(c) == 2 * 3 * 3 * 2
+    assert len(c) == 2 * 3 * 3 * 2 * 3 * 2
     assert len({tuple(sorted(t.items())) for t in c}) == len(c)
     assert tune.DEFAULT in c
 
