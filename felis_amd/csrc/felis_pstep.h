@@ -33,6 +33,7 @@ struct TransArgs {
 struct PStepArgs {
   LatArgs l0, l1;  // transversePush at (it,0); everything else at (it,0.5)
   TransArgs t0, t1;
+  int trans_same;  // t0 and t1 are the same map (a uniform section): formed once, applied twice
   // push_theta_gamma constants of l1 (particlePush.cpp:260-300), formed on
   // the host: K2 = 1 + p^2 + K2a (1 + K2b r^2), xi = xic / gamma0^2,
   // xi2 = xi2c |p|_ue, sq = K0 sqrt(1 + ue^2), rotm = e^{-i Phi/2}
@@ -166,8 +167,10 @@ __host__ inline TransArgs trans_args(const LatArgs &l) {
   return t;
 }
 
+// times: 1, or 2 for the same half step applied twice (the map's
+// coefficients are formed once)
 __device__ inline void transverse_f32(const TransArgs &t, float dz, float gam, float &x, float &y, float &px,
-                                      float &py) {
+                                      float &py, int times = 1) {
   const float aa = t.aa, bb = t.bb, cc = t.cc;
   const int ka = t.ka;
   const float ig = __builtin_amdgcn_rcpf(gam), ig2 = ig * ig;
@@ -183,35 +186,40 @@ __device__ inline void transverse_f32(const TransArgs &t, float dz, float gam, f
       const float hw = dz * dz * (ig * f);
       const float c = fmaf(hw, fmaf(hw, 1.0f / 24.0f, -0.5f), 1.0f);
       const float s = dz * fmaf(hw, -1.0f / 6.0f, 1.0f);
-      const float u0 = u;
-      u = fmaf(c, u0, (s * ig) * p);
-      p = fmaf(c, p, -(s * f) * u0);
+      const float si = s * ig, sf = s * f;
+      for (int k = 0; k < times; ++k) {
+        const float u0 = u;
+        u = fmaf(c, u0, si * p);
+        p = fmaf(c, p, -sf * u0);
+      }
     };
     plane(x, px, fx);
     plane(y, py, fy);
     return;
   }
-  float kx = 0, ky = 0, kpx = 0, kpy = 0, sx = 0, sy = 0, spx = 0, spy = 0;
   const float cm[5] = {0, 0, 0.5f, 0.5f, 1};
   const float wt[5] = {0, 1, 2, 2, 1};
+  for (int rep = 0; rep < times; ++rep) {
+    float kx = 0, ky = 0, kpx = 0, kpy = 0, sx = 0, sy = 0, spx = 0, spy = 0;
 #pragma unroll
-  for (int m = 1; m < 5; ++m) {
-    float xx = x + dz * kx * cm[m], yy = y + dz * ky * cm[m];
-    float ppx = px + dz * kpx * cm[m], ppy = py + dz * kpy * cm[m];
-    kx = ppx * ig - aa * (ka * xx + yy) * ig2;
-    ky = ppy * ig + aa * (ka * xx - yy) * ig2;
-    kpx = ka * (ppx - ppy) * dd - fx * xx;
-    kpy = ka * (ppx + ppy) * dd - fy * yy;
-    sx += wt[m] * kx;
-    sy += wt[m] * ky;
-    spx += wt[m] * kpx;
-    spy += wt[m] * kpy;
+    for (int m = 1; m < 5; ++m) {
+      float xx = x + dz * kx * cm[m], yy = y + dz * ky * cm[m];
+      float ppx = px + dz * kpx * cm[m], ppy = py + dz * kpy * cm[m];
+      kx = ppx * ig - aa * (ka * xx + yy) * ig2;
+      ky = ppy * ig + aa * (ka * xx - yy) * ig2;
+      kpx = ka * (ppx - ppy) * dd - fx * xx;
+      kpy = ka * (ppx + ppy) * dd - fy * yy;
+      sx += wt[m] * kx;
+      sy += wt[m] * ky;
+      spx += wt[m] * kpx;
+      spy += wt[m] * kpy;
+    }
+    const float dz6 = dz * (1.0f / 6.0f);
+    x += dz6 * sx;
+    y += dz6 * sy;
+    px += dz6 * spx;
+    py += dz6 * spy;
   }
-  const float dz6 = dz * (1.0f / 6.0f);
-  x += dz6 * sx;
-  y += dz6 * sy;
-  px += dz6 * spx;
-  py += dz6 * spy;
 }
 
 // Em of the longitudinal space-charge modes (the first L <= ML) for an
@@ -358,6 +366,10 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
     }
     cu[h] = UY ? etaX * fx * ux - etaY * fy * uy : etaX * fx * ux;
   }
+  // the coupling's imaginary part enters theta over H: cuH = cu / H, once
+  cpx<float> cuH[NH];
+#pragma unroll
+  for (int h = 0; h < NH; ++h) cuH[h] = cu[h] * a.hinv[h];
   const float sq = a.sq;
   const float ks_f = (float)ks;
   const float e_mc2 = (float)(kECharge / kEMass / kC / kC);
@@ -379,6 +391,8 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
   const float Akf = (float)Ak, rate0f = (float)rate0, ig0f = (float)ig0, dzf = (float)dz;
   const float c20 = (float)(ks * 0.5 * ig0 * ig0);
   const cpx<float> e0 = expi_f32(th0 + halfPhi);
+  // the first mode's Em times e^{-i Phi/2}: 2 Re(Em e^{i theta}) = 2 Re(Em' e1)
+  const cpx<float> Em0r = Em[0] * rotm;
   float kth = 0.f, kg = 0.f, sth = 0.f, sg = 0.f;
 #pragma unroll
   for (int m = 1; m < 5; ++m) {
@@ -393,21 +407,24 @@ __device__ inline bool push_tg_f32(const GridArgs &g, const PStepArgs &a, const 
     const float igm = 0.5f * ig0f * r;
     cpx<float> eh[NH];
     harm_powers<NH>(e1, a.hstep, eh);
-    float sumTh = 0.f, sumG = 0.f;
+    // sumTh = -sum Im(e_h cu_h) / H, sumG = igm sum Re(e_h cu_h), by FMAs
+    float sumTh = 0.f, sumG0 = 0.f;
 #pragma unroll
     for (int h = 0; h < NH; ++h) {
-      const cpx<float> t = eh[h] * cu[h];
-      sumTh -= t.im * a.hinv[h];
-      sumG += t.re * igm;
+      sumTh = fmaf(-eh[h].re, cuH[h].im, fmaf(-eh[h].im, cuH[h].re, sumTh));
+      sumG0 = fmaf(eh[h].re, cu[h].re, fmaf(-eh[h].im, cu[h].im, sumG0));
     }
-    const cpx<float> eth = e1 * rotm;  // e^{i theta}
-    float sumEz = 2.0f * (Em[0].re * eth.re - Em[0].im * eth.im);
-    cpx<float> ethl = eth;  // e^{i (l+1) theta}
+    const float sumG = sumG0 * igm;
+    float sumEz = 2.0f * (Em0r.re * e1.re - Em0r.im * e1.im);
+    if (ML > 1) {
+      const cpx<float> eth = e1 * rotm;  // e^{i theta}
+      cpx<float> ethl = eth;             // e^{i (l+1) theta}
 #pragma unroll
-    for (int l = 1; l < ML; ++l) {
-      if (l >= L) break;
-      ethl = ethl * eth;
-      sumEz += 2.0f * (Em[l].re * ethl.re - Em[l].im * ethl.im);
+      for (int l = 1; l < ML; ++l) {
+        if (l >= L) break;
+        ethl = ethl * eth;
+        sumEz += 2.0f * (Em[l].re * ethl.re - Em[l].im * ethl.im);
+      }
     }
     const float r2 = r * r;
     // kth - rate0
@@ -446,6 +463,10 @@ __device__ inline void transverse_step(const GridArgs &g, const PStepArgs &a, do
                                        float &ppx, float &ppy) {
   const float dzh = (float)(g.dz * 0.5);
   const float gf = (float)gm;
+  if (a.trans_same) {
+    transverse_f32(a.t0, dzh, gf, xx, yy, ppx, ppy, 2);
+    return;
+  }
   transverse_f32(a.t0, dzh, gf, xx, yy, ppx, ppy);
   transverse_f32(a.t1, dzh, gf, xx, yy, ppx, ppy);
 }
@@ -486,25 +507,25 @@ struct EmCarryMixed {
 // than one longitudinal space-charge mode.
 // With tile_bb the tile's box of in-grid cells (min i, max i, min j, max j of
 // the pushed positions) goes to tile_bb[tile], from which k_deposit_slot takes
-// its window without reading the particles' positions for it. The tile is
-// blockIdx.x + t0 (a launch may cover a range of tiles).
+// its window without reading the particles' positions for it.
 template <int NH, bool UY, bool LM>
 __global__ void __launch_bounds__(kTile)
     k_pstep(GridArgs g0, PStepArgs a, Tiles tl, double *__restrict__ theta, float *__restrict__ gamma,
             float *__restrict__ x, float *__restrict__ y, float *__restrict__ px, float *__restrict__ py,
             const double *__restrict__ seg_w, const cpx<float> *__restrict__ Ux, const cpx<float> *__restrict__ Uy,
             const cpx<double> *__restrict__ Ez, const float *__restrict__ bes, const double *__restrict__ wakeE,
-            const cpx<float> *__restrict__ emc, double *__restrict__ tile_red, int4 *__restrict__ tile_bb, int t0,
+            const cpx<float> *__restrict__ emc, double *__restrict__ tile_red, int4 *__restrict__ tile_bb,
             unsigned *err) {
-  // per-particle moments (FP32): e^{iH theta} re, im per harmonic, the six
-  // twiss products and 1; their row sums, one set per 16-lane row
-  constexpr int NF = 2 * NH + 7;
+  // per-particle moments (FP32): e^{iH theta} re, im per harmonic and the six
+  // twiss products; their row sums, one set per 16-lane row (the tile's
+  // particle count comes from the tile table)
+  constexpr int NF = 2 * NH + 6;
   __shared__ float rows[kTile / 16][NF];
   __shared__ int rbb[kTile / 16][4];
   __shared__ int scell[kTile];  // each particle's cell (j << 16 | i), -1 off the grid
   GridArgs g = g0;
   g.ring_base = ring_of(g0);
-  const int t = blockIdx.x + t0, k = threadIdx.x;
+  const int t = blockIdx.x, k = threadIdx.x;
   const int seg = tl.seg[t];
   const int sl = seg % g.slice_n, slot = sl + 1;
   const bool on = k < tl.cnt[t];
@@ -585,7 +606,6 @@ __global__ void __launch_bounds__(kTile)
     acc[2 * NH + 3] = yp * yp;
     acc[2 * NH + 4] = xx * xp;
     acc[2 * NH + 5] = yy * yp;
-    acc[2 * NH + 6] = 1.f;
   }
   // tile sums in a fixed order: each 16-lane row adds its particles by a
   // DPP tree in FP32 (16 terms), the 16 row sums are added in FP64; the
@@ -635,16 +655,21 @@ __global__ void __launch_bounds__(kTile)
     }
     tile_bb[t] = b;
   }
-  for (int q = k; q < NF; q += kTile) {
+  const long long ntl = gridDim.x;  // tiles: rows of tile_red are this long
+  for (int q = k; q <= NF; q += kTile) {
     double v = 0.0;
+    if (q < NF) {
 #pragma unroll
-    for (int r = 0; r < kTile / 16; ++r) v += (double)rows[r][q];
+      for (int r = 0; r < kTile / 16; ++r) v += (double)rows[r][q];
+    } else {
+      v = (double)tl.cnt[t];
+    }
     // output order: bunching re, im per harmonic, weight, six twiss sums, count
     if (q < 2 * NH) {
-      tile_red[(long long)q * gridDim.x + t] = v * w;
+      tile_red[(long long)q * ntl + t] = v * w;
     } else {
-      tile_red[(long long)(q + 1) * gridDim.x + t] = v;
-      if (q == NF - 1) tile_red[(long long)(2 * NH) * gridDim.x + t] = v * w;
+      tile_red[(long long)(q + 1) * ntl + t] = v;
+      if (q == NF) tile_red[(long long)(2 * NH) * ntl + t] = v * w;
     }
   }
 }

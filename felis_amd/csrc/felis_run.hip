@@ -152,11 +152,11 @@ void em_carry_mixed(felis_ctx *ctx, const GridArgs &g, const PStepArgs &a) {
                      ctx->tiles(), ctx->em_last, ctx->em_src, (cpx<float> *)ctx->emc);
 }
 
-// k_pstep over tiles [t0, t1) on stream st; uy: the Uy-live form
+// k_pstep over every tile; uy: the Uy-live form
 template <int NH>
-int launch_particles(felis_ctx *ctx, const PStepArgs &a, double *tile_red, void *Ufield, int4 *tile_bb, int t0,
-                     int t1, bool uy, hipStream_t st) {
-  if (t1 <= t0) return FELIS_OK;
+int launch_particles(felis_ctx *ctx, const PStepArgs &a, double *tile_red, void *Ufield, int4 *tile_bb, bool uy,
+                     hipStream_t st) {
+  if (ctx->ntiles <= 0) return FELIS_OK;
   GridArgs g = ctx->grid();
   KTimer kt(ctx, FELIS_K_PARTICLE_STEP);
   const cpx<float> *Ux = (const cpx<float> *)Ufield, *Uy = (const cpx<float> *)((char *)Ufield + ctx->pol_bytes());
@@ -164,10 +164,10 @@ int launch_particles(felis_ctx *ctx, const PStepArgs &a, double *tile_red, void 
   const bool carry = ctx->em_carry && !a.drift;
   const cpx<float> *emc = carry ? (const cpx<float> *)ctx->emc : nullptr;
 #define PSTEP_LAUNCH(UY, LM)                                                                                    \
-  hipLaunchKernelGGL((k_pstep<NH, UY, LM>), dim3(t1 - t0), dim3(kTile), 0, st, g, a, ctx->tiles(), ctx->theta,     \
+  hipLaunchKernelGGL((k_pstep<NH, UY, LM>), dim3(ctx->ntiles), dim3(kTile), 0, st, g, a, ctx->tiles(), ctx->theta, \
                      (float *)ctx->gamma, (float *)ctx->x, (float *)ctx->y, (float *)ctx->px, (float *)ctx->py,    \
                      ctx->seg_w, Ux, Uy, (const cpx<double> *)ctx->Ez, ctx->bessel_f, ctx->wakeE, emc, tile_red,   \
-                     tile_bb, t0, ctx->errw)
+                     tile_bb, ctx->errw)
   if (uy) {
     if (lm) PSTEP_LAUNCH(true, true);
     else PSTEP_LAUNCH(true, false);
@@ -243,9 +243,9 @@ int launch_deposit_slot(felis_ctx *ctx, const DepArgs &dep, int lds, bool tile_b
     default: return FELIS_ERR_UNSUPPORTED;       \
   }
 
-int particles_range(felis_ctx *ctx, const PStepArgs &a, double *tile_red, void *Ufield, int4 *tile_bb, int t0,
-                    int t1, bool uy, hipStream_t st) {
-  NH_DISPATCH(launch_particles, ctx->d.n_harm, ctx, a, tile_red, Ufield, tile_bb, t0, t1, uy, st)
+int particles_all(felis_ctx *ctx, const PStepArgs &a, double *tile_red, void *Ufield, int4 *tile_bb, bool uy,
+                  hipStream_t st) {
+  NH_DISPATCH(launch_particles, ctx->d.n_harm, ctx, a, tile_red, Ufield, tile_bb, uy, st)
 }
 int deposit_range(felis_ctx *ctx, const DepArgs &dep, int lds, bool tile_boxes, int s0, int s1, bool uy,
                   hipStream_t st) {
@@ -274,7 +274,7 @@ int particles_and_deposit(felis_ctx *ctx, const PStepArgs &a, double *tile_red, 
     if (d.sc_l > 1) em_carry_mixed<true>(ctx, g, a);
     else em_carry_mixed<false>(ctx, g, a);
   }
-  int e = particles_range(ctx, a, tile_red, Ufield, ctx->tile_bb, 0, ctx->ntiles, uy_push, ctx->stream);
+  int e = particles_all(ctx, a, tile_red, Ufield, ctx->tile_bb, uy_push, ctx->stream);
   ctx->dz_mul = 1.0;
   if (e) return e;
   return dep ? deposit_range(ctx, *dep, lds, true, 0, d.slice_n, uy_dep, ctx->stream) : FELIS_OK;
@@ -437,6 +437,8 @@ int pstep_args(felis_ctx *ctx, const felis_lattice *l0, const felis_lattice *l1,
   a.l1.und_type = l1->und_type;
   a.t0 = trans_args(a.l0);
   a.t1 = trans_args(a.l1);
+  a.trans_same = a.t0.aa == a.t1.aa && a.t0.bb == a.t1.bb && a.t0.cc == a.t1.cc && a.t0.ddc == a.t1.ddc &&
+                 a.t0.ue == a.t1.ue && a.t0.xCoef == a.t1.xCoef && a.t0.yCoef == a.t1.yCoef && a.t0.ka == a.t1.ka;
   if (!l1->drift_flag) {
     const double K0 = a.l1.K0, ue = a.l1.ue, ku = a.l1.ku;
     a.K2a = (1.0 + ue * ue) * K0 * K0 * 0.5;
