@@ -290,6 +290,33 @@ def test_run_mixed_window_from_the_push_tile_boxes(mode):
     assert np.max(np.abs(bg - b_o[:, :2])) < 1e-3
 
 
+def test_run_mixed_beam_partly_outside_the_grid():
+    """A beam wider than the field grid (rms 110 um in a +-300 um box): the
+    particles outside the grid are pushed without field (particlePush.cpp:
+    254-257 skips them) and deposit nothing (solveField.cpp:273), and the
+    push's tile boxes leave them out of the deposit windows. 12 periods of
+    the batched loop against the oracle, tolerances as in
+    test_run_mixed_matches_oracle."""
+    case = fc.make_case(slice_n=4, harmonics=(1, 3), ppslice=512, nx=33, ny=31, sig=110e-6, in_phases=1)
+    o = case.oracle_state()
+    p = o.params
+    ix = np.trunc((o.x - p.min_x) / p.dx)
+    iy = np.trunc((o.y - p.min_y) / p.dy)
+    outside = ~((ix >= 0) & (ix < p.nx - 1) & (iy >= 0) & (iy < p.ny - 1))
+    assert 0.005 < outside.mean() < 0.3
+    nsteps = 12
+    tab = [case.lattice(current_flag=1)] + [case.lattice() for _ in range(2 * nsteps)]
+    rc, e_o, b_o, t_o = o.run(tab, 0, nsteps, True)
+    assert rc == 0
+    ctx = fc.gpu_context(case, precision=fl.MIXED_FP32)
+    e_g, b_g, t_g = ctx.run([fc.to_lib_lattice(l) for l in tab], 0, nsteps)
+    tot_g, tot_o = e_g[:, :2].sum(1), e_o[:, :2].sum(1)
+    assert np.max(np.abs(tot_g - tot_o) / tot_o) < 1e-4
+    bg = np.abs(b_g[:, 0:4:2] + 1j * b_g[:, 1:4:2]) / b_g[:, 4:5]
+    assert np.max(np.abs(bg - b_o[:, :2])) < 1e-3
+    assert fc.rel_err(t_g, t_o) < 1e-4
+
+
 @pytest.mark.parametrize("n", [33, 151, 171], ids=["fused_adi", "split_lds_adi", "two_pass_adi"])
 def test_run_mixed_lattice_and_wake(n):
     """The batched loop through the features of the multi-undulator workload
