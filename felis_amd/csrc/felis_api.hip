@@ -1034,13 +1034,19 @@ int felis_ctx::solve_field(const felis_lattice *lat) {
 template <typename T>
 static int launch_energy(felis_ctx *ctx, double *dev_out) {
   GridArgs g = ctx->grid();
-  int nsys = 2 * ctx->d.n_harm * ctx->d.slice_n;
+  // the batched path keeps a dead y polarisation exactly zero (felis_ctx::lat_uy):
+  // its slice energies are zero without a pass over it
+  const int npol = ctx->mixed() ? ctx->npol() : 2, nsl = ctx->d.n_harm * ctx->d.slice_n;
+  if (npol < 2) {
+    int e = zero_async(ctx, ctx->slice_e + nsl, (size_t)nsl * sizeof(double));
+    if (e) return e;
+  }
   KTimer kt(ctx, FELIS_K_ENERGY);
-  hipLaunchKernelGGL((k_energy_slices<T>), dim3(nsys), dim3(256), 0, ctx->stream, g, (const cpx<T> *)ctx->U,
+  hipLaunchKernelGGL((k_energy_slices<T>), dim3(npol * nsl), dim3(256), 0, ctx->stream, g, (const cpx<T> *)ctx->U,
                      ctx->slice_e);
   LAUNCHCHK();
-  hipLaunchKernelGGL(k_sum_rows, dim3(2 * ctx->d.n_harm), dim3(64), 0, ctx->stream, ctx->slice_e, ctx->d.slice_n,
-                     ctx->d.slice_n, 1, dev_out);
+  hipLaunchKernelGGL(k_sum_rows, dim3(2 * ctx->d.n_harm), dim3(kSumRowThreads), 0, ctx->stream, ctx->slice_e,
+                     ctx->d.slice_n, ctx->d.slice_n, 1, dev_out);
   LAUNCHCHK();
   return FELIS_OK;
 }
@@ -1055,10 +1061,12 @@ static int launch_bfactor(felis_ctx *ctx, double *dev_out) {
   int nv = 2 * ctx->d.n_harm + 1;
   if (!ctx->ntiles) return zero_async(ctx, dev_out, nv * sizeof(double));
   KTimer kt(ctx, FELIS_K_BFACTOR);
-  hipLaunchKernelGGL((k_bfactor_tiles<R>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, g, ctx->tiles(),
-                     ctx->theta, ctx->seg_w, ctx->tile_red);
+  hipLaunchKernelGGL((k_bfactor_tiles<R>), dim3((ctx->ntiles + kTilesPerBlock - 1) / kTilesPerBlock),
+                     dim3(64 * kTilesPerBlock), 0, ctx->stream, g, ctx->tiles(), ctx->ntiles, ctx->theta, ctx->seg_w,
+                     ctx->tile_red);
   LAUNCHCHK();
-  hipLaunchKernelGGL(k_sum_rows, dim3(nv), dim3(64), 0, ctx->stream, ctx->tile_red, ctx->ntiles, 1, nv, dev_out);
+  hipLaunchKernelGGL(k_sum_rows, dim3(nv), dim3(kSumRowThreads), 0, ctx->stream, ctx->tile_red, ctx->ntiles, 1, nv,
+                     dev_out);
   LAUNCHCHK();
   return FELIS_OK;
 }
@@ -1074,14 +1082,15 @@ static int launch_twiss(felis_ctx *ctx, double *dev_out) {
   int e0 = zero_async(ctx, dev_out, 7 * d.n_species * sizeof(double));
   if (e0 || !ctx->ntiles) return e0;
   KTimer kt(ctx, FELIS_K_TWISS);
-  hipLaunchKernelGGL((k_twiss_tiles<R>), dim3(ctx->ntiles), dim3(kTile), 0, ctx->stream, g, ctx->tiles(),
-                     (const R *)ctx->x, (const R *)ctx->y, (const R *)ctx->px, (const R *)ctx->py,
+  hipLaunchKernelGGL((k_twiss_tiles<R>), dim3((ctx->ntiles + kTilesPerBlock - 1) / kTilesPerBlock),
+                     dim3(64 * kTilesPerBlock), 0, ctx->stream, g, ctx->tiles(), ctx->ntiles, (const R *)ctx->x, (const R *)ctx->y, (const R *)ctx->px, (const R *)ctx->py,
                      (const R *)ctx->gamma, ctx->tile_red);
   LAUNCHCHK();
   for (int s = 0; s < d.n_species; ++s) {
     int t0 = ctx->species_tile[s], t1 = ctx->species_tile[s + 1];
     if (t1 <= t0) continue;
-    hipLaunchKernelGGL(k_sum_rows, dim3(7), dim3(64), 0, ctx->stream, ctx->tile_red + (size_t)t0 * 7, t1 - t0, 1, 7,
+    hipLaunchKernelGGL(k_sum_rows, dim3(7), dim3(kSumRowThreads), 0, ctx->stream, ctx->tile_red + (size_t)t0 * 7,
+                       t1 - t0, 1, 7,
                        dev_out + 7 * s);
     LAUNCHCHK();
   }

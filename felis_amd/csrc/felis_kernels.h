@@ -946,67 +946,125 @@ __global__ void __launch_bounds__(256) k_energy_slices(GridArgs g, const cpx<T> 
   if (threadIdx.x == 0) out[sys] = s;
 }
 
-// rows of a [nrow][ncol] matrix summed in column order by one wave per row
-static __global__ void __launch_bounds__(64) k_sum_rows(const double *__restrict__ in, int ncol, int stride_row,
-                                                 int stride_col, double *__restrict__ out) {
-  int r = blockIdx.x;
-  double acc = 0.0;
-  for (int c = threadIdx.x; c < ncol; c += 64) acc += in[(long long)r * stride_row + (long long)c * stride_col];
-  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o);
-  if (threadIdx.x == 0) out[r] = acc;
+// rows of a [nrow][ncol] matrix, one workgroup of kSumRowThreads per row in a
+// fixed order: strided partial sums in four independent chains per thread
+// (the loads of a chain do not wait for each other's sums), then a tree
+constexpr int kSumRowThreads = 256;
+static __global__ void __launch_bounds__(kSumRowThreads) k_sum_rows(const double *__restrict__ in, int ncol,
+                                                                    int stride_row, int stride_col,
+                                                                    double *__restrict__ out) {
+  constexpr int NT = kSumRowThreads;
+  __shared__ double sh[NT];
+  const int tid = threadIdx.x;
+  const double *row = in + (long long)blockIdx.x * stride_row;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int c = tid;
+  for (; c + 3 * NT < ncol; c += 4 * NT) {
+    s0 += row[(long long)c * stride_col];
+    s1 += row[(long long)(c + NT) * stride_col];
+    s2 += row[(long long)(c + 2 * NT) * stride_col];
+    s3 += row[(long long)(c + 3 * NT) * stride_col];
+  }
+  for (; c < ncol; c += NT) s0 += row[(long long)c * stride_col];
+  sh[tid] = (s0 + s1) + (s2 + s3);
+  __syncthreads();
+  for (int w = NT / 2; w > 0; w >>= 1) {
+    if (tid < w) sh[tid] += sh[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) out[blockIdx.x] = sh[0];
 }
 
-// bFactor partials per tile: [tile][2*nh + 1]
+// One wave per tile for the per-tile diagnostics below: lane l takes the
+// tile's particles l, l+64, ..., and the wave sums each value by a fixed
+// shuffle tree (no LDS, no barrier; a quarter of the shuffles of a
+// 256-thread block per tile).
+constexpr int kTilesPerBlock = 4;
+__device__ inline double wave_sum(double a) {
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o);
+  return a;
+}
+
+// b^n for a unit complex b by binary powers (n < 0: the conjugate's)
+__device__ inline cpx<double> cpow_int_d(cpx<double> b, int n) {
+  if (n < 0) {
+    b = conjg(b);
+    n = -n;
+  }
+  cpx<double> r = mk(1.0, 0.0);
+  while (n) {
+    if (n & 1) r = r * b;
+    b = b * b;
+    n >>= 1;
+  }
+  return r;
+}
+
+// bFactor partials per tile: [tile][2*nh + 1]. e^{iH theta} from one FP64
+// sincos of theta by products (binary powers, then the steps between the
+// harmonics), a few ulp from sincos(H theta) directly.
 template <typename R>
-__global__ void __launch_bounds__(kTile) k_bfactor_tiles(GridArgs g, Tiles tl,
-                                                         const double *__restrict__ theta,
-                                                         const double *__restrict__ seg_w,
-                                                         double *__restrict__ out) {
-  __shared__ double sh[4];
-  int t = blockIdx.x;
-  int k = threadIdx.x;
-  int seg = tl.seg[t];
-  double w = seg_w[seg];
-  bool on = k < tl.cnt[t];
-  double th = on ? theta[tl.start[t] + k] : 0.0;
-  const int nv = 2 * g.nh + 1;
-  for (int h = 0; h < g.nh; ++h) {
-    double s = 0, c = 0;
-    if (on) sincos((double)g.harm[h] * th, &s, &c);
-    double vr = block_sum<kTile>(on ? c * w : 0.0, sh);
-    double vi = block_sum<kTile>(on ? s * w : 0.0, sh);
-    if (k == 0) {
-      out[(long long)t * nv + 2 * h] = vr;
-      out[(long long)t * nv + 2 * h + 1] = vi;
+__global__ void __launch_bounds__(64 * kTilesPerBlock) k_bfactor_tiles(GridArgs g, Tiles tl, int ntiles,
+                                                                       const double *__restrict__ theta,
+                                                                       const double *__restrict__ seg_w,
+                                                                       double *__restrict__ out) {
+  const int t = blockIdx.x * kTilesPerBlock + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (t >= ntiles) return;  // whole waves
+  const int nh = g.nh, nv = 2 * nh + 1, cnt = tl.cnt[t];
+  const long long st = tl.start[t];
+  const double w = seg_w[tl.seg[t]];
+  double v[2 * FELIS_MAX_HARM];
+#pragma unroll
+  for (int q = 0; q < 2 * FELIS_MAX_HARM; ++q) v[q] = 0.0;
+  for (int k = lane; k < cnt; k += 64) {
+    double s, c;
+    sincos(theta[st + k], &s, &c);
+    const cpx<double> e1 = mk(c, s);
+    cpx<double> pw = cpow_int_d(e1, g.harm[0]);
+#pragma unroll
+    for (int h = 0; h < FELIS_MAX_HARM; ++h) {
+      if (h >= nh) break;
+      if (h > 0) pw = pw * cpow_int_d(e1, g.harm[h] - g.harm[h - 1]);
+      v[2 * h] += pw.re * w;
+      v[2 * h + 1] += pw.im * w;
     }
   }
-  double wt = block_sum<kTile>(on ? w : 0.0, sh);
-  if (k == 0) out[(long long)t * nv + 2 * g.nh] = wt;
+  double* o = out + (long long)t * nv;
+#pragma unroll
+  for (int q = 0; q < 2 * FELIS_MAX_HARM; ++q) {
+    if (q >= 2 * nh) break;
+    const double r = wave_sum(v[q]);
+    if (lane == 0) o[q] = r;
+  }
+  if (lane == 0) o[2 * nh] = w * cnt;
 }
 
 // twiss partials per tile: [tile][7]
 template <typename R>
-__global__ void __launch_bounds__(kTile) k_twiss_tiles(GridArgs g, Tiles tl, const R *__restrict__ x,
-                                                       const R *__restrict__ y, const R *__restrict__ px,
-                                                       const R *__restrict__ py,
-                                                       const R *__restrict__ gamma,
-                                                       double *__restrict__ out) {
-  __shared__ double sh[4];
-  int t = blockIdx.x;
-  int k = threadIdx.x;
-  bool on = k < tl.cnt[t];
-  double v[7] = {0, 0, 0, 0, 0, 0, 0};
-  if (on) {
-    long long n = tl.start[t] + k;
-    double xx = x[n], yy = y[n], inv = 1.0 / ((double)gamma[n] + g.gref);
-    double xp = (double)px[n] * inv, yp = (double)py[n] * inv;
-    v[0] = xx * xx; v[1] = yy * yy; v[2] = xp * xp; v[3] = yp * yp;
-    v[4] = xx * xp; v[5] = yy * yp; v[6] = 1.0;
+__global__ void __launch_bounds__(64 * kTilesPerBlock) k_twiss_tiles(GridArgs g, Tiles tl, int ntiles,
+                                                                     const R *__restrict__ x,
+                                                                     const R *__restrict__ y, const R *__restrict__ px,
+                                                                     const R *__restrict__ py,
+                                                                     const R *__restrict__ gamma,
+                                                                     double *__restrict__ out) {
+  const int t = blockIdx.x * kTilesPerBlock + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (t >= ntiles) return;  // whole waves
+  const int cnt = tl.cnt[t];
+  const long long st = tl.start[t];
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = lane; k < cnt; k += 64) {
+    const long long n = st + k;
+    const double xx = x[n], yy = y[n], inv = 1.0 / ((double)gamma[n] + g.gref);
+    const double xp = (double)px[n] * inv, yp = (double)py[n] * inv;
+    v[0] += xx * xx; v[1] += yy * yy; v[2] += xp * xp; v[3] += yp * yp;
+    v[4] += xx * xp; v[5] += yy * yp;
   }
-  for (int q = 0; q < 7; ++q) {
-    double s = block_sum<kTile>(v[q], sh);
-    if (k == 0) out[(long long)t * 7 + q] = s;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const double r = wave_sum(v[q]);
+    if (lane == 0) out[(long long)t * 7 + q] = r;
   }
+  if (lane == 0) out[(long long)t * 7 + 6] = (double)cnt;
 }
 
 // wakeField.cpp:28-35: wakeE[ii] = sum_{i>=ii} den[i] wakeF[i-ii] dZ coef,
