@@ -600,7 +600,9 @@ void fno_backward(felis_fno *n, int B) {
 // is built for: width 32, 8 modes, projection 64, at most 16 outputs
 bool fno_fusable(const felis_fno *n) { return n->w == 32 && n->m == 8 && n->ph == 64 && n->cout <= 16; }
 
-void fno_forward_fused(felis_fno *n, int B, const float *X) {
+// X: FP32 input maps (the input affine applied on load), or XB: the BF16
+// bits of the affine-applied inputs
+void fno_forward_fused(felis_fno *n, int B, const float *X, const uint16_t *XB = nullptr) {
   const hipStream_t st = n->stream;
   constexpr int w = 32, m = 8;
   const int lw = lg2(w);
@@ -611,8 +613,12 @@ void fno_forward_fused(felis_fno *n, int B, const float *X) {
   uint16_t *hA = reinterpret_cast<uint16_t *>(n->h[0]), *hB = reinterpret_cast<uint16_t *>(n->a[0]);
   uint16_t *Xf = reinterpret_cast<uint16_t *>(n->Xf), *Yb = reinterpret_cast<uint16_t *>(n->Y[0]),
            *Zb = reinterpret_cast<uint16_t *>(n->Zs), *Xb = reinterpret_cast<uint16_t *>(n->Xb);
-  hipLaunchKernelGGL((k_fno_row<32, 16, float>), dim3(nb), dim3(256), 0, st, rows, n->cin, X, n->mu, n->isd,
-                     n->P + n->o_lw, n->P + n->o_lb, nullptr, nullptr, n->Fx, 0, hA, Xf);
+  if (XB)
+    hipLaunchKernelGGL((k_fno_row<32, 16, uint16_t>), dim3(nb), dim3(256), 0, st, rows, n->cin, XB, nullptr, nullptr,
+                       n->P + n->o_lw, n->P + n->o_lb, nullptr, nullptr, n->Fx, 0, hA, Xf);
+  else
+    hipLaunchKernelGGL((k_fno_row<32, 16, float>), dim3(nb), dim3(256), 0, st, rows, n->cin, X, n->mu, n->isd,
+                       n->P + n->o_lw, n->P + n->o_lb, nullptr, nullptr, n->Fx, 0, hA, Xf);
   for (int l = 0; l < n->L; ++l) {
     bgemm<float, uint16_t, uint16_t>(st, B, 4 * m, m * w, 2 * kN, Op{n->Gy, U1(0), U1(2 * kN), U1(1)},
                                      Op{(const float *)Xf, U1((long long)kN * sX), U1((long long)m * w), U1(1)},
@@ -665,6 +671,29 @@ int felis_fno_forward_dev(felis_fno *n, int B, const float *X, const float **out
   (void)hipEventDestroy(ev);
   if (out) *out = n->out;
   return e ? e : (hipGetLastError() == hipSuccess ? FELIS_OK : FELIS_ERR_HIP);
+}
+
+// forward of B device maps given as the BF16 bits of the affine-applied
+// inputs (the fused shape only), as felis_fno_forward_dev
+int felis_fno_forward_bf16_dev(felis_fno *n, int B, const uint16_t *XB, const float **out, hipStream_t st) {
+  if (!n || B < 1 || B > n->cap || !XB || !fno_fusable(n)) return FELIS_ERR_ARG;
+  hipEvent_t ev;
+  if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return FELIS_ERR_HIP;
+  (void)hipEventRecord(ev, st);
+  (void)hipStreamWaitEvent(n->stream, ev, 0);
+  fno_forward_fused(n, B, nullptr, XB);
+  (void)hipEventRecord(ev, n->stream);
+  (void)hipStreamWaitEvent(st, ev, 0);
+  (void)hipEventDestroy(ev);
+  if (out) *out = n->out;
+  return hipGetLastError() == hipSuccess ? FELIS_OK : FELIS_ERR_HIP;
+}
+
+int felis_fno_input_affine_dev(const felis_fno *n, const float **mu, const float **isd) {
+  if (!n) return FELIS_ERR_ARG;
+  if (mu) *mu = n->mu;
+  if (isd) *isd = n->isd;
+  return fno_fusable(n) ? FELIS_OK : FELIS_ERR_UNSUPPORTED;
 }
 
 int felis_fno_channels(const felis_fno *n, int *cin, int *cout) {
@@ -1009,6 +1038,219 @@ __global__ void __launch_bounds__(256) k_fno_coarse(GridArgs g, FnoGeom fg, int 
   }
 }
 
+// The inference path's maps in one pass: one workgroup per (slice,
+// harmonic) forms the cells' E_h and U_h exactly as k_fno_coarse does (J = 0:
+// the field at the slot the source was solved at), their sums as there,
+// then writes the network's input channels of this harmonic (and, for the
+// first, the slice's 12 scalar channels) as k_fno_inputs would, with the
+// input affine applied and rounded to BF16 as the lift rounds them: the
+// BF16 bits the lift would form from k_fno_inputs' FP32 maps. BB: field cells
+// per map cell (2 unrolled; 0: fg.b).
+template <int BB>
+__global__ void __launch_bounds__(256) k_fno_maps_x(GridArgs g, FnoGeom fg, const cpx<float> *__restrict__ A,
+                                                    const cpx<float> *__restrict__ Sw, const int4 *__restrict__ win2,
+                                                    const float *__restrict__ feat, int nfeat,
+                                                    const float *__restrict__ mu, const float *__restrict__ isd,
+                                                    uint16_t *__restrict__ X) {
+  constexpr int NQ = kP / 256;  // map cells per thread
+  __shared__ double red[2][4];
+  __shared__ float scs[2];
+  const int sl = blockIdx.x, h = blockIdx.y, nh = g.nh, cin = 4 * nh + kFnoScalars;
+  const int b = BB > 0 ? BB : fg.b;
+  const int s_src = sl + 1;
+  const float k2 = 2.f / (float)g.harm[h];
+  const int4 w = win2[s_src];
+  const cpx<float> *src = Sw + ((long long)h * g.S + s_src) * g.N;
+  const long long fo = ((long long)h * g.S + phys_slot(s_src, g.ring_base, g.S)) * g.N;
+  const float inv = 1.f / (float)(b * b);
+  cpx<float> ev[NQ], uv[NQ];
+  double se = 0.0, su = 0.0;
+#pragma unroll
+  for (int qi = 0; qi < NQ; ++qi) {
+    const int q = threadIdx.x + 256 * qi;
+    const int I = q % kN, Jc = q / kN;
+    cpx<float> e = mk(0.f, 0.f), u = mk(0.f, 0.f);
+#pragma unroll
+    for (int dj = 0; dj < (BB > 0 ? BB : 64); ++dj) {
+      if (BB == 0 && dj >= b) break;
+#pragma unroll
+      for (int di = 0; di < (BB > 0 ? BB : 64); ++di) {
+        if (BB == 0 && di >= b) break;
+        const int i = fg.off + b * I + di, j = fg.off + b * Jc + dj;
+        const long long c = (long long)j * g.nx + i;
+        const cpx<float> a = A[fo + c];
+        if (i >= w.x && i <= w.y && j >= w.z && j <= w.w) e = e + src[c] * k2;
+        u = u + a;
+      }
+    }
+    e = e * inv;
+    u = u * inv;
+    ev[qi] = e;
+    uv[qi] = u;
+    se += (double)e.re * e.re + (double)e.im * e.im;
+    su += (double)u.re * u.re + (double)u.im * u.im;
+  }
+  double v[2] = {se, su};
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off);
+    if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    const int k = threadIdx.x;
+    scs[k] = (float)((red[k][0] + red[k][1]) + (red[k][2] + red[k][3]));
+  }
+  __syncthreads();
+  // as k_fno_inputs: 1 / rms, the maps scaled (each product rounded to FP32
+  // before the affine, as it was stored there)
+  const float sE = sqrtf(scs[0] / kP), sU = sqrtf(scs[1] / kP);
+  const float is = sE > 0.f ? 1.f / sE : 0.f, iu = sU > 0.f ? 1.f / sU : 0.f;
+  auto bf = [&](int ch, float x) {
+    const float m0 = mu ? mu[ch] : 0.f, s0 = isd ? isd[ch] : 1.f;
+    return (uint16_t)f2bf(__fmul_rn(__fsub_rn(x, m0), s0));
+  };
+#pragma unroll
+  for (int qi = 0; qi < NQ; ++qi) {
+    const int q = threadIdx.x + 256 * qi;
+    uint16_t *x = X + ((long long)sl * kP + q) * cin;
+    x[2 * h] = bf(2 * h, __fmul_rn(ev[qi].re, is));
+    x[2 * h + 1] = bf(2 * h + 1, __fmul_rn(ev[qi].im, is));
+    x[2 * nh + 2 * h] = bf(2 * nh + 2 * h, __fmul_rn(uv[qi].re, iu));
+    x[2 * nh + 2 * h + 1] = bf(2 * nh + 2 * h + 1, __fmul_rn(uv[qi].im, iu));
+    if (h == 0) {
+      // the slice's scalars (k_fno_inputs' order): 8 features, log10 of the
+      // first harmonic's rms emission and field, the cell's coordinates
+      const float *f = feat + (long long)sl * nfeat;
+      const int c0 = 4 * nh;
+      x[c0 + 0] = bf(c0 + 0, f[0]);
+      x[c0 + 1] = bf(c0 + 1, f[1]);
+      x[c0 + 2] = bf(c0 + 2, f[2]);
+      x[c0 + 3] = bf(c0 + 3, f[3]);
+      x[c0 + 4] = bf(c0 + 4, nh > 1 ? f[2 + 5] : f[2]);
+      x[c0 + 5] = bf(c0 + 5, nh > 2 ? f[2 + 10] : f[2]);
+      x[c0 + 6] = bf(c0 + 6, f[1 + 5 * nh]);
+      x[c0 + 7] = bf(c0 + 7, f[2 + 5 * nh]);
+      x[c0 + 8] = bf(c0 + 8, log10f(sqrtf(scs[0] / kP) + 1e-30f));
+      x[c0 + 9] = bf(c0 + 9, log10f(sqrtf(scs[1] / kP) + 1e-30f));
+      x[c0 + 10] = bf(c0 + 10, (2.f * (q % kN) + 1.f) / kN - 1.f);
+      x[c0 + 11] = bf(c0 + 11, (2.f * (q / kN) + 1.f) / kN - 1.f);
+    }
+  }
+}
+
+// The same for NH harmonics at once (cin = 4 NH + 12 BF16 channels, a 16-byte
+// multiple for NH = 1 and 3: 48 bytes a cell at NH = 3), one workgroup of
+// 1024 threads per slice: each thread forms its 4 cells' maps of every
+// harmonic, the sums go through LDS in a fixed order, and each cell's
+// channels leave in 16-byte stores.
+template <int NH>
+__global__ void __launch_bounds__(1024) k_fno_maps_xs(GridArgs g, FnoGeom fg, const cpx<float> *__restrict__ A,
+                                                      const cpx<float> *__restrict__ Sw,
+                                                      const int4 *__restrict__ win2, const float *__restrict__ feat,
+                                                      int nfeat, const float *__restrict__ mu,
+                                                      const float *__restrict__ isd, uint16_t *__restrict__ X) {
+  constexpr int NT = 1024, NQ = kP / NT, CIN = 4 * NH + kFnoScalars, NW = NT / 64;
+  static_assert(CIN % 8 == 0, "16-byte records");
+  __shared__ double red[2 * NH][NW];
+  __shared__ float scs[2 * NH];
+  const int sl = blockIdx.x, b = fg.b;
+  const int s_src = sl + 1;
+  const int4 w = win2[s_src];
+  const float inv = 1.f / (float)(b * b);
+  cpx<float> ev[NQ][NH], uv[NQ][NH];
+  double se[NH], su[NH];
+#pragma unroll
+  for (int h = 0; h < NH; ++h) {
+    se[h] = su[h] = 0.0;
+    const float k2 = 2.f / (float)g.harm[h];
+    const cpx<float> *src = Sw + ((long long)h * g.S + s_src) * g.N;
+    const cpx<float> *fld = A + ((long long)h * g.S + phys_slot(s_src, g.ring_base, g.S)) * g.N;
+#pragma unroll
+    for (int qi = 0; qi < NQ; ++qi) {
+      const int q = threadIdx.x + NT * qi;
+      const int I = q % kN, Jc = q / kN;
+      cpx<float> e = mk(0.f, 0.f), u = mk(0.f, 0.f);
+      for (int dj = 0; dj < b; ++dj)
+        for (int di = 0; di < b; ++di) {
+          const int i = fg.off + b * I + di, j = fg.off + b * Jc + dj;
+          const long long c = (long long)j * g.nx + i;
+          const cpx<float> a = fld[c];
+          if (i >= w.x && i <= w.y && j >= w.z && j <= w.w) e = e + src[c] * k2;
+          u = u + a;
+        }
+      e = e * inv;
+      u = u * inv;
+      ev[qi][h] = e;
+      uv[qi][h] = u;
+      se[h] += (double)e.re * e.re + (double)e.im * e.im;
+      su[h] += (double)u.re * u.re + (double)u.im * u.im;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 2 * NH; ++k) {
+    double v = k < NH ? se[k] : su[k - NH];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * NH) {
+    double t = 0.0;
+    for (int k = 0; k < NW; ++k) t += red[threadIdx.x][k];
+    scs[threadIdx.x] = (float)t;
+  }
+  __syncthreads();
+  float is[NH], iu[NH];
+#pragma unroll
+  for (int h = 0; h < NH; ++h) {
+    const float sE = sqrtf(scs[h] / kP), sU = sqrtf(scs[NH + h] / kP);
+    is[h] = sE > 0.f ? 1.f / sE : 0.f;
+    iu[h] = sU > 0.f ? 1.f / sU : 0.f;
+  }
+  const float *f = feat + (long long)sl * nfeat;
+  float sc[kFnoScalars];
+  sc[0] = f[0];
+  sc[1] = f[1];
+  sc[2] = f[2];
+  sc[3] = f[3];
+  sc[4] = NH > 1 ? f[2 + 5] : f[2];
+  sc[5] = NH > 2 ? f[2 + 10] : f[2];
+  sc[6] = f[1 + 5 * NH];
+  sc[7] = f[2 + 5 * NH];
+  sc[8] = log10f(sqrtf(scs[0] / kP) + 1e-30f);
+  sc[9] = log10f(sqrtf(scs[NH] / kP) + 1e-30f);
+  auto bf = [&](int ch, float x) {
+    const float m0 = mu ? mu[ch] : 0.f, s0 = isd ? isd[ch] : 1.f;
+    return (unsigned)(uint16_t)f2bf(__fmul_rn(__fsub_rn(x, m0), s0));
+  };
+#pragma unroll
+  for (int qi = 0; qi < NQ; ++qi) {
+    const int q = threadIdx.x + NT * qi;
+    float x[CIN];
+#pragma unroll
+    for (int h = 0; h < NH; ++h) {
+      x[2 * h] = __fmul_rn(ev[qi][h].re, is[h]);
+      x[2 * h + 1] = __fmul_rn(ev[qi][h].im, is[h]);
+      x[2 * NH + 2 * h] = __fmul_rn(uv[qi][h].re, iu[h]);
+      x[2 * NH + 2 * h + 1] = __fmul_rn(uv[qi][h].im, iu[h]);
+    }
+#pragma unroll
+    for (int k = 0; k < 10; ++k) x[4 * NH + k] = sc[k];
+    x[4 * NH + 10] = (2.f * (q % kN) + 1.f) / kN - 1.f;
+    x[4 * NH + 11] = (2.f * (q / kN) + 1.f) / kN - 1.f;
+    uint4 *o = reinterpret_cast<uint4 *>(X + ((long long)sl * kP + q) * CIN);
+#pragma unroll
+    for (int v = 0; v < CIN / 8; ++v) {
+      uint4 pk;
+      pk.x = bf(8 * v + 0, x[8 * v + 0]) | (bf(8 * v + 1, x[8 * v + 1]) << 16);
+      pk.y = bf(8 * v + 2, x[8 * v + 2]) | (bf(8 * v + 3, x[8 * v + 3]) << 16);
+      pk.z = bf(8 * v + 4, x[8 * v + 4]) | (bf(8 * v + 5, x[8 * v + 5]) << 16);
+      pk.w = bf(8 * v + 6, x[8 * v + 6]) | (bf(8 * v + 7, x[8 * v + 7]) << 16);
+      o[v] = pk;
+    }
+  }
+}
+
 // the network's inputs [sl][P][cin] and the normalised E (and R) in place
 __global__ void k_fno_inputs(int slice_n, int nh, const float *__restrict__ scal, const float *__restrict__ feat,
                              int nfeat, float *__restrict__ Ec, const float *__restrict__ Uc, float *__restrict__ Rc,
@@ -1116,7 +1358,7 @@ int felis_fno_maps_ok(const felis_ctx *ctx) {
 
 size_t felis_fno_work_floats(const felis_ctx *ctx) {
   const size_t sn = ctx->d.slice_n, nh = ctx->d.n_harm;
-  return sn * kP * 2 * nh * 2 + sn * 3 * nh + sn * kP * (4 * nh + felis::kFnoScalars);
+  return sn * kP * 2 * nh * 2 + ((sn * 3 * nh + 3) & ~(size_t)3) + sn * kP * (4 * nh + felis::kFnoScalars);
 }
 
 // The learned correction of a jump in the run (after its push and deposit,
@@ -1132,17 +1374,42 @@ int felis_fno_correct_dev(felis_ctx *ctx, felis_fno *net, const float *feat, flo
     ctx->err = "FNO channels do not match the harmonics";
     return FELIS_ERR_ARG;
   }
+  // (X 16-byte aligned for the records of k_fno_maps_xs)
   float *Ec = work, *Uc = Ec + (size_t)sn * kP * 2 * nh, *scal = Uc + (size_t)sn * kP * 2 * nh,
-        *X = scal + (size_t)sn * 3 * nh;
+        *X = scal + (((size_t)sn * 3 * nh + 3) & ~(size_t)3);
   felis::GridArgs g = ctx->grid();
   const FnoGeom fg = felis::fno_geom(d);
-  hipLaunchKernelGGL(felis::k_fno_coarse, dim3(sn, nh), dim3(256), 0, ctx->stream, g, fg, 0,
-                     (const felis::cpx<float> *)ctx->U, nullptr, (const felis::cpx<float> *)ctx->Sw, ctx->win2, Ec, Uc,
-                     nullptr, scal);
-  hipLaunchKernelGGL(felis::k_fno_inputs, dim3(nblk((long long)sn * kP)), dim3(256), 0, ctx->stream, sn, nh, scal, feat,
-                     felis::features_per_slice(nh), Ec, Uc, nullptr, X);
   const float *o = nullptr;
-  int e = felis_fno_forward_dev(net, sn, X, &o, ctx->stream);
+  const float *mu = nullptr, *isd = nullptr;
+  int e;
+  if (felis_fno_input_affine_dev(net, &mu, &isd) == FELIS_OK) {
+    // the maps straight into the lift's BF16 inputs (in X's space)
+    uint16_t *XB = reinterpret_cast<uint16_t *>(X);
+    if (nh == 3)
+      hipLaunchKernelGGL((felis::k_fno_maps_xs<3>), dim3(sn), dim3(1024), 0, ctx->stream, g, fg,
+                         (const felis::cpx<float> *)ctx->U, (const felis::cpx<float> *)ctx->Sw, ctx->win2, feat,
+                         felis::features_per_slice(nh), mu, isd, XB);
+    else if (nh == 1)
+      hipLaunchKernelGGL((felis::k_fno_maps_xs<1>), dim3(sn), dim3(1024), 0, ctx->stream, g, fg,
+                         (const felis::cpx<float> *)ctx->U, (const felis::cpx<float> *)ctx->Sw, ctx->win2, feat,
+                         felis::features_per_slice(nh), mu, isd, XB);
+    else if (fg.b == 2)
+      hipLaunchKernelGGL((felis::k_fno_maps_x<2>), dim3(sn, nh), dim3(256), 0, ctx->stream, g, fg,
+                         (const felis::cpx<float> *)ctx->U, (const felis::cpx<float> *)ctx->Sw, ctx->win2, feat,
+                         felis::features_per_slice(nh), mu, isd, XB);
+    else
+      hipLaunchKernelGGL((felis::k_fno_maps_x<0>), dim3(sn, nh), dim3(256), 0, ctx->stream, g, fg,
+                         (const felis::cpx<float> *)ctx->U, (const felis::cpx<float> *)ctx->Sw, ctx->win2, feat,
+                         felis::features_per_slice(nh), mu, isd, XB);
+    e = felis_fno_forward_bf16_dev(net, sn, XB, &o, ctx->stream);
+  } else {
+    hipLaunchKernelGGL(felis::k_fno_coarse, dim3(sn, nh), dim3(256), 0, ctx->stream, g, fg, 0,
+                       (const felis::cpx<float> *)ctx->U, nullptr, (const felis::cpx<float> *)ctx->Sw, ctx->win2, Ec,
+                       Uc, nullptr, scal);
+    hipLaunchKernelGGL(felis::k_fno_inputs, dim3(nblk((long long)sn * kP)), dim3(256), 0, ctx->stream, sn, nh, scal,
+                       feat, felis::features_per_slice(nh), Ec, Uc, nullptr, X);
+    e = felis_fno_forward_dev(net, sn, X, &o, ctx->stream);
+  }
   if (e) {
     ctx->err = "FNO forward";
     return e;
