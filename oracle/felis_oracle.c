@@ -35,8 +35,11 @@
  *       shows that the checks above catch each of them.
  *   orc_push_theta_gamma_1d, orc_solve_sc_1d, orc_solve_field_u_1d
  *                                    -> zero-field slippage, quiet-start
- *       cancellation and U += Sc closed forms; their coupling and source
- *       coefficients are parity unpinned (no reference build covers them).
+ *       cancellation and U += Sc closed forms; the push's field coupling by
+ *       the Madey low-gain curve, the radiation source by the energy the
+ *       push's field work puts into the field, the space-charge modes by the
+ *       1D plasma wavenumber of a modulated cold beam (test_oracle_1d.py,
+ *       with planted mistakes in test_oracle_mutations.py).
  */
 #include "felis_oracle.h"
 

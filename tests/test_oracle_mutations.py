@@ -27,7 +27,8 @@ def _flags() -> list[str]:
     return m.group(1).split()
 
 
-# name: (text in felis_oracle.c, replacement, the pinning test that must fail)
+# name: (text in felis_oracle.c, replacement, the pinning test that must fail[, which
+# occurrence of the text: 0 when omitted])
 MUTATIONS = {
     "adi_last_row_fixed": ("CCv[nx - 2] = H - alP[nx - 2] - al[nx - 2] - 1.0 / CCv[nx - 3];",
                            "CCv[nx - 2] = H - alP[nx - 2] - al[nx - 2] - al[nx - 2] / CCv[nx - 3];",
@@ -83,7 +84,31 @@ MUTATIONS = {
                        "test_push_carries_em_past_the_radial_grid"),
     "abc_amplifying": ("cx[i] = 1.0 + I * sx[i];", "cx[i] = 1.0 - I * sx[i];",
                        "test_absorbing_layer_absorbs_a_beam_and_never_amplifies"),
+    # the 1D push and source (the second occurrence of a line the 3D path shares)
+    "push1d_energy_factor": ("sumG += creal(t) / (2.0 * gam);", "sumG += creal(t) / gam;",
+                             "test_oracle_1d.py::test_push_madey_low_gain_curve_1d", 1),
+    "push1d_phase_term_sign": ("sumTh += creal(I * t) / (1.0 * H);", "sumTh -= creal(I * t) / (1.0 * H);",
+                               "test_oracle_1d.py::test_push_madey_low_gain_curve_1d", 1),
+    "push1d_ez_factor": ("sumEz += 2.0 * creal(Em[l] * cexp(I * (l + 1.0) * th));",
+                         "sumEz += creal(Em[l] * cexp(I * (l + 1.0) * th));",
+                         "test_oracle_1d.py::test_space_charge_plasma_oscillation_1d", 1),
+    "source1d_coupling": ("fx = sign * (J1 - K0_alpha * J2);", "fx = sign * (K0_alpha * J1 - J2);",
+                          "test_oracle_1d.py::test_source_and_push_conserve_energy_1d", 1),
+    "source1d_coef_half": ("(2.0 * ks * K_EMASS * (p->lambda0 * p->num_slice) * area)",
+                           "(4.0 * ks * K_EMASS * (p->lambda0 * p->num_slice) * area)",
+                           "test_oracle_1d.py::test_source_and_push_conserve_energy_1d"),
+    "ez1d_sign": ("+= -1.0 * I * coef / (1.0 + l) * macro", "+= 1.0 * I * coef / (1.0 + l) * macro",
+                  "test_oracle_1d.py::test_space_charge_plasma_oscillation_1d"),
 }
+
+
+def _replace_nth(src: str, old: str, new: str, nth: int) -> str:
+    """src with the nth (0-based) occurrence of old replaced"""
+    at = -1
+    for _ in range(nth + 1):
+        at = src.find(old, at + 1)
+        assert at >= 0, f"occurrence {nth} of the text to mutate is gone from felis_oracle.c"
+    return src[:at] + new + src[at + len(old):]
 
 _RUNNER = """
 import sys
@@ -97,7 +122,8 @@ sys.exit(pytest.main(["-q", "-x", "-p", "no:cacheprovider", "-W", "ignore", {tes
 
 
 def _run_pinning_test(lib: Path, test: str) -> subprocess.CompletedProcess:
-    node = f"{ROOT / 'tests' / 'test_oracle_pinning.py'}::{test}"
+    """test: a test of test_oracle_pinning.py, or file::test for another file"""
+    node = f"{ROOT / 'tests' / test}" if "::" in test else f"{ROOT / 'tests' / 'test_oracle_pinning.py'}::{test}"
     code = _RUNNER.format(root=str(ROOT), tests=str(ROOT / "tests"), lib=str(lib), test=node)
     return subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=str(ROOT), timeout=300)
 
@@ -112,10 +138,10 @@ def _build(d: Path) -> Path:
 
 @pytest.mark.parametrize("name", sorted(MUTATIONS))
 def test_pinning_catches_planted_mistake(name, tmp_path):
-    old, new, test = MUTATIONS[name]
+    old, new, test, *nth = MUTATIONS[name]
     src = (ORACLE / "felis_oracle.c").read_text()
     assert old in src, f"{name}: the text to mutate is gone from felis_oracle.c"
-    (tmp_path / "felis_oracle.c").write_text(src.replace(old, new, 1))
+    (tmp_path / "felis_oracle.c").write_text(_replace_nth(src, old, new, nth[0] if nth else 0))
     (tmp_path / "felis_oracle.h").write_text((ORACLE / "felis_oracle.h").read_text())
     lib = _build(tmp_path)
     r = _run_pinning_test(lib, test)

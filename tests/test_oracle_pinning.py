@@ -435,7 +435,7 @@ def test_push_madey_low_gain_curve():
     (c = ks K2/gamma^3 from particlePush.cpp:366, a = ks K0 f |U| / (2 gamma)
     from :362-367) gives the Madey curve
       <dgamma> = -(a^2 c L^3/(2u)) (2(1 - cos u)/u^2 - sin u/u)
-                 - a^2 L^2 (1 - cos u)/(2 gamma u^2):
+                 - a^2 L^2 (1 - cos u)/(gamma u^2):
     energy loss above resonance (u > 0), gain below, antisymmetric shape. The
     oracle follows it to 1% of the peak (the rest is of relative order
     1/(2 ku L) from the coupling's 1/gamma, which the formula leaves out)."""
@@ -463,7 +463,7 @@ def test_push_madey_low_gain_curve():
     a = ks * K0 * (special.jv(0, xi) - special.jv(1, xi)) * U0 / (2 * gam)
     c = ks * K2 / gam ** 3
     want = (-(a * a * c * L ** 3 / (2 * us)) * (2 * (1 - np.cos(us)) / us ** 2 - np.sin(us) / us)
-            - a * a * L ** 2 * (1 - np.cos(us)) / (2 * gam * us * us))
+            - a * a * L ** 2 * (1 - np.cos(us)) / (gam * us * us))
     assert np.all(np.sign(dg) == -np.sign(us))
     np.testing.assert_allclose(dg, want, rtol=0, atol=0.01 * np.max(np.abs(want)))
 
