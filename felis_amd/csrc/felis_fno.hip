@@ -132,6 +132,63 @@ __global__ void __launch_bounds__(256) k_bgemm(int batch, int M, int N, int K, O
       }
 }
 
+// The y transforms of the fused inference path (width 32, 8 modes), one
+// workgroup per map, with the operands' layouts fixed at compile time:
+//   forward (INV false): Y = Gy (32 x 128) Xf (128 x 256), Xf [k][n]
+//     contiguous, Y written in the mode mixing's layout
+//     [(i>>1)*512 + (i&1)*32 + (n>>5)*64 + (n&31)];
+//   inverse (INV true): Xb = Giy (128 x 32) Zv (32 x 256), Zv read in that
+//     layout, Xb [i][n] contiguous.
+// The table goes to LDS as BF16 once per workgroup; each wave takes 32 x 32
+// output tiles in turn with the same 16 x 16 x 16 MFMA sequence per tile as
+// k_bgemm (k ascending, the same BF16 operands), so the results are k_bgemm's
+// bit for bit, without its general index arithmetic.
+template <bool INV>
+__global__ void __launch_bounds__(256) k_fno_ytr(int batch, const float *__restrict__ T,
+                                                 const uint16_t *__restrict__ Bm, uint16_t *__restrict__ Cm) {
+  constexpr int M = INV ? 128 : 32, K = INV ? 32 : 128, N = 256;
+  constexpr long long SB = INV ? 8192 : 32768, SC = INV ? 32768 : 8192;  // per map
+  __shared__ short At[M * K];
+  const int map = blockIdx.x;
+  if (map >= batch) return;
+  for (int q = threadIdx.x; q < M * K; q += 256) At[q] = f2bf(T[q]);
+  __syncthreads();
+  const uint16_t *B = Bm + map * SB;
+  uint16_t *Cp = Cm + map * SC;
+  auto boff = [](int k, int n) -> int {
+    return INV ? (k >> 1) * 512 + (k & 1) * 32 + (n >> 5) * 64 + (n & 31) : k * N + n;
+  };
+  auto coff = [](int i, int n) -> int {
+    return INV ? i * N + n : (i >> 1) * 512 + (i & 1) * 32 + (n >> 5) * 64 + (n & 31);
+  };
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, kq = (l >> 4) * 4;
+  constexpr int TN = N / 32, TILES = (M / 32) * TN;
+  for (int t = wave; t < TILES; t += 4) {
+    const int m0 = (t / TN) * 32, n0 = (t % TN) * 32;
+    v4f d[2][2] = {};
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 16) {
+      v4s a[2], b[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        a[u] = *reinterpret_cast<const v4s *>(&At[(m0 + 16 * u + r) * K + k0 + kq]);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) b[u][q] = (short)B[boff(k0 + kq + q, n0 + 16 * u + r)];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) d[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], d[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Cp[coff(m0 + 16 * i + kq + q, n0 + 16 * j + r)] = (uint16_t)f2bf(d[i][j][q]);
+  }
+}
+
 // GELU in its tanh form, with tanh(u) = 1 - 2 / (1 + e^{2u}) from the
 // hardware exponential and reciprocal (saturating to +-1 for large |u|)
 __device__ inline float tanh_fast(float u) { return 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + __expf(2.f * u)); }
@@ -620,18 +677,12 @@ void fno_forward_fused(felis_fno *n, int B, const float *X, const uint16_t *XB =
     hipLaunchKernelGGL((k_fno_row<32, 16, float>), dim3(nb), dim3(256), 0, st, rows, n->cin, X, n->mu, n->isd,
                        n->P + n->o_lw, n->P + n->o_lb, nullptr, nullptr, n->Fx, 0, hA, Xf);
   for (int l = 0; l < n->L; ++l) {
-    bgemm<float, uint16_t, uint16_t>(st, B, 4 * m, m * w, 2 * kN, Op{n->Gy, U1(0), U1(2 * kN), U1(1)},
-                                     Op{(const float *)Xf, U1((long long)kN * sX), U1((long long)m * w), U1(1)},
-                                     OpW{(float *)Yb, U1(sY), U2(1, (long long)m * 2 * w, w), U2(lw, 2LL * w, 1)}, 0,
-                                     nullptr);
+    hipLaunchKernelGGL((k_fno_ytr<false>), dim3(B), dim3(256), 0, st, B, n->Gy, Xf, Yb);
     bgemm<uint16_t, float, uint16_t>(st, 2 * m * m, B, 2 * w, 2 * w,
                                      Op{(const float *)Yb, U1(2LL * w), U1(sY), U1(1)},
                                      Op{n->P + n->o_Wm[l], U1(4LL * w * w), U1(2LL * w), U1(1)},
                                      OpW{(float *)Zb, U1(2LL * w), U1(sY), U1(1)}, 0, nullptr);
-    bgemm<float, uint16_t, uint16_t>(st, B, 2 * kN, m * w, 4 * m, Op{n->Giy, U1(0), U1(4 * m), U1(1)},
-                                     Op{(const float *)Zb, U1(sY), U2(1, (long long)m * 2 * w, w), U2(lw, 2LL * w, 1)},
-                                     OpW{(float *)Xb, U1((long long)kN * sX), U1((long long)m * w), U1(1)}, 0,
-                                     nullptr);
+    hipLaunchKernelGGL((k_fno_ytr<true>), dim3(B), dim3(256), 0, st, B, n->Giy, Zb, Xb);
     const bool last = l == n->L - 1;
     hipLaunchKernelGGL((k_fno_row<32, 16, uint16_t>), dim3(nb), dim3(256), 0, st, rows, w, hA, nullptr, nullptr,
                        n->P + n->o_W[l], n->P + n->o_b[l], n->Fxi, Xb, n->Fx, last ? 0 : 1, hB,
