@@ -189,6 +189,47 @@ __global__ void __launch_bounds__(256) k_fno_ytr(int batch, const float *__restr
   }
 }
 
+// The mode mixing of the fused inference path (width 32, 8 modes: 128
+// modes, 64 x 64 real matrices): Z[map][md*64 + n] = sum_k Y[map][md*64 + k]
+// Wm[md][k][n], maps sY = 8192 apart. One workgroup per (mode, 64 maps): the
+// mode's matrix goes to LDS as BF16, transposed so that a lane's four k of
+// one column are one 8-byte read; each wave forms 16 maps x 64 columns with
+// k ascending per 16 x 16 block, as k_bgemm does, so the results are its
+// bit for bit.
+__global__ void __launch_bounds__(256) k_fno_mix(int batch, const uint16_t *__restrict__ Y,
+                                                 const float *__restrict__ Wm, uint16_t *__restrict__ Z) {
+  constexpr int W2 = 64, SY = 8192, LDW = W2 + 4;  // padded rows: a 16-lane read spreads over the banks
+  __shared__ short Wt[W2 * LDW];  // [n][k]
+  const int md = blockIdx.y, map0 = blockIdx.x * 64;
+  const float *wm = Wm + (long long)md * W2 * W2;
+  for (int q = threadIdx.x; q < W2 * W2; q += 256) {
+    const int k = q / W2, n = q % W2;
+    Wt[n * LDW + k] = f2bf(wm[q]);
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, kq = (l >> 4) * 4;
+  const int mp = map0 + 16 * wave + r;  // this lane's map for the A operand
+  const bool va = mp < batch;
+  const uint16_t *ya = Y + (long long)(va ? mp : 0) * SY + md * W2;
+  v4f d[4] = {};
+#pragma unroll
+  for (int k0 = 0; k0 < W2; k0 += 16) {
+    const v4s a = va ? *reinterpret_cast<const v4s *>(ya + k0 + kq) : v4s{0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const v4s b = *reinterpret_cast<const v4s *>(&Wt[(16 * j + r) * LDW + k0 + kq]);
+      d[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, d[j], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int m = map0 + 16 * wave + kq + q;
+      if (m < batch) Z[(long long)m * SY + md * W2 + 16 * j + r] = (uint16_t)f2bf(d[j][q]);
+    }
+}
+
 // GELU in its tanh form, with tanh(u) = 1 - 2 / (1 + e^{2u}) from the
 // hardware exponential and reciprocal (saturating to +-1 for large |u|)
 __device__ inline float tanh_fast(float u) { return 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + __expf(2.f * u)); }
@@ -678,10 +719,7 @@ void fno_forward_fused(felis_fno *n, int B, const float *X, const uint16_t *XB =
                        n->P + n->o_lw, n->P + n->o_lb, nullptr, nullptr, n->Fx, 0, hA, Xf);
   for (int l = 0; l < n->L; ++l) {
     hipLaunchKernelGGL((k_fno_ytr<false>), dim3(B), dim3(256), 0, st, B, n->Gy, Xf, Yb);
-    bgemm<uint16_t, float, uint16_t>(st, 2 * m * m, B, 2 * w, 2 * w,
-                                     Op{(const float *)Yb, U1(2LL * w), U1(sY), U1(1)},
-                                     Op{n->P + n->o_Wm[l], U1(4LL * w * w), U1(2LL * w), U1(1)},
-                                     OpW{(float *)Zb, U1(2LL * w), U1(sY), U1(1)}, 0, nullptr);
+    hipLaunchKernelGGL(k_fno_mix, dim3((B + 63) / 64, 2 * m * m), dim3(256), 0, st, B, Yb, n->P + n->o_Wm[l], Zb);
     hipLaunchKernelGGL((k_fno_ytr<true>), dim3(B), dim3(256), 0, st, B, n->Giy, Zb, Xb);
     const bool last = l == n->L - 1;
     hipLaunchKernelGGL((k_fno_row<32, 16, uint16_t>), dim3(nb), dim3(256), 0, st, rows, w, hA, nullptr, nullptr,
