@@ -327,7 +327,8 @@ def _bessel_interp(st, H, xi):
 
 
 def _couplings(st, lat, H, xi, xi2, B, Phi):
-    """fx, fy of particlePush.cpp:340-360 (Appendix C of the survey)."""
+    """fx, fy of particlePush.cpp:340-360: the odd harmonics' Bessel couplings
+    J1 -+ K0_alpha J2 (sign by (H-1)/2), the even ones' e^{+-iB} sums of J1, J2, J3."""
     J = _bessel_interp(st, H, xi)
     a = lat.K0_alpha
     if H % 2:
