@@ -1,4 +1,4 @@
-// Host helpers shared by the deck, loader and driver translation units.
+// Host helpers shared by the deck, loader and run-loop translation units.
 #pragma once
 
 #include "felis.h"

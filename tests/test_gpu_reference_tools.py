@@ -1,4 +1,4 @@
-"""The reference's own post-processing tools read this build's dumps.
+"""The reference's own post-processing tools read this library's dumps.
 
 diag/felField.cpp and diag/felParticle.cpp need only serial HDF5, so
 `make -C oracle ref` compiles them from the reference's sources against the
