@@ -400,7 +400,10 @@ __global__ void __launch_bounds__(256) k_fno_row(long long rows, int kin, const 
 #pragma unroll
       for (int j = 0; j < NB; ++j) d[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], d[i][j], 0, 0, 0);
   }
-  uint16_t *o = out + rr * kN * W;
+  // the row's outputs go through this wave's LDS tile and leave in 16-byte
+  // stores (a lane's values are 2-byte pieces of 16 rows)
+  __shared__ uint4 tile_o[4][kN * W * 2 / 16], tile_x[4][M2 * W * 2 / 16];
+  uint16_t *so = reinterpret_cast<uint16_t *>(tile_o[wave]);
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -412,9 +415,19 @@ __global__ void __launch_bounds__(256) k_fno_row(long long rows, int kin, const 
         float v = d[i][j][q] + bc;
         if (act) v = gelu(v);
         d[i][j][q] = v;
-        o[(16 * i + kq + q) * W + c] = (uint16_t)f2bf(v);
+        so[(16 * i + kq + q) * W + c] = (uint16_t)f2bf(v);
       }
     }
+  // the wave's LDS writes complete before its lanes read each other's
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  {
+    uint4 *og = reinterpret_cast<uint4 *>(out + rr * kN * W);
+    constexpr int NV = kN * W * 2 / 16;  // 16-byte pieces of the row
+#pragma unroll
+    for (int v = l; v < NV; v += 64) og[v] = tile_o[wave][v];
+  }
   if (Xf_next) {
     v4f e[NB] = {};
 #pragma unroll
@@ -429,11 +442,18 @@ __global__ void __launch_bounds__(256) k_fno_row(long long rows, int kin, const 
 #pragma unroll
       for (int j = 0; j < NB; ++j) e[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[j], e[j], 0, 0, 0);
     }
-    uint16_t *xf = Xf_next + rr * M2 * W;
+    uint16_t *sx = reinterpret_cast<uint16_t *>(tile_x[wave]);
 #pragma unroll
     for (int j = 0; j < NB; ++j)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) xf[(kq + q) * W + 16 * j + r] = (uint16_t)f2bf(e[j][q]);
+      for (int q = 0; q < 4; ++q) sx[(kq + q) * W + 16 * j + r] = (uint16_t)f2bf(e[j][q]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    uint4 *xg = reinterpret_cast<uint4 *>(Xf_next + rr * M2 * W);
+    constexpr int NX = M2 * W * 2 / 16;
+#pragma unroll
+    for (int v = l; v < NX; v += 64) xg[v] = tile_x[wave][v];
   }
 }
 
