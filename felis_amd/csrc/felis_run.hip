@@ -858,11 +858,14 @@ int run_mixed(felis_ctx *ctx, const felis_lattice *lat_tab, int it0, int n_steps
     const int kn = k + (jump ? J : 1);  // the next iteration
     const felis_lattice *ln = jump ? &lat_tab[2 * kn] : l1;
     // the next solveField's deposit, with the lattice it will run under,
-    // after the push (pipelined with it over slice chunks when tuned so)
+    // after the push (pipelined with it over slice chunks when tuned so);
+    // none after the call's last step: no step of this call reads it, and a
+    // later call deposits the state it is handed before its first solve
     DepArgs dep{};
-    if (ln->current_flag && (err = dep_args(ctx, ln, &dep))) break;
+    const bool deposit_next = ln->current_flag && kn < n_steps;
+    if (deposit_next && (err = dep_args(ctx, ln, &dep))) break;
     if ((err = particles_and_deposit(ctx, a, tile_red, jump ? ctx->Ub : ctx->U, jump ? J : 1.0, ln,
-                                     ln->current_flag ? &dep : nullptr, dep_lds)))
+                                     deposit_next ? &dep : nullptr, dep_lds)))
       break;
     if (jump) {
       // the learned emission correction, before the field slips
