@@ -3,8 +3,8 @@
  * the FELIS per-undulator-step slice kernel.
  *
  * Each entry point replaces one free function of the reference's operator
- * surface (mesh.h:150-178), called by the driver loop in the same order
- * (main.cpp:92-156). The reference functions mutate a Domain; here a context
+ * surface (mesh.h:150-178), called by the reference's main loop in the same
+ * order (main.cpp:92-156). The reference functions mutate a Domain; here a context
  * owns the equivalent device state, and the caller passes the lattice state
  * that the reference reads from the Domain (K0, ue, K0_alpha, ku, g,
  * undType, currentFlag, driftFlag: updateK_quadG.cpp:8-94).
